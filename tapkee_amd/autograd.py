@@ -1,0 +1,82 @@
+"""KL(P||Q) as a torch.autograd.Function on ROCm tensors.
+
+forward  : KL(P||Q) = sum_ij p_ij log(p_ij / q_ij) over the CSR pattern of P,
+           q_ij = (1 + |y_i - y_j|^2)^-1 / Z with Z recomputed on the device
+           (exact FP64 pair sum at theta == 0, Barnes-Hut estimate at theta > 0).
+backward : dKL/dY = 4 * (F_attr - F_rep / Z), computed by the engine's force
+           kernels (the gradient tapkee returns, tsne.hpp:325-407, times 4).
+Nothing of size N^2 is stored: Y, P and theta are saved, Z is recomputed.
+
+`TSNELoss` wraps it for a fixed P built from data by the engine, so any
+torch optimiser or a parametric map X -> Y can be driven by it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import build_p as _build_p
+
+
+def _stream(t: torch.Tensor) -> int:
+    return torch.cuda.current_stream(t.device).cuda_stream
+
+
+def _check_inputs(Y, row_ptr, col, val):
+    if not Y.is_cuda:
+        raise _lib.DeviceError("KLDivergence runs on the GPU: Y must be a device tensor")
+    if Y.dtype != torch.float32 or val.dtype != torch.float32:
+        raise _lib.WrongParameterError("Y and P values must be float32")
+    if row_ptr.dtype != torch.int32 or col.dtype != torch.int32:
+        raise _lib.WrongParameterError("row_ptr and col must be int32")
+    if Y.dim() != 2 or row_ptr.numel() != Y.shape[0] + 1:
+        raise _lib.WrongParameterError("row_ptr must have N+1 entries for Y of shape (N, d)")
+
+
+class KLDivergence(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, Y, row_ptr, col, val, theta: float = 0.0):
+        _check_inputs(Y, row_ptr, col, val)
+        Y = Y.contiguous()
+        kl = C.c_double(0.0)
+        z = C.c_double(0.0)
+        N, d = Y.shape
+        _lib.check(_lib.lib().tka_kl_dev(Y.data_ptr(), N, d, row_ptr.data_ptr(), col.data_ptr(), val.data_ptr(),
+                                         float(theta), C.byref(kl), C.byref(z), _stream(Y)))
+        ctx.save_for_backward(Y, row_ptr, col, val)
+        ctx.theta = float(theta)
+        return torch.tensor(kl.value, dtype=Y.dtype, device=Y.device)
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        Y, row_ptr, col, val = ctx.saved_tensors
+        N, d = Y.shape
+        dY = torch.empty_like(Y)
+        z = C.c_double(0.0)
+        _lib.check(_lib.lib().tka_gradient_dev(Y.data_ptr(), N, d, row_ptr.data_ptr(), col.data_ptr(),
+                                               val.data_ptr(), ctx.theta, dY.data_ptr(), C.byref(z),
+                                               _stream(Y)))
+        return 4.0 * dY * grad_out, None, None, None, None
+
+
+def kl_divergence(Y, row_ptr, col, val, theta: float = 0.0):
+    return KLDivergence.apply(Y, row_ptr, col, val, theta)
+
+
+class TSNELoss(torch.nn.Module):
+    """KL(P||Q) for the P of data X (N x D), built on the GPU by the engine."""
+
+    def __init__(self, X, perplexity: float = 30.0, theta: float = 0.0, dist_dtype: str = "bf16",
+                 device="cuda"):
+        super().__init__()
+        row, col, val = _build_p(np.asarray(X, dtype=np.float64), perplexity, dist_dtype=dist_dtype)
+        self.theta = float(theta)
+        self.register_buffer("row_ptr", torch.from_numpy(row).to(device))
+        self.register_buffer("col", torch.from_numpy(col).to(device))
+        self.register_buffer("val", torch.from_numpy(val.astype(np.float32)).to(device))
+
+    def forward(self, Y):
+        return KLDivergence.apply(Y, self.row_ptr, self.col, self.val, self.theta)

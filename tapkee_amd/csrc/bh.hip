@@ -1,0 +1,573 @@
+// Barnes-Hut quadtree from Morton keys, and the per-point traversal.
+//
+// Tree semantics follow quadtree.hpp: the root is the rectangle centred on
+// mean(Y) with half-widths max(max - mean, mean - min) + 1e-5 per axis
+// (:104-129); a cell splits into NW, NE, SW, SE quadrants with points on a
+// split line going west/north (closed intervals tried in that order, :53-64,
+// :241-248); leaves hold one point; a cell is used as a summary when it is a
+// leaf or max(hh, hw)/sqrt(D) < theta (:335).
+//
+// Construction (all integer, bit-exact):
+//   keys   point -> 16+16 bit Morton key of its leaf cell, y bit above x bit,
+//          so digit order 0..3 is NW, NE, SW, SE and sorting keys gives the
+//          reference's depth-first child order.  Quantisation is
+//          ceil(t * 2^16) - 1, so coordinates on a split line go west/north.
+//   sort   stable LSD radix sort of (key, point).
+//   pairs  adjacent sorted keys i, i+1 first differ at quadtree level
+//          l = clz(k_i ^ k_i+1)/2; the level-l cell holding both has >= 2
+//          non-empty children, i.e. it is a node of the compressed tree.
+//          The pair whose keys straddle the end of that cell's first child
+//          owns the node; galloping searches give the cell's key range [a,b).
+//          Chains of one-child cells are skipped: such a cell has the same
+//          count and centre of mass as its child and a larger size, so the
+//          reference accepts it only if it would accept the child.
+//   scan   pre-order positions: the internal nodes whose ranges start at key
+//          j (a chain, shallowest first) precede leaf j, so with
+//          c_j = popcount(mask_j) + 1, node (a, l) sits at P[a] + #levels < l
+//          in mask_a, leaf j at P[j] + popcount(mask_j), and a node's subtree
+//          ends at P[b] (its skip pointer).  Centres of mass come from FP64
+//          prefix sums of the sorted coordinates.
+//   Equal keys (points closer than 2^-16 of the box) form one level-16 node
+//   whose children are the individual points.
+//
+// Traversal is stackless: visit node t; if it is a leaf, interact unless it
+// is the point itself and go to t+1; if accepted, interact and jump to its
+// skip pointer; otherwise descend to t+1.
+#include "bh.hpp"
+
+#include "radix_sort.hpp"
+
+namespace tka {
+namespace {
+
+constexpr int kB = kMortonBits;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = 256 * kScanItems;
+
+// ---------------------------------------------------------------- bbox
+template <class T>
+__global__ __launch_bounds__(kBoxBlock) void bbox_part(const T* __restrict__ Y, int N, double* __restrict__ part)
+{
+    __shared__ double sh[6][4];
+    const int i = blockIdx.x * kBoxBlock + threadIdx.x;
+    double v[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    if (i < N) {
+        const double x = (double)Y[2 * (size_t)i], y = (double)Y[2 * (size_t)i + 1];
+        v[0] = x;
+        v[1] = y;
+        v[2] = v[4] = x;
+        v[3] = v[5] = y;
+    }
+    v[0] = wave_sum(v[0]);
+    v[1] = wave_sum(v[1]);
+    v[2] = wave_min(v[2]);
+    v[3] = wave_min(v[3]);
+    v[4] = wave_max(v[4]);
+    v[5] = wave_max(v[5]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+        for (int ww = 0; ww < kBoxBlock / 64; ww++) {
+            r[0] += sh[0][ww];
+            r[1] += sh[1][ww];
+            r[2] = fmin(r[2], sh[2][ww]);
+            r[3] = fmin(r[3], sh[3][ww]);
+            r[4] = fmax(r[4], sh[4][ww]);
+            r[5] = fmax(r[5], sh[5][ww]);
+        }
+        double* o = part + (size_t)kBoxStride * blockIdx.x;
+        o[0] = r[0];
+        o[1] = r[1];
+        o[2] = 0.0;
+        o[3] = r[2];
+        o[4] = r[3];
+        o[5] = r[4];
+        o[6] = r[5];
+        o[7] = 0.0;
+    }
+}
+
+// Reduces the box partials (every block does it redundantly, in the same
+// fixed order, so all blocks agree bit for bit).  out6 = mean x, mean y,
+// min x, min y, max x, max y.
+TKA_DEV void reduce_box(const double* __restrict__ part, int nb, int N, double* out6)
+{
+    __shared__ double sh[6][4];
+    double v[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        const double* p = part + (size_t)kBoxStride * b;
+        v[0] += p[0];
+        v[1] += p[1];
+        v[2] = fmin(v[2], p[3]);
+        v[3] = fmin(v[3], p[4]);
+        v[4] = fmax(v[4], p[5]);
+        v[5] = fmax(v[5], p[6]);
+    }
+    v[0] = wave_sum(v[0]);
+    v[1] = wave_sum(v[1]);
+    v[2] = wave_min(v[2]);
+    v[3] = wave_min(v[3]);
+    v[4] = wave_max(v[4]);
+    v[5] = wave_max(v[5]);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) sh[k][w] = v[k];
+    __syncthreads();
+    double r[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int ww = 0; ww < nw; ww++) {
+        r[0] += sh[0][ww];
+        r[1] += sh[1][ww];
+        r[2] = fmin(r[2], sh[2][ww]);
+        r[3] = fmin(r[3], sh[3][ww]);
+        r[4] = fmax(r[4], sh[4][ww]);
+        r[5] = fmax(r[5], sh[5][ww]);
+    }
+    out6[0] = r[0] / (double)N;
+    out6[1] = r[1] / (double)N;
+    for (int k = 2; k < 6; k++) out6[k] = r[k];
+}
+
+// ---------------------------------------------------------------- keys
+TKA_DEV uint32_t spread16(uint32_t v)
+{
+    v &= 0xffffu;
+    v = (v | (v << 8)) & 0x00ff00ffu;
+    v = (v | (v << 4)) & 0x0f0f0f0fu;
+    v = (v | (v << 2)) & 0x33333333u;
+    v = (v | (v << 1)) & 0x55555555u;
+    return v;
+}
+
+// Quantise t in (0, 1) to ceil(t * 2^16) - 1 in [0, 2^16 - 1].
+TKA_DEV uint32_t quantise(double t)
+{
+    double c = ceil(ldexp(t, kB)) - 1.0;
+    c = c < 0.0 ? 0.0 : (c > 65535.0 ? 65535.0 : c);
+    return (uint32_t)c;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void bh_keys(T* __restrict__ Y, int N, const double* __restrict__ box_part, int nb,
+                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ perm,
+                                               uint32_t* __restrict__ mask, RootBox* __restrict__ root)
+{
+    __shared__ double box[6];
+    double r6[6];
+    reduce_box(box_part, nb, N, r6);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) box[k] = r6[k];
+    __syncthreads();
+    const double mx = box[0], my = box[1];
+    // Root cell of the centred points (centre 0): quadtree.hpp:128-129.
+    const double hw = fmax(box[4] - mx, mx - box[2]) + 1e-5;
+    const double hh = fmax(box[5] - my, my - box[3]) + 1e-5;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        root->mean_x = mx;
+        root->mean_y = my;
+        root->hw = hw;
+        root->hh = hh;
+        root->size0 = fmax(hw, hh);
+    }
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const T x = (T)((double)Y[2 * (size_t)i] - mx);
+    const T y = (T)((double)Y[2 * (size_t)i + 1] - my);
+    Y[2 * (size_t)i] = x;
+    Y[2 * (size_t)i + 1] = y;
+    const uint32_t qx = quantise(((double)x + hw) / (2.0 * hw));
+    const uint32_t qy = quantise(((double)y + hh) / (2.0 * hh));
+    keys[i] = (spread16(qy) << 1) | spread16(qx);
+    perm[i] = (uint32_t)i;
+    mask[i] = 0u;
+}
+
+// ---------------------------------------------------------------- pairs
+TKA_DEV int gallop_left(const uint32_t* __restrict__ k, int i, uint32_t lo)
+{
+    // smallest a <= i with k[a] >= lo (k[i] >= lo)
+    int hi = i, lw = -1, step = 1;
+    while (true) {
+        const int probe = i - step;
+        if (probe < 0) break;
+        if (k[probe] < lo) {
+            lw = probe;
+            break;
+        }
+        hi = probe;
+        step <<= 1;
+    }
+    while (hi - lw > 1) {
+        const int mid = (hi + lw) >> 1;
+        if (k[mid] >= lo)
+            hi = mid;
+        else
+            lw = mid;
+    }
+    return hi;
+}
+
+TKA_DEV int gallop_right(const uint32_t* __restrict__ k, int N, int i, uint32_t hi_key)
+{
+    // smallest b > i with k[b] > hi_key, or N (k[i] <= hi_key)
+    int lw = i, hi = N, step = 1;
+    while (true) {
+        const int probe = i + step;
+        if (probe >= N) break;
+        if (k[probe] > hi_key) {
+            hi = probe;
+            break;
+        }
+        lw = probe;
+        step <<= 1;
+    }
+    while (hi - lw > 1) {
+        const int mid = (hi + lw) >> 1;
+        if (k[mid] <= hi_key)
+            lw = mid;
+        else
+            hi = mid;
+    }
+    return hi;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void bh_pairs(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ perm,
+                                                const T* __restrict__ Y, int N, uint32_t* __restrict__ mask,
+                                                int* __restrict__ node_a, int* __restrict__ node_b,
+                                                int* __restrict__ node_lvl,
+                                                typename Vec2<T>::type* __restrict__ ys)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    {
+        const uint32_t p = perm[i];
+        typename Vec2<T>::type v;
+        v.x = Y[2 * (size_t)p];
+        v.y = Y[2 * (size_t)p + 1];
+        ys[i] = v;
+    }
+    if (i >= N - 1) return;
+    const uint32_t ki = keys[i], kj = keys[i + 1];
+    const uint32_t x = ki ^ kj;
+    int lvl, a, b;
+    bool owner;
+    if (x == 0u) {
+        lvl = kB;
+        owner = (i == 0) || (keys[i - 1] != ki);
+        a = i;
+        b = owner ? gallop_right(keys, N, i + 1, ki) : 0;
+    } else {
+        lvl = __clz(x) >> 1;
+        const int lowbits = 2 * (kB - lvl);  // 2..32
+        const uint32_t lowmask = (lowbits >= 32) ? 0xffffffffu : ((1u << lowbits) - 1u);
+        const uint32_t lo = ki & ~lowmask, hi = ki | lowmask;
+        a = gallop_left(keys, i, lo);
+        const int sh = lowbits - 2;
+        owner = (ki >> sh) == (keys[a] >> sh);
+        b = owner ? gallop_right(keys, N, i + 1, hi) : 0;
+    }
+    if (owner) {
+        atomicOr(&mask[a], 1u << lvl);
+        node_a[i] = a;
+        node_b[i] = b;
+        node_lvl[i] = lvl;
+    } else {
+        node_lvl[i] = -1;
+    }
+}
+
+// ---------------------------------------------------------------- scan
+template <class T>
+__global__ __launch_bounds__(256) void bh_scan_part(const uint32_t* __restrict__ mask,
+                                                    const typename Vec2<T>::type* __restrict__ ys, int N,
+                                                    double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    double c = 0.0, sx = 0.0, sy = 0.0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int j = base + k;
+        if (j < N) {
+            c += (double)(__popc(mask[j]) + 1);
+            sx += (double)ys[j].x;
+            sy += (double)ys[j].y;
+        }
+    }
+    c = block_sum(c, sh);
+    sx = block_sum(sx, sh);
+    sy = block_sum(sy, sh);
+    if (threadIdx.x == 0) {
+        part[3 * (size_t)blockIdx.x] = c;
+        part[3 * (size_t)blockIdx.x + 1] = sx;
+        part[3 * (size_t)blockIdx.x + 2] = sy;
+    }
+}
+
+TKA_DEV double wave_incl_scan_d(double v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const double t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void bh_scan_write(const uint32_t* __restrict__ mask,
+                                                     const typename Vec2<T>::type* __restrict__ ys, int N,
+                                                     const double* __restrict__ part, uint32_t* __restrict__ pre,
+                                                     double* __restrict__ psx, double* __restrict__ psy)
+{
+    __shared__ double sh[4];
+    __shared__ double wsum[3][4];
+    // prefix of earlier tiles
+    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    for (int b = threadIdx.x; b < (int)blockIdx.x; b += 256) {
+        b0 += part[3 * (size_t)b];
+        b1 += part[3 * (size_t)b + 1];
+        b2 += part[3 * (size_t)b + 2];
+    }
+    b0 = block_sum(b0, sh);
+    b1 = block_sum(b1, sh);
+    b2 = block_sum(b2, sh);
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t c[kScanItems];
+    double x[kScanItems], y[kScanItems];
+    double tc = 0.0, tx = 0.0, ty = 0.0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int j = base + k;
+        const bool ok = j < N;
+        c[k] = ok ? (uint32_t)(__popc(mask[j]) + 1) : 0u;
+        x[k] = ok ? (double)ys[j].x : 0.0;
+        y[k] = ok ? (double)ys[j].y : 0.0;
+        tc += (double)c[k];
+        tx += x[k];
+        ty += y[k];
+    }
+    const double ic = wave_incl_scan_d(tc), ix = wave_incl_scan_d(tx), iy = wave_incl_scan_d(ty);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 63) {
+        wsum[0][w] = ic;
+        wsum[1][w] = ix;
+        wsum[2][w] = iy;
+    }
+    __syncthreads();
+    double oc = b0, ox = b1, oy = b2;
+    for (int ww = 0; ww < w; ww++) {
+        oc += wsum[0][ww];
+        ox += wsum[1][ww];
+        oy += wsum[2][ww];
+    }
+    double rc = oc + ic - tc, rx = ox + ix - tx, ry = oy + iy - ty;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+        const int j = base + k;
+        if (j < N) {
+            pre[j] = (uint32_t)rc;
+            psx[j] = rx;
+            psy[j] = ry;
+        }
+        rc += (double)c[k];
+        rx += x[k];
+        ry += y[k];
+        if (j == N - 1) {
+            pre[N] = (uint32_t)rc;
+            psx[N] = rx;
+            psy[N] = ry;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- emit
+template <class T>
+__global__ __launch_bounds__(256) void bh_emit(const uint32_t* __restrict__ mask, const int* __restrict__ node_a,
+                                               const int* __restrict__ node_b, const int* __restrict__ node_lvl,
+                                               const typename Vec2<T>::type* __restrict__ ys,
+                                               const uint32_t* __restrict__ perm, int N,
+                                               const uint32_t* __restrict__ pre, const double* __restrict__ psx,
+                                               const double* __restrict__ psy, const RootBox* __restrict__ root,
+                                               typename Vec4<T>::type* __restrict__ nodes, int* __restrict__ aux)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= N) return;
+    {  // leaf j
+        const uint32_t m = mask[j];
+        const uint32_t t = pre[j] + (uint32_t)__popc(m);
+        typename Vec4<T>::type nd;
+        nd.x = ys[j].x;
+        nd.y = ys[j].y;
+        nd.z = (T)1;
+        nd.w = (T)-1;
+        nodes[t] = nd;
+        aux[t] = (int)perm[j];
+    }
+    if (j < N - 1) {
+        const int lvl = node_lvl[j];
+        if (lvl >= 0) {
+            const int a = node_a[j], b = node_b[j];
+            const uint32_t m = mask[a];
+            const uint32_t t = pre[a] + (uint32_t)__popc(m & ((1u << lvl) - 1u));
+            const double cnt = (double)(b - a);
+            const double size = ldexp(root->size0, -lvl);
+            typename Vec4<T>::type nd;
+            nd.x = (T)((psx[b] - psx[a]) / cnt);
+            nd.y = (T)((psy[b] - psy[a]) / cnt);
+            nd.z = (T)cnt;
+            nd.w = (T)(size * size);
+            nodes[t] = nd;
+            aux[t] = (int)pre[b];
+        }
+    }
+}
+
+// ---------------------------------------------------------------- traverse
+template <class T>
+__global__ __launch_bounds__(256) void bh_traverse(const typename Vec4<T>::type* __restrict__ nodes,
+                                                   const int* __restrict__ aux, const uint32_t* __restrict__ pre,
+                                                   const typename Vec2<T>::type* __restrict__ ys,
+                                                   const uint32_t* __restrict__ perm, int N, T theta2,
+                                                   T* __restrict__ neg, double* __restrict__ zpart,
+                                                   unsigned long long* __restrict__ interactions)
+{
+    __shared__ double sh[4];
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    double zd = 0.0;
+    unsigned cnt = 0;
+    if (j < N) {
+        const int total = (int)pre[N];
+        const int p = (int)perm[j];
+        const T px = ys[j].x, py = ys[j].y;
+        T fx = 0, fy = 0, z = 0;
+        int t = 0;
+        while (t < total) {
+            const typename Vec4<T>::type nd = nodes[t];
+            const int ax = aux[t];
+            const T dx = px - nd.x, dy = py - nd.y;
+            const T D = dx * dx + dy * dy;
+            if (nd.w < (T)0) {
+                if (ax != p) {
+                    const T q = (T)1 / ((T)1 + D);
+                    z += q;
+                    const T q2 = q * q;
+                    fx += q2 * dx;
+                    fy += q2 * dy;
+                    cnt++;
+                }
+                t++;
+            } else if (nd.w < theta2 * D) {
+                const T q = (T)1 / ((T)1 + D);
+                const T cq = nd.z * q;
+                z += cq;
+                const T m = cq * q;
+                fx += m * dx;
+                fy += m * dy;
+                cnt++;
+                t = ax > t ? ax : t + 1;  // skip pointers point forward; never loop
+            } else {
+                t++;
+            }
+        }
+        neg[2 * (size_t)p] = fx;
+        neg[2 * (size_t)p + 1] = fy;
+        zd = (double)z;
+    }
+    zd = block_sum(zd, sh);
+    if (threadIdx.x == 0) zpart[blockIdx.x] = zd;
+    if (interactions != nullptr) {
+        const double c = block_sum((double)cnt, sh);
+        if (threadIdx.x == 0) atomicAdd(interactions, (unsigned long long)c);
+    }
+}
+
+}  // namespace
+
+template <class T>
+void BhTree<T>::alloc(int n, MemBudget* mb)
+{
+    N = n;
+    keys.alloc(n, mb);
+    keys_alt.alloc(n, mb);
+    perm.alloc(n, mb);
+    perm_alt.alloc(n, mb);
+    sort_scratch.alloc(radix_sort_scratch_words(n), mb);
+    mask.alloc(n, mb);
+    node_a.alloc(n, mb);
+    node_b.alloc(n, mb);
+    node_lvl.alloc(n, mb);
+    ys.alloc(n, mb);
+    pre.alloc(n + 1, mb);
+    sx.alloc(n + 1, mb);
+    sy.alloc(n + 1, mb);
+    part.alloc(3 * (size_t)ceil_div(n, kScanTile), mb);
+    nodes.alloc(2 * (size_t)n, mb);
+    aux.alloc(2 * (size_t)n, mb);
+    root.alloc(1, mb);
+}
+
+template <class T>
+void bbox_partials_device(const T* Y, int N, double* part, hipStream_t s)
+{
+    hipLaunchKernelGGL(bbox_part<T>, dim3(bbox_blocks(N)), dim3(kBoxBlock), 0, s, Y, N, part);
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T>
+void bh_build(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s)
+{
+    require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "tree storage too small");
+    const unsigned g = ceil_div(N, 256);
+    hipLaunchKernelGGL(bh_keys<T>, dim3(g), dim3(256), 0, s, Y, N, box_part, nbox, tr.keys.get(),
+                       tr.perm.get(), tr.mask.get(), tr.root.get());
+    TKA_LAUNCH_CHECK();
+    const bool alt = radix_sort_pairs<uint32_t, uint32_t>(tr.keys.get(), tr.perm.get(), tr.keys_alt.get(),
+                                                          tr.perm_alt.get(), N, 2 * kB, tr.sort_scratch.get(), s);
+    const uint32_t* k = alt ? tr.keys_alt.get() : tr.keys.get();
+    const uint32_t* p = alt ? tr.perm_alt.get() : tr.perm.get();
+    tr.sorted_perm = p;
+    hipLaunchKernelGGL(bh_pairs<T>, dim3(g), dim3(256), 0, s, k, p, Y, N, tr.mask.get(), tr.node_a.get(),
+                       tr.node_b.get(), tr.node_lvl.get(), tr.ys.get());
+    TKA_LAUNCH_CHECK();
+    const unsigned gs = ceil_div(N, kScanTile);
+    hipLaunchKernelGGL(bh_scan_part<T>, dim3(gs), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N, tr.part.get());
+    TKA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bh_scan_write<T>, dim3(gs), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N, tr.part.get(),
+                       tr.pre.get(), tr.sx.get(), tr.sy.get());
+    TKA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bh_emit<T>, dim3(g), dim3(256), 0, s, tr.mask.get(), tr.node_a.get(), tr.node_b.get(),
+                       tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(), tr.root.get(),
+                       tr.nodes.get(), tr.aux.get());
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T>
+int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* zpart,
+                 unsigned long long* interactions, hipStream_t s)
+{
+    (void)Y;
+    const int g = bh_zblocks(tr.N);
+    hipLaunchKernelGGL(bh_traverse<T>, dim3(g), dim3(256), 0, s, tr.nodes.get(), tr.aux.get(), tr.pre.get(),
+                       tr.ys.get(), tr.sorted_perm, tr.N, (T)(theta * theta), neg, zpart, interactions);
+    TKA_LAUNCH_CHECK();
+    return g;
+}
+
+template struct BhTree<float>;
+template struct BhTree<double>;
+template void bbox_partials_device<float>(const float*, int, double*, hipStream_t);
+template void bbox_partials_device<double>(const double*, int, double*, hipStream_t);
+template void bh_build<float>(float*, int, const double*, int, BhTree<float>&, hipStream_t);
+template void bh_build<double>(double*, int, const double*, int, BhTree<double>&, hipStream_t);
+template int bh_repulsion<float>(const BhTree<float>&, const float*, double, float*, double*, unsigned long long*,
+                                 hipStream_t);
+template int bh_repulsion<double>(const BhTree<double>&, const double*, double, double*, double*,
+                                  unsigned long long*, hipStream_t);
+
+}  // namespace tka

@@ -1,0 +1,168 @@
+// Shared helpers for the MI355X (gfx950) t-SNE engine: error mapping, device
+// buffers, wave64 reductions.  Host and device code are compiled by hipcc.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <utility>
+
+#include "../../include/tapkee_amd.h"
+
+namespace tka {
+
+// Exception carrying a tka_status; the C-ABI layer converts it to a return
+// code plus tka_last_error() text.
+struct Error : std::runtime_error {
+    tka_status status;
+    Error(tka_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+[[noreturn]] inline void throw_hip(hipError_t e, const char* what, const char* file, int line)
+{
+    const tka_status s = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? TKA_E_NOT_ENOUGH_MEMORY
+                                                                                      : TKA_E_DEVICE;
+    throw Error(s, std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")");
+}
+
+#define TKA_HIP(x)                                                          \
+    do {                                                                    \
+        hipError_t tka_e_ = (x);                                            \
+        if (tka_e_ != hipSuccess) ::tka::throw_hip(tka_e_, #x, __FILE__, __LINE__); \
+    } while (0)
+
+#define TKA_LAUNCH_CHECK() TKA_HIP(hipGetLastError())
+
+[[noreturn]] inline void fail(tka_status s, const std::string& m) { throw Error(s, m); }
+
+inline void require(bool ok, tka_status s, const std::string& m)
+{
+    if (!ok) fail(s, m);
+}
+
+// Owning device allocation.  Allocation size is tracked against an optional
+// cap so tests can force the out-of-memory path from the host.
+struct MemBudget {
+    int64_t cap = 0;   // 0 = unlimited
+    int64_t used = 0;
+};
+
+template <class T>
+class DevBuf {
+  public:
+    DevBuf() = default;
+    DevBuf(size_t n, MemBudget* b = nullptr) { alloc(n, b); }
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept { swap(o); }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this != &o) {
+            release();
+            swap(o);
+        }
+        return *this;
+    }
+    void alloc(size_t n, MemBudget* b = nullptr)
+    {
+        release();
+        const int64_t bytes = (int64_t)(n ? n : 1) * (int64_t)sizeof(T);
+        if (b && b->cap > 0 && b->used + bytes > b->cap)
+            fail(TKA_E_NOT_ENOUGH_MEMORY, "device allocation of " + std::to_string(bytes) +
+                                              " bytes exceeds the configured cap");
+        TKA_HIP(hipMalloc((void**)&p_, (size_t)bytes));
+        n_ = n;
+        budget_ = b;
+        bytes_ = bytes;
+        if (b) b->used += bytes;
+    }
+    void release()
+    {
+        if (p_) {
+            (void)hipFree(p_);
+            if (budget_) budget_->used -= bytes_;
+        }
+        p_ = nullptr;
+        n_ = 0;
+        budget_ = nullptr;
+        bytes_ = 0;
+    }
+    T* get() const { return p_; }
+    operator T*() const { return p_; }
+    size_t size() const { return n_; }
+
+  private:
+    void swap(DevBuf& o)
+    {
+        std::swap(p_, o.p_);
+        std::swap(n_, o.n_);
+        std::swap(budget_, o.budget_);
+        std::swap(bytes_, o.bytes_);
+    }
+    T* p_ = nullptr;
+    size_t n_ = 0;
+    MemBudget* budget_ = nullptr;
+    int64_t bytes_ = 0;
+};
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// ---------------------------------------------------------------- device side
+#ifdef __HIP_DEVICE_COMPILE__
+#define TKA_DEV __device__ __forceinline__
+#else
+#define TKA_DEV __device__ inline
+#endif
+
+constexpr int kWave = 64;
+constexpr int kBoxStride = 8;  // doubles per block of Y statistics
+
+template <class T>
+TKA_DEV T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <class T>
+TKA_DEV T wave_min(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+template <class T>
+TKA_DEV T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        T w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// Block-wide sum of a double for blockDim.x <= 1024 (multiple of 64).
+// Result valid in every thread.  `scratch` needs blockDim.x/64 doubles.
+TKA_DEV double block_sum(double v, double* scratch)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) scratch[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < nw; i++) t += scratch[i];
+    return t;
+}
+
+}  // namespace tka

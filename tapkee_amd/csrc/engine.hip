@@ -1,0 +1,1152 @@
+// Host engine and C-ABI entry points (include/tapkee_amd.h).
+//
+// tka_tsne_run reproduces the schedule of tsne::TSNE::run
+// (tsne.hpp:61-205): normalise X, build P (kNN + perplexity + symmetrise +
+// normalise at theta > 0, dense at theta == 0), exaggerate by 12, draw Y0
+// from std::rand, then 1000 iterations of gradient + gains/momentum update +
+// zeroMean, switching off the exaggeration and raising the momentum after
+// iteration 250, with a KL evaluation every 50 iterations and at the last.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "affinity.hpp"
+#include "bh.hpp"
+#include "common.hpp"
+#include "forces.hpp"
+#include "knn.hpp"
+
+namespace tka {
+namespace {
+
+thread_local std::string g_last_error;
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Stream {
+    hipStream_t s = nullptr;
+    bool owned = false;
+    explicit Stream(void* ext = nullptr)
+    {
+        if (ext) {
+            s = (hipStream_t)ext;
+        } else {
+            TKA_HIP(hipStreamCreate(&s));
+            owned = true;
+        }
+    }
+    ~Stream()
+    {
+        if (owned && s) (void)hipStreamDestroy(s);
+    }
+    void sync() { TKA_HIP(hipStreamSynchronize(s)); }
+};
+
+template <class T>
+void upload(DevBuf<T>& d, const T* h, size_t n, hipStream_t s, MemBudget* mb = nullptr)
+{
+    d.alloc(n, mb);
+    if (n) TKA_HIP(hipMemcpyAsync(d.get(), h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+template <class T>
+void download(T* h, const T* d, size_t n, hipStream_t s)
+{
+    if (n) TKA_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+
+__global__ void cast_kernel_dd(const double* __restrict__ a, double* __restrict__ b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) b[i] = a[i];
+}
+__global__ void cast_kernel_df(const double* __restrict__ a, float* __restrict__ b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) b[i] = (float)a[i];
+}
+__global__ void cast_kernel_fd(const float* __restrict__ a, double* __restrict__ b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) b[i] = (double)a[i];
+}
+
+template <class A, class B>
+void cast_device(const A* a, B* b, size_t n, hipStream_t s)
+{
+    const unsigned g = (unsigned)std::min<size_t>(8192, ceil_div(n ? n : 1, 256));
+    if constexpr (std::is_same<A, double>::value && std::is_same<B, float>::value)
+        hipLaunchKernelGGL(cast_kernel_df, dim3(g), dim3(256), 0, s, a, b, n);
+    else if constexpr (std::is_same<A, float>::value && std::is_same<B, double>::value)
+        hipLaunchKernelGGL(cast_kernel_fd, dim3(g), dim3(256), 0, s, a, b, n);
+    else
+        hipLaunchKernelGGL(cast_kernel_dd, dim3(g), dim3(256), 0, s, (const double*)a, (double*)b, n);
+    TKA_LAUNCH_CHECK();
+}
+
+__global__ void fill_kernel(double* __restrict__ a, size_t n, double v)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) a[i] = v;
+}
+__global__ void fill_kernel_f(float* __restrict__ a, size_t n, float v)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) a[i] = v;
+}
+template <class T>
+void fill_device(T* a, size_t n, double v, hipStream_t s)
+{
+    const unsigned g = (unsigned)std::min<size_t>(8192, ceil_div(n ? n : 1, 256));
+    if constexpr (std::is_same<T, float>::value)
+        hipLaunchKernelGGL(fill_kernel_f, dim3(g), dim3(256), 0, s, a, n, (float)v);
+    else
+        hipLaunchKernelGGL(fill_kernel, dim3(g), dim3(256), 0, s, a, n, v);
+    TKA_LAUNCH_CHECK();
+}
+
+void check_finite(const double* X, size_t n, const char* what)
+{
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(X[i])) fail(TKA_E_WRONG_PARAMETER, std::string(what) + " contains a non-finite value");
+}
+
+// std::rand replay of the reference's Y0 (tsne.hpp:137-138 with
+// defines/random.hpp:38-54), after the VP-tree build's draws
+// (vptree.hpp:226: one draw per node with more than one item).
+long long vptree_draws(long long n)
+{
+    long long calls = 0;
+    std::vector<long long> st{n};
+    while (!st.empty()) {
+        const long long m = st.back();
+        st.pop_back();
+        if (m <= 1) continue;
+        calls++;
+        st.push_back(m / 2 - 1);
+        st.push_back(m - m / 2);
+    }
+    return calls;
+}
+
+double gaussian_random()
+{
+    double x, y, r;
+    do {
+        x = 2 * (std::rand() / ((double)RAND_MAX + 1)) - 1;
+        y = 2 * (std::rand() / ((double)RAND_MAX + 1)) - 1;
+        r = x * x + y * y;
+    } while (r >= 1.0 || r == 0.0);
+    r = std::sqrt(-2 * std::log(r) / r);
+    return x * r;
+}
+
+void validate(int N, int D, int no_dims, double perplexity, double theta)
+{
+    // methods/base.hpp:40-43, methods/tsne.hpp:20-21
+    if (N <= 0) fail(TKA_E_NO_DATA, "No data to embed");
+    if (!(no_dims >= 1 && no_dims < N))
+        fail(TKA_E_WRONG_PARAMETER, "[1, " + std::to_string(N) + ") range check failed for target dimension, its value is " +
+                                        std::to_string(no_dims));
+    if (!(perplexity >= 0.0 && perplexity <= (N - 1) / 3.0))
+        fail(TKA_E_WRONG_PARAMETER, "[0, " + std::to_string((N - 1) / 3.0) +
+                                        "] range check failed for SNE perplexity, its value is " +
+                                        std::to_string(perplexity));
+    if (!(theta >= 0.0)) fail(TKA_E_WRONG_PARAMETER, "Non-negativity check failed for SNE theta");
+    if (D <= 0) fail(TKA_E_WRONG_PARAMETER, "feature dimension must be positive");
+    // Engine limits (documented deviations): the Barnes-Hut tree is 2-D, the
+    // exact path supports 1..3 dims, and K = (int)(3*perplexity) must be >= 1.
+    if (theta > 0.0 && no_dims != 2)
+        fail(TKA_E_WRONG_PARAMETER, "Barnes-Hut t-SNE (theta > 0) needs target dimension 2");
+    if (no_dims > 3) fail(TKA_E_WRONG_PARAMETER, "target dimension above 3 is not supported");
+}
+
+template <class T>
+struct LoopState {
+    DevBuf<T> Ya, Yb, uY, gains, neg, pos;
+    DevBuf<double> zpart, box;
+    DevBuf<int> row, col;
+    DevBuf<T> val;   // CSR values (sparse) or dense P (N x N)
+    bool dense = false;
+};
+
+// Accumulates HIP-event timings of one kernel class on the session stream.
+struct EventTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    hipEvent_t pending = nullptr;
+    double ms = 0.0;
+    int64_t launches = 0;
+    void begin(hipStream_t s)
+    {
+        TKA_HIP(hipEventCreate(&pending));
+        TKA_HIP(hipEventRecord(pending, s));
+    }
+    void end(hipStream_t s)
+    {
+        hipEvent_t e;
+        TKA_HIP(hipEventCreate(&e));
+        TKA_HIP(hipEventRecord(e, s));
+        ev.emplace_back(pending, e);
+        pending = nullptr;
+    }
+    void collect()
+    {
+        for (auto& p : ev) {
+            float t = 0.f;
+            TKA_HIP(hipEventElapsedTime(&t, p.first, p.second));
+            ms += t;
+            launches++;
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+        ev.clear();
+    }
+    ~EventTimer()
+    {
+        for (auto& p : ev) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+    }
+};
+
+enum { kTimBuild = 0, kTimRepulsion = 1, kTimStep = 2, kTimExact = 3, kTimCount = 4 };
+
+}  // namespace
+}  // namespace tka
+
+// Opaque handle of the C-ABI: one embedding problem with its device state.
+struct tka_session {
+    tka_tsne_options o;
+    tka_tsne_stats st{};
+    tka::MemBudget mb;
+    tka::Stream stream;
+    int N = 0, dims = 0;
+    double theta = 0.0;
+    bool exact = false;
+    int dd_mode = TKA_DD_TEXTBOOK;
+    int iter = 0;
+    double exag = 12.0, momentum = 0.5;
+    bool profile = false;
+    tka::EventTimer timers[tka::kTimCount];
+    tka::DevBuf<unsigned long long> interactions;
+    virtual ~tka_session() = default;
+    virtual void step(int n) = 0;
+    virtual void embedding(double* Y) = 0;
+    virtual double kl_exact() = 0;
+    hipStream_t s() const { return stream.s; }
+};
+
+namespace tka {
+namespace {
+
+void log_kl(tka_session& c, int iter, double C)
+{
+    tka_tsne_stats& st = c.st;
+    if (st.kl_count < TKA_MAX_KL_TRACE) {
+        st.kl_iter[st.kl_count] = iter;
+        st.kl_value[st.kl_count] = C;
+        st.kl_count++;
+    }
+    if (c.o.verbose) fprintf(stderr, "Iteration %d: error is %.17g\n", iter, C);
+}
+
+template <class T>
+struct Session : tka_session {
+    LoopState<T> L;
+    BhTree<T> tree;
+    T* Ycur = nullptr;
+    T* Ynext = nullptr;
+    int nbox = 0, nz = 0;
+    bool tree_ready = false;  // tree, neg and zpart already computed for Ycur
+    bool centred = false;     // Ycur is zero-mean (else its mean is in the box partials)
+
+    void init(const std::vector<double>& Y0)
+    {
+        hipStream_t s = this->s();
+        MemBudget* m = &mb;
+        const size_t nd = (size_t)N * dims;
+        std::vector<T> h(nd);
+        for (size_t i = 0; i < nd; i++) h[i] = (T)Y0[i];
+        upload(L.Ya, h.data(), nd, s, m);
+        TKA_HIP(hipStreamSynchronize(s));
+        L.Yb.alloc(nd, m);
+        L.uY.alloc(nd, m);
+        L.gains.alloc(nd, m);
+        L.neg.alloc(nd, m);
+        fill_device(L.uY.get(), nd, 0.0, s);
+        fill_device(L.gains.get(), nd, 1.0, s);
+        const int nstep = tsne_step_blocks(N);
+        const int nbox0 = bbox_blocks(N);
+        L.box.alloc((size_t)kBoxStride * std::max(nstep, nbox0), m);
+        nz = exact ? exact_zblocks(N) : bh_zblocks(N);
+        L.zpart.alloc(nz, m);
+        if (exact && L.dense) L.pos.alloc(nd, m);
+        // tsne.hpp:75-80: uY = 0, gains = 1; Y0 is not centred before iteration 0
+        bbox_partials_device<T>(L.Ya.get(), N, L.box.get(), s);
+        if (!exact) tree.alloc(N, m);
+        nbox = nbox0;
+        Ycur = L.Ya.get();
+        Ynext = L.Yb.get();
+        exag = o.exaggeration;
+        momentum = o.momentum;
+        interactions.alloc(1, m);
+        TKA_HIP(hipMemsetAsync(interactions.get(), 0, sizeof(unsigned long long), s));
+    }
+
+    void step(int n) override
+    {
+        hipStream_t s = this->s();
+        MemBudget* m = &mb;
+        const int* row = L.dense ? nullptr : L.row.get();
+        const int* col = L.dense ? nullptr : L.col.get();
+        for (int k = 0; k < n; k++, iter++) {
+            if (exact) {
+                if (profile) timers[kTimExact].begin(s);
+                exact_forces_device<T>(dims, Ycur, N, dd_mode, L.dense ? L.val.get() : nullptr, exag, L.neg.get(),
+                                       L.dense ? L.pos.get() : nullptr, L.zpart.get(), s);
+                if (profile) timers[kTimExact].end(s);
+            } else if (!tree_ready) {
+                if (profile) timers[kTimBuild].begin(s);
+                bh_build<T>(Ycur, N, L.box.get(), nbox, tree, s);
+                if (profile) timers[kTimBuild].end(s);
+                if (profile) timers[kTimRepulsion].begin(s);
+                bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(),
+                                profile ? interactions.get() : nullptr, s);
+                if (profile) timers[kTimRepulsion].end(s);
+            }
+            tree_ready = false;
+            StepParams sp{momentum, o.eta, exag};
+            if (profile) timers[kTimStep].begin(s);
+            nbox = tsne_step_device<T>(dims, Ycur, Ynext, L.uY.get(), L.gains.get(), L.neg.get(),
+                                       L.dense ? L.pos.get() : nullptr, row, col, L.dense ? nullptr : L.val.get(),
+                                       L.zpart.get(), nz, N, sp, L.box.get(), nullptr, s);
+            if (profile) timers[kTimStep].end(s);
+            std::swap(Ycur, Ynext);
+            centred = false;
+            if (exact) {
+                center_device<T>(dims, Ycur, N, L.box.get(), nbox, s);  // tsne.hpp:167
+                centred = true;
+            }
+            if (iter == o.stop_lying_iter) exag = 1.0;                   // tsne.hpp:170-179
+            if (iter == o.mom_switch_iter) momentum = o.final_momentum;  // tsne.hpp:180-181
+            const bool log_now =
+                o.log_every > 0 && iter > 0 && ((iter % o.log_every == 0) || (iter == o.max_iter - 1));
+            if (!log_now) continue;
+            double C;
+            if (exact) {
+                if (L.dense) {
+                    C = kl_dense_device<T>(dims, Ycur, L.val.get(), exag, N, dd_mode, m, s);
+                } else {
+                    const double Z = exact_z_device<T>(dims, Ycur, N, m, s);
+                    C = kl_edges_exact_device<T>(dims, Ycur, row, col, L.val.get(), exag, Z, N, m, s);
+                }
+            } else {
+                // tsne.hpp:454-486: Barnes-Hut estimate of Z from a tree on the
+                // updated, zero-mean Y, then the sum over P's pattern.  The
+                // next iteration reuses this tree and these forces.
+                bh_build<T>(Ycur, N, L.box.get(), nbox, tree, s);
+                bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(), nullptr, s);
+                tree_ready = true;
+                centred = true;
+                if (o.kl_mode == TKA_KL_EXACT_Z) {
+                    const double Z = exact_z_device<T>(dims, Ycur, N, m, s);
+                    C = kl_edges_exact_device<T>(dims, Ycur, row, col, L.val.get(), exag, Z, N, m, s);
+                } else {
+                    const double Z = sum_device(L.zpart.get(), nz, m, s);
+                    C = kl_edges_reference_device<T>(dims, Ycur, row, col, L.val.get(), exag, Z, N, m, s);
+                }
+            }
+            log_kl(*this, iter, C);
+        }
+    }
+
+    // Current solution, zero-mean (tsne.hpp:167), as FP64 N x dims.
+    void embedding(double* Y) override
+    {
+        hipStream_t s = this->s();
+        const size_t nd = (size_t)N * dims;
+        DevBuf<T> tmp(nd, &mb);
+        TKA_HIP(hipMemcpyAsync(tmp.get(), Ycur, nd * sizeof(T), hipMemcpyDeviceToDevice, s));
+        if (!centred) center_device<T>(dims, tmp.get(), N, L.box.get(), nbox, s);
+        DevBuf<double> Yd(nd, &mb);
+        cast_device<T, double>(tmp.get(), Yd.get(), nd, s);
+        download(Y, Yd.get(), nd, s);
+        TKA_HIP(hipStreamSynchronize(s));
+    }
+
+    // KL(P||Q) of the current Y with the exact normaliser, FP64, P without
+    // exaggeration (translation invariant: no centring needed).
+    double kl_exact() override
+    {
+        hipStream_t s = this->s();
+        MemBudget* m = &mb;
+        const size_t nd = (size_t)N * dims;
+        DevBuf<double> Yd(nd, m);
+        cast_device<T, double>(Ycur, Yd.get(), nd, s);
+        if (L.dense) {
+            DevBuf<double> Pd((size_t)N * N, m);
+            cast_device<T, double>(L.val.get(), Pd.get(), (size_t)N * N, s);
+            return kl_dense_device<double>(dims, Yd.get(), Pd.get(), 1.0, N, TKA_DD_TEXTBOOK, m, s);
+        }
+        DevBuf<double> vd(L.val.size(), m);
+        cast_device<T, double>(L.val.get(), vd.get(), L.val.size(), s);
+        const double Z = exact_z_device<double>(dims, Yd.get(), N, m, s);
+        return kl_edges_exact_device<double>(dims, Yd.get(), L.row.get(), L.col.get(), vd.get(), 1.0, Z, N, m, s);
+    }
+};
+
+// Validation, normalisation, P and Y0: everything tsne.hpp:61-139 does
+// before the main loop.  The session's stream is created on o.device.
+tka_session* create_session(const double* Xh, int N, int D, int no_dims, double perplexity, double theta,
+                            const tka_tsne_options* opt_in)
+{
+    tka_tsne_options o;
+    tka_default_options(&o);
+    if (opt_in) {
+        require(opt_in->struct_size == (int32_t)sizeof(tka_tsne_options), TKA_E_WRONG_PARAMETER,
+                "tka_tsne_options.struct_size mismatch");
+        o = *opt_in;
+    }
+    validate(N, D, no_dims, perplexity, theta);
+    require(Xh != nullptr, TKA_E_WRONG_PARAMETER, "null data pointer");
+    check_finite(Xh, (size_t)N * D, "input data");
+    if (o.cancel && o.cancel()) fail(TKA_E_CANCELLED, "cancelled");  // methods.hpp:30-31
+    const bool exact = (theta == 0.0);
+    const bool sparse = !exact || o.exact_p_mode == TKA_EXACT_P_KNN;
+    const int K = o.knn_k > 0 ? o.knn_k : (int)(3 * perplexity);
+    if (sparse) {
+        require(K >= 1, TKA_E_WRONG_PARAMETER, "perplexity must be at least 1/3 (K = 3*perplexity neighbours)");
+        require(K <= N - 1, TKA_E_WRONG_PARAMETER, "K must be below the number of points");
+        if (o.verbose && perplexity > K) fprintf(stderr, "Perplexity should be lower than K!\n");  // tsne.hpp:604-605
+    }
+    if (o.y0_mode == TKA_Y0_EXPLICIT) require(o.y0 != nullptr, TKA_E_WRONG_PARAMETER, "TKA_Y0_EXPLICIT needs y0");
+    if (o.verbose) fprintf(stderr, exact ? "Using exact t-SNE algorithm\n" : "Using Barnes-Hut-SNE algorithm\n");
+    TKA_HIP(hipSetDevice(o.device));
+    std::unique_ptr<tka_session> sess;
+    if (o.loop_precision == TKA_FP64)
+        sess.reset(new Session<double>());
+    else
+        sess.reset(new Session<float>());
+    tka_session& c = *sess;
+    c.o = o;
+    c.mb.cap = o.device_mem_cap;
+    c.N = N;
+    c.dims = no_dims;
+    c.theta = theta;
+    c.exact = exact;
+    c.dd_mode = o.dd_mode;
+    hipStream_t s = c.s();
+    MemBudget* mb = &c.mb;
+    tka_tsne_stats& st = c.st;
+    const double t0 = now_ms();
+
+    DevBuf<double> X;  // normalise (tsne.hpp:88-89)
+    upload(X, Xh, (size_t)N * D, s, mb);
+    normalize_device(X.get(), N, D, mb, s);
+    c.stream.sync();
+    const double t1 = now_ms();
+    st.ms_normalize = t1 - t0;
+
+    auto fill = [&](auto* S) {
+        using TT = typename std::remove_reference<decltype(*S->L.val.get())>::type;
+        if (sparse) {
+            DevBuf<int> idx((size_t)N * K, mb);
+            DevBuf<double> dist((size_t)N * K, mb), val((size_t)N * K, mb);
+            KnnStats ks;
+            knn_device(X.get(), N, D, K, o.dist_dtype, o.candidates, idx.get(), dist.get(), &ks, mb, s);
+            c.stream.sync();
+            const double t2 = now_ms();
+            X.release();
+            const int unconv = perplexity_knn_device(dist.get(), N, K, perplexity, val.get(), nullptr, nullptr, mb, s);
+            c.stream.sync();
+            const double t3 = now_ms();
+            Csr P;
+            symmetrize_knn_device(idx.get(), val.get(), N, K, P, mb, s);
+            normalize_sum_device(P.val.get(), P.nnz, mb, s);  // tsne.hpp:120-124
+            c.stream.sync();
+            const double t4 = now_ms();
+            st.ms_knn = t2 - t1;
+            st.ms_perplexity = t3 - t2;
+            st.ms_symmetrize = t4 - t3;
+            st.nnz = P.nnz;
+            st.knn_k = K;
+            st.bisection_unconverged = unconv;
+            st.certificate_failures = ks.certificate_failures;
+            S->L.row = std::move(P.row);
+            S->L.col = std::move(P.col);
+            S->L.val.alloc(P.nnz, mb);
+            cast_device<double, TT>(P.val.get(), S->L.val.get(), P.nnz, s);
+        } else {
+            DevBuf<double> DD((size_t)N * N, mb), P((size_t)N * N, mb);
+            sqdist_dense_device(X.get(), N, D, o.dd_mode, DD.get(), s);
+            const int unconv = perplexity_dense_device(DD.get(), N, perplexity, P.get(), mb, s);
+            DD.release();
+            symmetrize_dense_device(P.get(), N, mb, s);  // tsne.hpp:100-108
+            c.stream.sync();
+            st.ms_perplexity = now_ms() - t1;
+            st.nnz = (int64_t)N * N;
+            st.bisection_unconverged = unconv;
+            S->L.dense = true;
+            S->L.val.alloc((size_t)N * N, mb);
+            cast_device<double, TT>(P.get(), S->L.val.get(), (size_t)N * N, s);
+        }
+        c.stream.sync();
+    };
+    if (o.loop_precision == TKA_FP64)
+        fill(static_cast<Session<double>*>(sess.get()));
+    else
+        fill(static_cast<Session<float>*>(sess.get()));
+
+    // Y0 (tsne.hpp:137-138): std::rand after the VP-tree's draws on the BH path.
+    std::vector<double> Y0((size_t)N * no_dims);
+    if (o.y0_mode == TKA_Y0_EXPLICIT) {
+        std::memcpy(Y0.data(), o.y0, Y0.size() * sizeof(double));
+        check_finite(Y0.data(), Y0.size(), "y0");
+    } else {
+        if (o.y0_mode == TKA_Y0_SEEDED) std::srand(o.seed);
+        if (!exact) {
+            const long long burn = vptree_draws(N);
+            for (long long i = 0; i < burn; i++) (void)std::rand();
+        }
+        for (auto& v : Y0) v = gaussian_random() * .0001;
+    }
+    if (o.loop_precision == TKA_FP64)
+        static_cast<Session<double>*>(sess.get())->init(Y0);
+    else
+        static_cast<Session<float>*>(sess.get())->init(Y0);
+    c.stream.sync();
+    return sess.release();
+}
+
+void tsne_run_impl(const double* Xh, int N, int D, double* Yh, int no_dims, double perplexity, double theta,
+                   const tka_tsne_options* opt_in, tka_tsne_stats* st)
+{
+    require(Yh != nullptr, TKA_E_WRONG_PARAMETER, "null output pointer");
+    const double t0 = now_ms();
+    std::unique_ptr<tka_session> sess(create_session(Xh, N, D, no_dims, perplexity, theta, opt_in));
+    const double t5 = now_ms();
+    sess->step(sess->o.max_iter);                       // tsne.hpp:143-193
+    sess->embedding(Yh);
+    const double t6 = now_ms();
+    sess->st.kl_final_exact = sess->kl_exact();
+    sess->st.ms_loop = t6 - t5;
+    sess->st.ms_total = t6 - t0;
+    if (st) *st = sess->st;
+}
+
+template <class F>
+tka_status guard(F&& f)
+{
+    try {
+        f();
+        g_last_error.clear();
+        return TKA_OK;
+    } catch (const Error& e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        g_last_error = "Not enough memory";
+        return TKA_E_NOT_ENOUGH_MEMORY;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return TKA_E_INTERNAL;
+    }
+}
+
+template <class T>
+void gradient_impl(const double* Yh, int N, int dims, const int* rowh, const int* colh, const double* valh,
+                   double theta, double* dYh, double* sum_q)
+{
+    MemBudget mb;
+    Stream st;
+    hipStream_t s = st.s;
+    const size_t nd = (size_t)N * dims;
+    const int64_t nnz = rowh[N];
+    DevBuf<double> tmp;
+    DevBuf<T> Y(nd), neg(nd), dC(nd), val(nnz ? nnz : 1);
+    upload(tmp, Yh, nd, s);
+    cast_device<double, T>(tmp.get(), Y.get(), nd, s);
+    DevBuf<int> row, col;
+    upload(row, rowh, (size_t)N + 1, s);
+    upload(col, colh, (size_t)nnz, s);
+    DevBuf<double> vtmp;
+    upload(vtmp, valh, (size_t)nnz, s);
+    cast_device<double, T>(vtmp.get(), val.get(), nnz, s);
+    int nz;
+    DevBuf<double> zpart;
+    if (theta > 0.0) {
+        require(dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut gradient needs 2 dimensions");
+        nz = bh_zblocks(N);
+        zpart.alloc(nz);
+        DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
+        bbox_partials_device<T>(Y.get(), N, box.get(), s);
+        BhTree<T> tree;
+        tree.alloc(N, &mb);
+        bh_build<T>(Y.get(), N, box.get(), bbox_blocks(N), tree, s);
+        bh_repulsion<T>(tree, Y.get(), theta, neg.get(), zpart.get(), nullptr, s);
+        gradient_combine_device<T>(dims, Y.get(), neg.get(), nullptr, row.get(), col.get(), val.get(), 1.0,
+                                   zpart.get(), nz, N, dC.get(), s);
+    } else {
+        nz = exact_zblocks(N);
+        zpart.alloc(nz);
+        exact_forces_device<T>(dims, Y.get(), N, TKA_DD_TEXTBOOK, nullptr, 1.0, neg.get(), nullptr, zpart.get(), s);
+        gradient_combine_device<T>(dims, Y.get(), neg.get(), nullptr, row.get(), col.get(), val.get(), 1.0,
+                                   zpart.get(), nz, N, dC.get(), s);
+    }
+    DevBuf<double> out(nd);
+    cast_device<T, double>(dC.get(), out.get(), nd, s);
+    download(dYh, out.get(), nd, s);
+    const double z = sum_device(zpart.get(), nz, &mb, s);
+    st.sync();
+    if (sum_q) *sum_q = z;
+}
+
+template <class T>
+void gradient_dense_impl(const double* Ph, const double* Yh, int N, int dims, int dd_mode, double* dYh, double* sum_q)
+{
+    MemBudget mb;
+    Stream st;
+    hipStream_t s = st.s;
+    const size_t nd = (size_t)N * dims;
+    DevBuf<double> tmp, ptmp;
+    DevBuf<T> Y(nd), P((size_t)N * N), neg(nd), pos(nd), dC(nd);
+    upload(tmp, Yh, nd, s);
+    cast_device<double, T>(tmp.get(), Y.get(), nd, s);
+    upload(ptmp, Ph, (size_t)N * N, s);
+    cast_device<double, T>(ptmp.get(), P.get(), (size_t)N * N, s);
+    const int nz = exact_zblocks(N);
+    DevBuf<double> zpart(nz);
+    exact_forces_device<T>(dims, Y.get(), N, dd_mode, P.get(), 1.0, neg.get(), pos.get(), zpart.get(), s);
+    gradient_combine_device<T>(dims, Y.get(), neg.get(), pos.get(), nullptr, nullptr, nullptr, 1.0, zpart.get(), nz,
+                               N, dC.get(), s);
+    DevBuf<double> out(nd);
+    cast_device<T, double>(dC.get(), out.get(), nd, s);
+    download(dYh, out.get(), nd, s);
+    const double z = sum_device(zpart.get(), nz, &mb, s);
+    st.sync();
+    if (sum_q) *sum_q = z;
+}
+
+}  // namespace
+}  // namespace tka
+
+using namespace tka;
+
+extern "C" {
+
+TKA_API void tka_default_options(tka_tsne_options* o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->struct_size = (int32_t)sizeof(tka_tsne_options);
+    o->max_iter = 1000;        // tsne.hpp:71
+    o->stop_lying_iter = 250;
+    o->mom_switch_iter = 250;
+    o->momentum = 0.5;         // tsne.hpp:72
+    o->final_momentum = 0.8;
+    o->eta = 200.0;            // tsne.hpp:73
+    o->exaggeration = 12.0;    // tsne.hpp:129-134
+    o->knn_k = 0;
+    o->dist_dtype = TKA_DIST_BF16;
+    o->candidates = 0;
+    o->loop_precision = TKA_FP32;
+    o->exact_p_mode = TKA_EXACT_P_DENSE;
+    o->dd_mode = TKA_DD_TEXTBOOK;
+    o->y0_mode = TKA_Y0_RAND_COMPAT;
+    o->seed = 1;
+    o->y0 = nullptr;
+    o->log_every = 50;         // tsne.hpp:184
+    o->kl_mode = TKA_KL_AS_REFERENCE;
+    o->verbose = 0;
+    o->device = 0;
+    o->device_mem_cap = 0;
+    o->cancel = nullptr;
+}
+
+TKA_API const char* tka_last_error(void) { return g_last_error.c_str(); }
+
+TKA_API int32_t tka_version(void) { return 10000; }
+
+TKA_API tka_status tka_tsne_run(const double* X, int32_t N, int32_t D, double* Y, int32_t no_dims, double perplexity,
+                                double theta, const tka_tsne_options* opt, tka_tsne_stats* stats)
+{
+    return guard([&] { tsne_run_impl(X, N, D, Y, no_dims, perplexity, theta, opt, stats); });
+}
+
+TKA_API tka_status tka_session_create(const double* X, int32_t N, int32_t D, int32_t no_dims, double perplexity,
+                                      double theta, const tka_tsne_options* opt, tka_session** out)
+{
+    return guard([&] {
+        require(out != nullptr, TKA_E_WRONG_PARAMETER, "null session pointer");
+        *out = nullptr;
+        *out = create_session(X, N, D, no_dims, perplexity, theta, opt);
+    });
+}
+
+TKA_API tka_status tka_session_step(tka_session* s, int32_t iterations)
+{
+    return guard([&] {
+        require(s != nullptr && iterations >= 0, TKA_E_WRONG_PARAMETER, "bad arguments");
+        s->step(iterations);
+    });
+}
+
+TKA_API tka_status tka_session_synchronize(tka_session* s)
+{
+    return guard([&] {
+        require(s != nullptr, TKA_E_WRONG_PARAMETER, "null session");
+        s->stream.sync();
+    });
+}
+
+TKA_API tka_status tka_session_embedding(tka_session* s, double* Y)
+{
+    return guard([&] {
+        require(s != nullptr && Y != nullptr, TKA_E_WRONG_PARAMETER, "bad arguments");
+        s->embedding(Y);
+    });
+}
+
+TKA_API tka_status tka_session_kl(tka_session* s, double* kl)
+{
+    return guard([&] {
+        require(s != nullptr && kl != nullptr, TKA_E_WRONG_PARAMETER, "bad arguments");
+        *kl = s->kl_exact();
+    });
+}
+
+TKA_API tka_status tka_session_stats(tka_session* s, tka_tsne_stats* stats)
+{
+    return guard([&] {
+        require(s != nullptr && stats != nullptr, TKA_E_WRONG_PARAMETER, "bad arguments");
+        *stats = s->st;
+    });
+}
+
+TKA_API tka_status tka_session_profile(tka_session* s, int32_t enable)
+{
+    return guard([&] {
+        require(s != nullptr, TKA_E_WRONG_PARAMETER, "null session");
+        s->profile = enable != 0;
+    });
+}
+
+TKA_API tka_status tka_session_kernel_timing(tka_session* s, tka_kernel_timing* t)
+{
+    return guard([&] {
+        require(s != nullptr && t != nullptr, TKA_E_WRONG_PARAMETER, "bad arguments");
+        s->stream.sync();
+        std::memset(t, 0, sizeof(*t));
+        for (auto& tm : s->timers) tm.collect();
+        t->launches_build = s->timers[kTimBuild].launches;
+        t->launches_repulsion = s->timers[kTimRepulsion].launches;
+        t->launches_step = s->timers[kTimStep].launches;
+        t->launches_exact = s->timers[kTimExact].launches;
+        t->ms_build = s->timers[kTimBuild].ms;
+        t->ms_repulsion = s->timers[kTimRepulsion].ms;
+        t->ms_step = s->timers[kTimStep].ms;
+        t->ms_exact = s->timers[kTimExact].ms;
+        unsigned long long inter = 0;
+        if (s->interactions.get()) {
+            TKA_HIP(hipMemcpy(&inter, s->interactions.get(), sizeof(inter), hipMemcpyDeviceToHost));
+            TKA_HIP(hipMemsetAsync(s->interactions.get(), 0, sizeof(inter), s->s()));
+            s->stream.sync();
+        }
+        t->interactions = (int64_t)inter;
+        t->nnz = s->st.nnz;
+        t->n = s->N;
+        t->dims = s->dims;
+        for (auto& tm : s->timers) {
+            tm.ms = 0.0;
+            tm.launches = 0;
+        }
+    });
+}
+
+TKA_API void tka_session_destroy(tka_session* s) { delete s; }
+
+TKA_API tka_status tka_normalize(double* X, int32_t N, int32_t D)
+{
+    return guard([&] {
+        require(N > 0 && D > 0 && X, TKA_E_WRONG_PARAMETER, "bad shape");
+        check_finite(X, (size_t)N * D, "input data");
+        Stream st;
+        DevBuf<double> d;
+        upload(d, X, (size_t)N * D, st.s);
+        normalize_device(d.get(), N, D, nullptr, st.s);
+        download(X, d.get(), (size_t)N * D, st.s);
+        st.sync();
+    });
+}
+
+TKA_API tka_status tka_knn(const double* X, int32_t N, int32_t D, int32_t K, int32_t dist_dtype, int32_t candidates,
+                           int32_t* idx, double* dist, int32_t* certificate_failures)
+{
+    return guard([&] {
+        require(N > 0 && D > 0 && X && idx && dist, TKA_E_WRONG_PARAMETER, "bad arguments");
+        check_finite(X, (size_t)N * D, "input data");
+        Stream st;
+        DevBuf<double> d;
+        upload(d, X, (size_t)N * D, st.s);
+        DevBuf<int> di((size_t)N * K);
+        DevBuf<double> dd((size_t)N * K);
+        KnnStats ks;
+        knn_device(d.get(), N, D, K, dist_dtype, candidates, di.get(), dd.get(), &ks, nullptr, st.s);
+        download(idx, di.get(), (size_t)N * K, st.s);
+        download(dist, dd.get(), (size_t)N * K, st.s);
+        st.sync();
+        if (certificate_failures) *certificate_failures = ks.certificate_failures;
+    });
+}
+
+TKA_API tka_status tka_perplexity(const double* dist, int32_t N, int32_t K, double perplexity, double* val,
+                                  double* beta, int32_t* iters, int32_t* unconverged)
+{
+    return guard([&] {
+        require(N > 0 && K > 0 && dist && val, TKA_E_WRONG_PARAMETER, "bad arguments");
+        require(perplexity > 0.0, TKA_E_WRONG_PARAMETER, "perplexity must be positive");
+        Stream st;
+        DevBuf<double> d, v((size_t)N * K), b(N);
+        DevBuf<int> it(N);
+        upload(d, dist, (size_t)N * K, st.s);
+        const int u = perplexity_knn_device(d.get(), N, K, perplexity, v.get(), b.get(), it.get(), nullptr, st.s);
+        download(val, v.get(), (size_t)N * K, st.s);
+        if (beta) download(beta, b.get(), N, st.s);
+        if (iters) download(iters, it.get(), N, st.s);
+        st.sync();
+        if (unconverged) *unconverged = u;
+    });
+}
+
+TKA_API tka_status tka_perplexity_dense(const double* X, int32_t N, int32_t D, double perplexity, int32_t dd_mode,
+                                        double* P)
+{
+    return guard([&] {
+        require(N > 1 && D > 0 && X && P, TKA_E_WRONG_PARAMETER, "bad arguments");
+        Stream st;
+        DevBuf<double> x, DD((size_t)N * N), Pd((size_t)N * N);
+        upload(x, X, (size_t)N * D, st.s);
+        sqdist_dense_device(x.get(), N, D, dd_mode, DD.get(), st.s);
+        perplexity_dense_device(DD.get(), N, perplexity, Pd.get(), nullptr, st.s);
+        download(P, Pd.get(), (size_t)N * N, st.s);
+        st.sync();
+    });
+}
+
+TKA_API tka_status tka_symmetrize(const int32_t* idx, const double* val, int32_t N, int32_t K, int32_t* row_ptr,
+                                  int32_t* col, double* sval, int64_t* nnz)
+{
+    return guard([&] {
+        require(N > 0 && K > 0 && idx && val && row_ptr && col && sval, TKA_E_WRONG_PARAMETER, "bad arguments");
+        for (size_t e = 0; e < (size_t)N * K; e++)
+            require(idx[e] >= 0 && idx[e] < N, TKA_E_WRONG_PARAMETER, "neighbour index out of range");
+        Stream st;
+        DevBuf<int> di;
+        DevBuf<double> dv;
+        upload(di, idx, (size_t)N * K, st.s);
+        upload(dv, val, (size_t)N * K, st.s);
+        Csr P;
+        symmetrize_knn_device(di.get(), dv.get(), N, K, P, nullptr, st.s);
+        download(row_ptr, P.row.get(), (size_t)N + 1, st.s);
+        download(col, P.col.get(), (size_t)P.nnz, st.s);
+        download(sval, P.val.get(), (size_t)P.nnz, st.s);
+        st.sync();
+        if (nnz) *nnz = P.nnz;
+    });
+}
+
+TKA_API tka_status tka_build_p(const double* X, int32_t N, int32_t D, double perplexity, int32_t knn_k,
+                               int32_t dist_dtype, int32_t* row_ptr, int32_t* col, double* val, int64_t* nnz)
+{
+    return guard([&] {
+        require(N > 1 && D > 0 && X && row_ptr && col && val, TKA_E_WRONG_PARAMETER, "bad arguments");
+        require(perplexity > 0.0 && perplexity <= (N - 1) / 3.0, TKA_E_WRONG_PARAMETER,
+                "perplexity must lie in (0, (N-1)/3]");
+        const int K = knn_k > 0 ? knn_k : (int)(3 * perplexity);
+        require(K >= 1 && K <= N - 1, TKA_E_WRONG_PARAMETER, "K must lie in [1, N-1]");
+        check_finite(X, (size_t)N * D, "input data");
+        Stream st;
+        hipStream_t s = st.s;
+        DevBuf<double> x;
+        upload(x, X, (size_t)N * D, s);
+        normalize_device(x.get(), N, D, nullptr, s);
+        DevBuf<int> idx((size_t)N * K);
+        DevBuf<double> dist((size_t)N * K), pv((size_t)N * K);
+        KnnStats ks;
+        knn_device(x.get(), N, D, K, dist_dtype, 0, idx.get(), dist.get(), &ks, nullptr, s);
+        perplexity_knn_device(dist.get(), N, K, perplexity, pv.get(), nullptr, nullptr, nullptr, s);
+        Csr P;
+        symmetrize_knn_device(idx.get(), pv.get(), N, K, P, nullptr, s);
+        normalize_sum_device(P.val.get(), P.nnz, nullptr, s);
+        download(row_ptr, P.row.get(), (size_t)N + 1, s);
+        download(col, P.col.get(), (size_t)P.nnz, s);
+        download(val, P.val.get(), (size_t)P.nnz, s);
+        st.sync();
+        if (nnz) *nnz = P.nnz;
+    });
+}
+
+static void check_csr(const int32_t* row_ptr, const int32_t* col, int N)
+{
+    require(row_ptr[0] == 0, TKA_E_WRONG_PARAMETER, "row_ptr[0] must be 0");
+    for (int i = 0; i < N; i++) require(row_ptr[i + 1] >= row_ptr[i], TKA_E_WRONG_PARAMETER, "row_ptr not monotone");
+    for (int64_t e = 0; e < row_ptr[N]; e++)
+        require(col[e] >= 0 && col[e] < N, TKA_E_WRONG_PARAMETER, "column index out of range");
+}
+
+TKA_API tka_status tka_gradient(const double* Y, int32_t N, int32_t no_dims, const int32_t* row_ptr, const int32_t* col,
+                                const double* val, double theta, int32_t precision, double* dY, double* sum_q)
+{
+    return guard([&] {
+        require(N >= 2 && no_dims >= 1 && no_dims <= 3 && Y && row_ptr && col && val && dY, TKA_E_WRONG_PARAMETER,
+                "bad arguments");
+        require(theta >= 0.0, TKA_E_WRONG_PARAMETER, "theta must be non-negative");
+        check_csr(row_ptr, col, N);
+        check_finite(Y, (size_t)N * no_dims, "Y");
+        if (precision == TKA_FP64)
+            gradient_impl<double>(Y, N, no_dims, row_ptr, col, val, theta, dY, sum_q);
+        else
+            gradient_impl<float>(Y, N, no_dims, row_ptr, col, val, theta, dY, sum_q);
+    });
+}
+
+TKA_API tka_status tka_gradient_dense(const double* P, const double* Y, int32_t N, int32_t no_dims, int32_t dd_mode,
+                                      int32_t precision, double* dY, double* sum_q)
+{
+    return guard([&] {
+        require(N >= 2 && no_dims >= 1 && no_dims <= 3 && P && Y && dY, TKA_E_WRONG_PARAMETER, "bad arguments");
+        if (precision == TKA_FP64)
+            gradient_dense_impl<double>(P, Y, N, no_dims, dd_mode, dY, sum_q);
+        else
+            gradient_dense_impl<float>(P, Y, N, no_dims, dd_mode, dY, sum_q);
+    });
+}
+
+TKA_API tka_status tka_kl(const double* Y, int32_t N, int32_t no_dims, const int32_t* row_ptr, const int32_t* col,
+                          const double* val, double theta, int32_t kl_mode, double* kl)
+{
+    return guard([&] {
+        require(N >= 2 && no_dims >= 1 && no_dims <= 3 && Y && row_ptr && col && val && kl, TKA_E_WRONG_PARAMETER,
+                "bad arguments");
+        check_csr(row_ptr, col, N);
+        MemBudget mb;
+        Stream st;
+        hipStream_t s = st.s;
+        const size_t nd = (size_t)N * no_dims;
+        DevBuf<double> Yd, vd;
+        DevBuf<int> row, colv;
+        upload(Yd, Y, nd, s);
+        upload(row, row_ptr, (size_t)N + 1, s);
+        upload(colv, col, (size_t)row_ptr[N], s);
+        upload(vd, val, (size_t)row_ptr[N], s);
+        if (kl_mode == TKA_KL_AS_REFERENCE && theta > 0.0) {
+            require(no_dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut KL needs 2 dimensions");
+            const int nz = bh_zblocks(N);
+            DevBuf<double> zpart(nz), neg(nd), box((size_t)kBoxStride * bbox_blocks(N));
+            bbox_partials_device<double>(Yd.get(), N, box.get(), s);
+            BhTree<double> tree;
+            tree.alloc(N, &mb);
+            bh_build<double>(Yd.get(), N, box.get(), bbox_blocks(N), tree, s);
+            bh_repulsion<double>(tree, Yd.get(), theta, neg.get(), zpart.get(), nullptr, s);
+            const double Z = sum_device(zpart.get(), nz, &mb, s);
+            *kl = kl_edges_reference_device<double>(no_dims, Yd.get(), row.get(), colv.get(), vd.get(), 1.0, Z, N, &mb, s);
+        } else {
+            const double Z = exact_z_device<double>(no_dims, Yd.get(), N, &mb, s);
+            *kl = kl_edges_exact_device<double>(no_dims, Yd.get(), row.get(), colv.get(), vd.get(), 1.0, Z, N, &mb, s);
+        }
+    });
+}
+
+TKA_API tka_status tka_kl_dense(const double* P, const double* Y, int32_t N, int32_t no_dims, int32_t dd_mode,
+                                double* kl)
+{
+    return guard([&] {
+        require(N >= 2 && no_dims >= 1 && no_dims <= 3 && P && Y && kl, TKA_E_WRONG_PARAMETER, "bad arguments");
+        MemBudget mb;
+        Stream st;
+        DevBuf<double> Yd, Pd;
+        upload(Yd, Y, (size_t)N * no_dims, st.s);
+        upload(Pd, P, (size_t)N * N, st.s);
+        *kl = kl_dense_device<double>(no_dims, Yd.get(), Pd.get(), 1.0, N, dd_mode, &mb, st.s);
+    });
+}
+
+TKA_API tka_status tka_update(const double* dY, double* uY, double* gains, double* Y, int32_t N, int32_t no_dims,
+                              double momentum, double eta, int32_t precision)
+{
+    return guard([&] {
+        require(N >= 1 && no_dims >= 1 && no_dims <= 3 && dY && uY && gains && Y, TKA_E_WRONG_PARAMETER,
+                "bad arguments");
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            Stream st;
+            hipStream_t s = st.s;
+            const size_t nd = (size_t)N * no_dims;
+            DevBuf<double> t0, t1, t2, t3;
+            DevBuf<T> d(nd), u(nd), g(nd), y(nd), yn(nd), zero(nd);
+            upload(t0, dY, nd, s);
+            upload(t1, uY, nd, s);
+            upload(t2, gains, nd, s);
+            upload(t3, Y, nd, s);
+            cast_device<double, T>(t0.get(), d.get(), nd, s);
+            cast_device<double, T>(t1.get(), u.get(), nd, s);
+            cast_device<double, T>(t2.get(), g.get(), nd, s);
+            cast_device<double, T>(t3.get(), y.get(), nd, s);
+            fill_device(zero.get(), nd, 0.0, s);
+            DevBuf<double> one(1), box((size_t)kBoxStride * tsne_step_blocks(N));
+            fill_device(one.get(), 1, 1.0, s);
+            StepParams sp{momentum, eta, 1.0};
+            const int nb = tsne_step_device<T>(no_dims, y.get(), yn.get(), u.get(), g.get(), zero.get(), d.get(),
+                                               nullptr, nullptr, nullptr, one.get(), 1, N, sp, box.get(), nullptr, s);
+            center_device<T>(no_dims, yn.get(), N, box.get(), nb, s);
+            cast_device<T, double>(u.get(), t1.get(), nd, s);
+            cast_device<T, double>(g.get(), t2.get(), nd, s);
+            cast_device<T, double>(yn.get(), t3.get(), nd, s);
+            download(uY, t1.get(), nd, s);
+            download(gains, t2.get(), nd, s);
+            download(Y, t3.get(), nd, s);
+            st.sync();
+        };
+        if (precision == TKA_FP64)
+            run(double{});
+        else
+            run(float{});
+    });
+}
+
+TKA_API tka_status tka_bh_tree(const double* Y, int32_t N, int32_t precision, int32_t* n_nodes, int32_t* level,
+                               int32_t* count, int32_t* leaf_point, int32_t* skip, double* com_x, double* com_y,
+                               double* root_box)
+{
+    return guard([&] {
+        require(N >= 2 && Y && n_nodes, TKA_E_WRONG_PARAMETER, "bad arguments");
+        check_finite(Y, (size_t)N * 2, "Y");
+        auto run = [&](auto tag) {
+            using T = decltype(tag);
+            using T4 = typename Vec4<T>::type;
+            MemBudget mb;
+            Stream st;
+            hipStream_t s = st.s;
+            DevBuf<double> tmp;
+            DevBuf<T> y((size_t)N * 2, &mb);
+            upload(tmp, Y, (size_t)N * 2, s);
+            cast_device<double, T>(tmp.get(), y.get(), (size_t)N * 2, s);
+            DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
+            bbox_partials_device<T>(y.get(), N, box.get(), s);
+            BhTree<T> tree;
+            tree.alloc(N, &mb);
+            bh_build<T>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
+            uint32_t total = 0;
+            RootBox rb;
+            download(&total, tree.pre.get() + N, 1, s);
+            download(&rb, tree.root.get(), 1, s);
+            st.sync();
+            std::vector<T4> nodes(total);
+            std::vector<int> aux(total);
+            download(nodes.data(), tree.nodes.get(), total, s);
+            download(aux.data(), tree.aux.get(), total, s);
+            st.sync();
+            *n_nodes = (int32_t)total;
+            for (uint32_t t = 0; t < total; t++) {
+                const bool leaf = nodes[t].w < (T)0;
+                if (level) level[t] = leaf ? -1 : (int)std::lround(std::log2(rb.size0 / std::sqrt((double)nodes[t].w)));
+                if (count) count[t] = (int)std::lround((double)nodes[t].z);
+                if (leaf_point) leaf_point[t] = leaf ? aux[t] : -1;
+                if (skip) skip[t] = leaf ? (int)t + 1 : aux[t];
+                if (com_x) com_x[t] = (double)nodes[t].x;
+                if (com_y) com_y[t] = (double)nodes[t].y;
+            }
+            if (root_box) {
+                root_box[0] = rb.mean_x;
+                root_box[1] = rb.mean_y;
+                root_box[2] = rb.hw;
+                root_box[3] = rb.hh;
+            }
+        };
+        if (precision == TKA_FP64)
+            run(double{});
+        else
+            run(float{});
+    });
+}
+
+TKA_API tka_status tka_gradient_dev(const float* Y, int32_t N, int32_t no_dims, const int32_t* row_ptr,
+                                    const int32_t* col, const float* val, double theta, float* dY, double* sum_q,
+                                    void* stream)
+{
+    return guard([&] {
+        require(N >= 2 && no_dims >= 1 && no_dims <= 3 && Y && row_ptr && col && val && dY, TKA_E_WRONG_PARAMETER,
+                "bad arguments");
+        require(theta >= 0.0, TKA_E_WRONG_PARAMETER, "theta must be non-negative");
+        MemBudget mb;
+        Stream st(stream);
+        hipStream_t s = st.s;
+        const size_t nd = (size_t)N * no_dims;
+        DevBuf<float> neg(nd);
+        int nz;
+        DevBuf<double> zpart;
+        if (theta > 0.0) {
+            require(no_dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut gradient needs 2 dimensions");
+            // the tree build centres its input: work on a copy
+            DevBuf<float> y(nd);
+            TKA_HIP(hipMemcpyAsync(y.get(), Y, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
+            nz = bh_zblocks(N);
+            zpart.alloc(nz);
+            DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
+            bbox_partials_device<float>(y.get(), N, box.get(), s);
+            BhTree<float> tree;
+            tree.alloc(N, &mb);
+            bh_build<float>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
+            bh_repulsion<float>(tree, y.get(), theta, neg.get(), zpart.get(), nullptr, s);
+            gradient_combine_device<float>(no_dims, y.get(), neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(),
+                                           nz, N, dY, s);
+            const double z = sum_device(zpart.get(), nz, &mb, s);
+            if (sum_q) *sum_q = z;
+        } else {
+            nz = exact_zblocks(N);
+            zpart.alloc(nz);
+            exact_forces_device<float>(no_dims, Y, N, TKA_DD_TEXTBOOK, nullptr, 1.0, neg.get(), nullptr, zpart.get(), s);
+            gradient_combine_device<float>(no_dims, Y, neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(), nz, N,
+                                           dY, s);
+            const double z = sum_device(zpart.get(), nz, &mb, s);
+            if (sum_q) *sum_q = z;
+        }
+    });
+}
+
+TKA_API tka_status tka_kl_dev(const float* Y, int32_t N, int32_t no_dims, const int32_t* row_ptr, const int32_t* col,
+                              const float* val, double theta, double* kl, double* sum_q, void* stream)
+{
+    return guard([&] {
+        require(N >= 2 && no_dims >= 1 && no_dims <= 3 && Y && row_ptr && col && val && kl, TKA_E_WRONG_PARAMETER,
+                "bad arguments");
+        MemBudget mb;
+        Stream st(stream);
+        hipStream_t s = st.s;
+        double Z;
+        if (theta > 0.0) {
+            require(no_dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut KL needs 2 dimensions");
+            const size_t nd = (size_t)N * 2;
+            DevBuf<float> y(nd), neg(nd);
+            TKA_HIP(hipMemcpyAsync(y.get(), Y, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
+            const int nz = bh_zblocks(N);
+            DevBuf<double> zpart(nz), box((size_t)kBoxStride * bbox_blocks(N));
+            bbox_partials_device<float>(y.get(), N, box.get(), s);
+            BhTree<float> tree;
+            tree.alloc(N, &mb);
+            bh_build<float>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
+            bh_repulsion<float>(tree, y.get(), theta, neg.get(), zpart.get(), nullptr, s);
+            Z = sum_device(zpart.get(), nz, &mb, s);
+        } else {
+            Z = exact_z_device<float>(no_dims, Y, N, &mb, s);
+        }
+        *kl = kl_edges_exact_device<float>(no_dims, Y, row_ptr, col, val, 1.0, Z, N, &mb, s);
+        if (sum_q) *sum_q = Z;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,572 @@
+// Attractive forces + optimiser step (fused), exact repulsion, KL.
+#include "forces.hpp"
+
+#include <cfloat>
+
+namespace tka {
+namespace {
+
+constexpr int kG = 16;                 // lanes per CSR row
+constexpr int kRowsPerBlock = 256 / kG;
+
+template <class T>
+TKA_DEV T sgn(T x)
+{
+    return x == (T)0 ? (T)0 : (x < (T)0 ? (T)-1 : (T)1);  // tsne.hpp:53-56
+}
+
+// Sum over the kG lanes of a row group (lanes g*kG .. g*kG+kG-1).
+template <class T>
+TKA_DEV T group_sum(T v)
+{
+#pragma unroll
+    for (int o = kG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+TKA_DEV double z_total(const double* __restrict__ zpart, int nz, double* sh)
+{
+    double z = 0.0;
+    for (int b = threadIdx.x; b < nz; b += blockDim.x) z += zpart[b];
+    return block_sum(z, sh);
+}
+
+// Attraction of row i (all kG lanes of the group call this).
+template <class T, int DIMS>
+TKA_DEV void row_attraction(const T* __restrict__ Y, const int* __restrict__ row_ptr, const int* __restrict__ col,
+                            const T* __restrict__ val, T exag, int i, const T (&yi)[DIMS], T (&acc)[DIMS])
+{
+    const int gl = threadIdx.x % kG;
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) acc[d] = (T)0;
+    const int e1 = row_ptr[i + 1];
+    for (int e = row_ptr[i] + gl; e < e1; e += kG) {
+        const int j = col[e];
+        T diff[DIMS];
+        T D = (T)0;
+#pragma unroll
+        for (int d = 0; d < DIMS; d++) {
+            diff[d] = yi[d] - Y[(size_t)j * DIMS + d];
+            D += diff[d] * diff[d];
+        }
+        const T m = (exag * val[e]) / ((T)1 + D);
+#pragma unroll
+        for (int d = 0; d < DIMS; d++) acc[d] += m * diff[d];
+    }
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) acc[d] = group_sum(acc[d]);
+}
+
+template <class T, int DIMS>
+__global__ __launch_bounds__(256) void tsne_step(const T* __restrict__ Y, T* __restrict__ Ynext, T* __restrict__ uY,
+                                                 T* __restrict__ gains, const T* __restrict__ neg,
+                                                 const T* __restrict__ pos_pre, const int* __restrict__ row_ptr,
+                                                 const int* __restrict__ col, const T* __restrict__ val,
+                                                 const double* __restrict__ zpart, int nz, int N, T momentum, T eta,
+                                                 T exag, double* __restrict__ box_part, T* __restrict__ dC_out)
+{
+    __shared__ double sh[4];
+    __shared__ double sbox[7][4];
+    const double Z = z_total(zpart, nz, sh);
+    const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
+    const int i = blockIdx.x * kRowsPerBlock + g;
+    T yi[DIMS], acc[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) {
+        yi[d] = (i < N) ? Y[(size_t)i * DIMS + d] : (T)0;
+        acc[d] = (T)0;
+    }
+    if (row_ptr != nullptr && i < N) row_attraction<T, DIMS>(Y, row_ptr, col, val, exag, i, yi, acc);
+    // statistics of Ynext: sums per dim (<= 3), min/max of the first two
+    double bx[7] = {0.0, 0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    if (gl == 0 && i < N) {
+        T yn[DIMS];
+#pragma unroll
+        for (int d = 0; d < DIMS; d++) {
+            const size_t o = (size_t)i * DIMS + d;
+            const T pos = (row_ptr != nullptr) ? acc[d] : pos_pre[o];
+            const T dC = pos - (T)((double)neg[o] / Z);
+            if (dC_out) dC_out[o] = dC;
+            T gn = gains[o];
+            T u = uY[o];
+            gn = (sgn(dC) != sgn(u)) ? (gn + (T)0.2) : (gn * (T)0.8);
+            if (gn < (T)0.01) gn = (T)0.01;
+            u = momentum * u - eta * gn * dC;
+            gains[o] = gn;
+            uY[o] = u;
+            yn[d] = yi[d] + u;
+            Ynext[o] = yn[d];
+            if (d < 3) bx[d] = (double)yn[d];
+        }
+        bx[3] = bx[5] = (double)yn[0];
+        bx[4] = bx[6] = (double)yn[DIMS > 1 ? 1 : 0];
+    }
+    if (box_part != nullptr) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) bx[k] = wave_sum(bx[k]);
+        bx[3] = wave_min(bx[3]);
+        bx[4] = wave_min(bx[4]);
+        bx[5] = wave_max(bx[5]);
+        bx[6] = wave_max(bx[6]);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0)
+            for (int k = 0; k < 7; k++) sbox[k][w] = bx[k];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double r[7] = {0.0, 0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+            for (int ww = 0; ww < 4; ww++) {
+                for (int k = 0; k < 3; k++) r[k] += sbox[k][ww];
+                r[3] = fmin(r[3], sbox[3][ww]);
+                r[4] = fmin(r[4], sbox[4][ww]);
+                r[5] = fmax(r[5], sbox[5][ww]);
+                r[6] = fmax(r[6], sbox[6][ww]);
+            }
+            double* o = box_part + (size_t)kBoxStride * blockIdx.x;
+            for (int k = 0; k < 7; k++) o[k] = r[k];
+            o[7] = 0.0;
+        }
+    }
+}
+
+// zeroMean (tsne.hpp:488-518) from box partials: Y -= sum/N per dimension.
+template <class T, int DIMS>
+__global__ __launch_bounds__(256) void center_rows(T* __restrict__ Y, int N, const double* __restrict__ part, int nb)
+{
+    __shared__ double sh[4];
+    double mean[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nb; b += 256) v += part[(size_t)kBoxStride * b + d];
+        mean[d] = block_sum(v, sh) / (double)N;
+    }
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < N) {
+#pragma unroll
+        for (int d = 0; d < DIMS; d++) Y[(size_t)i * DIMS + d] = (T)((double)Y[(size_t)i * DIMS + d] - mean[d]);
+    }
+}
+
+template <class T, int DIMS>
+__global__ __launch_bounds__(256) void grad_combine(const T* __restrict__ Y, const T* __restrict__ neg,
+                                                    const T* __restrict__ pos_pre, const int* __restrict__ row_ptr,
+                                                    const int* __restrict__ col, const T* __restrict__ val, T exag,
+                                                    const double* __restrict__ zpart, int nz, int N,
+                                                    T* __restrict__ dC)
+{
+    __shared__ double sh[4];
+    const double Z = z_total(zpart, nz, sh);
+    const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
+    const int i = blockIdx.x * kRowsPerBlock + g;
+    T yi[DIMS], acc[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) {
+        yi[d] = (i < N) ? Y[(size_t)i * DIMS + d] : (T)0;
+        acc[d] = (T)0;
+    }
+    if (row_ptr != nullptr && i < N) row_attraction<T, DIMS>(Y, row_ptr, col, val, exag, i, yi, acc);
+    if (gl == 0 && i < N) {
+#pragma unroll
+        for (int d = 0; d < DIMS; d++) {
+            const size_t o = (size_t)i * DIMS + d;
+            const T pos = (row_ptr != nullptr) ? acc[d] : pos_pre[o];
+            dC[o] = pos - (T)((double)neg[o] / Z);
+        }
+    }
+}
+
+// Exact all-pairs repulsion; one thread per row, columns streamed through LDS.
+template <class T, int DIMS, bool kCompat, bool kDense>
+__global__ __launch_bounds__(256) void exact_forces(const T* __restrict__ Y, int N, const T* __restrict__ P, T exag,
+                                                    T* __restrict__ neg, T* __restrict__ pos,
+                                                    double* __restrict__ zpart)
+{
+    __shared__ T sY[256 * DIMS];
+    __shared__ double sh[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    T yi[DIMS], rep[DIMS], att[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) {
+        yi[d] = (i < N) ? Y[(size_t)i * DIMS + d] : (T)0;
+        rep[d] = att[d] = (T)0;
+    }
+    T z = (T)0;
+    for (int j0 = 0; j0 < N; j0 += 256) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < 256 * DIMS; e += 256) {
+            const size_t src = (size_t)j0 * DIMS + e;
+            sY[e] = (src < (size_t)N * DIMS) ? Y[src] : (T)0;
+        }
+        __syncthreads();
+        const int jn = (N - j0) < 256 ? (N - j0) : 256;
+        if (i < N) {
+            for (int jj = 0; jj < jn; jj++) {
+                const int j = j0 + jj;
+                if (j == i) continue;
+                T diff[DIMS];
+                T D = (T)0;
+#pragma unroll
+                for (int d = 0; d < DIMS; d++) {
+                    diff[d] = yi[d] - sY[jj * DIMS + d];
+                    if (kCompat)
+                        D += yi[d] * sY[jj * DIMS + d];
+                    else
+                        D += diff[d] * diff[d];
+                }
+                if (kCompat) D = (T)-2 * D;
+                const T q = (T)1 / ((T)1 + D);
+                z += q;
+                const T q2 = q * q;
+                T pq = (T)0;
+                if (kDense) pq = exag * P[(size_t)j * N + i] * q;  // P symmetric: P[i][j] == P[j][i]
+#pragma unroll
+                for (int d = 0; d < DIMS; d++) {
+                    rep[d] += q2 * diff[d];
+                    if (kDense) att[d] += pq * diff[d];
+                }
+            }
+        }
+    }
+    if (i < N) {
+#pragma unroll
+        for (int d = 0; d < DIMS; d++) {
+            neg[(size_t)i * DIMS + d] = rep[d];
+            if (kDense) pos[(size_t)i * DIMS + d] = att[d];
+        }
+    }
+    const double zs = block_sum((double)z, sh);
+    if (threadIdx.x == 0) zpart[blockIdx.x] = zs;
+}
+
+// FP64 all-pairs sum of 1/(1+d^2) (textbook) or 1/(1-2<y_i,y_j>) (compat).
+template <class T, int DIMS, bool kCompat>
+__global__ __launch_bounds__(256) void pair_z(const T* __restrict__ Y, int N, double* __restrict__ zpart)
+{
+    __shared__ double sY[256 * DIMS];
+    __shared__ double sh[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double yi[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) yi[d] = (i < N) ? (double)Y[(size_t)i * DIMS + d] : 0.0;
+    double z = 0.0;
+    for (int j0 = 0; j0 < N; j0 += 256) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < 256 * DIMS; e += 256) {
+            const size_t src = (size_t)j0 * DIMS + e;
+            sY[e] = (src < (size_t)N * DIMS) ? (double)Y[src] : 0.0;
+        }
+        __syncthreads();
+        const int jn = (N - j0) < 256 ? (N - j0) : 256;
+        if (i < N) {
+            for (int jj = 0; jj < jn; jj++) {
+                if (j0 + jj == i) continue;
+                double D = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIMS; d++) {
+                    if (kCompat) {
+                        D += yi[d] * sY[jj * DIMS + d];
+                    } else {
+                        const double t = yi[d] - sY[jj * DIMS + d];
+                        D += t * t;
+                    }
+                }
+                if (kCompat) D = -2.0 * D;
+                z += 1.0 / (1.0 + D);
+            }
+        }
+    }
+    const double zs = block_sum(z, sh);
+    if (threadIdx.x == 0) zpart[blockIdx.x] = zs;
+}
+
+// mode 0: p log((p + FLT_MIN)/(q/Z + FLT_MIN))   (tsne.hpp:481-482)
+// mode 1: p log(p Z (1 + d^2)) for p > 0          (exact normaliser)
+template <class T, int DIMS, int kMode>
+__global__ __launch_bounds__(256) void kl_edges(const T* __restrict__ Y, const int* __restrict__ row_ptr,
+                                                const int* __restrict__ col, const T* __restrict__ val, double exag,
+                                                double Z, int N, double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
+    const int i = blockIdx.x * kRowsPerBlock + g;
+    double c = 0.0;
+    if (i < N) {
+        for (int e = row_ptr[i] + gl; e < row_ptr[i + 1]; e += kG) {
+            const int j = col[e];
+            double D = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIMS; d++) {
+                const double t = (double)Y[(size_t)i * DIMS + d] - (double)Y[(size_t)j * DIMS + d];
+                D += t * t;
+            }
+            const double p = exag * (double)val[e];
+            if (kMode == 0) {
+                const double Q = (1.0 / (1.0 + D)) / Z;
+                c += p * log((p + (double)FLT_MIN) / (Q + (double)FLT_MIN));
+            } else if (p > 0.0) {
+                c += p * log(p * Z * (1.0 + D));
+            }
+        }
+    }
+    c = block_sum(c, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = c;
+}
+
+// Dense KL, tsne.hpp:435-446: Q_nm = q_nm / sumQ, Q_nn = DBL_MIN / sumQ.
+template <class T, int DIMS, bool kCompat>
+__global__ __launch_bounds__(256) void kl_dense(const T* __restrict__ Y, const T* __restrict__ P, double exag,
+                                                int N, double sumQ, double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    double c = 0.0;
+    const size_t total = (size_t)N * N;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const int n = (int)(e / N), m = (int)(e % N);
+        double Q;
+        if (n == m) {
+            Q = DBL_MIN / sumQ;
+        } else {
+            double D = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIMS; d++) {
+                const double a = (double)Y[(size_t)n * DIMS + d], b = (double)Y[(size_t)m * DIMS + d];
+                if (kCompat)
+                    D += a * b;
+                else
+                    D += (a - b) * (a - b);
+            }
+            if (kCompat) D = -2.0 * D;
+            Q = (1.0 / (1.0 + D)) / sumQ;
+        }
+        const double p = exag * (double)P[e];
+        c += p * log((p + 1e-9) / (Q + 1e-9));
+    }
+    c = block_sum(c, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(256) void sum_fixed(const double* __restrict__ v, int n, double* __restrict__ out)
+{
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) s += v[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+template <class T, int DIMS, bool kCompat, bool kDense>
+void launch_exact(const T* Y, int N, const T* P, double exag, T* neg, T* pos, double* zpart, hipStream_t s)
+{
+    hipLaunchKernelGGL((exact_forces<T, DIMS, kCompat, kDense>), dim3(exact_zblocks(N)), dim3(256), 0, s, Y, N, P,
+                       (T)exag, neg, pos, zpart);
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T, int DIMS>
+void dispatch_exact(const T* Y, int N, int dd_mode, const T* P, double exag, T* neg, T* pos, double* zpart,
+                    hipStream_t s)
+{
+    const bool compat = dd_mode == TKA_DD_TAPKEE_COMPAT;
+    if (P) {
+        if (compat)
+            launch_exact<T, DIMS, true, true>(Y, N, P, exag, neg, pos, zpart, s);
+        else
+            launch_exact<T, DIMS, false, true>(Y, N, P, exag, neg, pos, zpart, s);
+    } else {
+        if (compat)
+            launch_exact<T, DIMS, true, false>(Y, N, P, exag, neg, pos, zpart, s);
+        else
+            launch_exact<T, DIMS, false, false>(Y, N, P, exag, neg, pos, zpart, s);
+    }
+}
+
+}  // namespace
+
+int tsne_step_blocks(int N) { return (int)ceil_div(N, kRowsPerBlock); }
+
+template <class T>
+void center_device(int dims, T* Y, int N, const double* box_part, int nb, hipStream_t s)
+{
+    const unsigned g = ceil_div(N, 256);
+    switch (dims) {
+        case 1: hipLaunchKernelGGL((center_rows<T, 1>), dim3(g), dim3(256), 0, s, Y, N, box_part, nb); break;
+        case 2: hipLaunchKernelGGL((center_rows<T, 2>), dim3(g), dim3(256), 0, s, Y, N, box_part, nb); break;
+        case 3: hipLaunchKernelGGL((center_rows<T, 3>), dim3(g), dim3(256), 0, s, Y, N, box_part, nb); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T>
+int tsne_step_device(int dims, const T* Y, T* Ynext, T* uY, T* gains, const T* neg, const T* pos_pre,
+                     const int* row_ptr, const int* col, const T* val, const double* zpart, int nz, int N,
+                     StepParams sp, double* box_part, T* dC_out, hipStream_t s)
+{
+    const int nb = tsne_step_blocks(N);
+#define TKA_STEP(DM)                                                                                              \
+    hipLaunchKernelGGL((tsne_step<T, DM>), dim3(nb), dim3(256), 0, s, Y, Ynext, uY, gains, neg, pos_pre, row_ptr, \
+                       col, val, zpart, nz, N, (T)sp.momentum, (T)sp.eta, (T)sp.exag, box_part, dC_out)
+    switch (dims) {
+        case 1: TKA_STEP(1); break;
+        case 2: TKA_STEP(2); break;
+        case 3: TKA_STEP(3); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+#undef TKA_STEP
+    TKA_LAUNCH_CHECK();
+    return nb;
+}
+
+template <class T>
+void gradient_combine_device(int dims, const T* Y, const T* neg, const T* pos_pre, const int* row_ptr,
+                             const int* col, const T* val, double exag, const double* zpart, int nz, int N, T* dC,
+                             hipStream_t s)
+{
+    const int nb = tsne_step_blocks(N);
+#define TKA_GC(DM)                                                                                                    \
+    hipLaunchKernelGGL((grad_combine<T, DM>), dim3(nb), dim3(256), 0, s, Y, neg, pos_pre, row_ptr, col, val, (T)exag, \
+                       zpart, nz, N, dC)
+    switch (dims) {
+        case 1: TKA_GC(1); break;
+        case 2: TKA_GC(2); break;
+        case 3: TKA_GC(3); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+#undef TKA_GC
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T>
+int exact_forces_device(int dims, const T* Y, int N, int dd_mode, const T* P, double exag, T* neg, T* pos,
+                        double* zpart, hipStream_t s)
+{
+    switch (dims) {
+        case 1: dispatch_exact<T, 1>(Y, N, dd_mode, P, exag, neg, pos, zpart, s); break;
+        case 2: dispatch_exact<T, 2>(Y, N, dd_mode, P, exag, neg, pos, zpart, s); break;
+        case 3: dispatch_exact<T, 3>(Y, N, dd_mode, P, exag, neg, pos, zpart, s); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+    return exact_zblocks(N);
+}
+
+double sum_device(const double* v, int n, MemBudget* mb, hipStream_t s)
+{
+    DevBuf<double> out(1, mb);
+    hipLaunchKernelGGL(sum_fixed, dim3(1), dim3(256), 0, s, v, n, out.get());
+    TKA_LAUNCH_CHECK();
+    double h = 0.0;
+    TKA_HIP(hipMemcpyAsync(&h, out.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+    TKA_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+template <class T>
+double kl_edges_reference_device(int dims, const T* Y, const int* row_ptr, const int* col, const T* val, double exag,
+                                 double Z, int N, MemBudget* mb, hipStream_t s)
+{
+    const int nb = tsne_step_blocks(N);
+    DevBuf<double> part(nb, mb);
+#define TKA_KL(DM) \
+    hipLaunchKernelGGL((kl_edges<T, DM, 0>), dim3(nb), dim3(256), 0, s, Y, row_ptr, col, val, exag, Z, N, part.get())
+    switch (dims) {
+        case 1: TKA_KL(1); break;
+        case 2: TKA_KL(2); break;
+        case 3: TKA_KL(3); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+#undef TKA_KL
+    TKA_LAUNCH_CHECK();
+    return sum_device(part.get(), nb, mb, s);
+}
+
+template <class T>
+double exact_z_device(int dims, const T* Y, int N, MemBudget* mb, hipStream_t s)
+{
+    const int nb = exact_zblocks(N);
+    DevBuf<double> part(nb, mb);
+    switch (dims) {
+        case 1: hipLaunchKernelGGL((pair_z<T, 1, false>), dim3(nb), dim3(256), 0, s, Y, N, part.get()); break;
+        case 2: hipLaunchKernelGGL((pair_z<T, 2, false>), dim3(nb), dim3(256), 0, s, Y, N, part.get()); break;
+        case 3: hipLaunchKernelGGL((pair_z<T, 3, false>), dim3(nb), dim3(256), 0, s, Y, N, part.get()); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+    TKA_LAUNCH_CHECK();
+    return sum_device(part.get(), nb, mb, s);
+}
+
+template <class T>
+double kl_edges_exact_device(int dims, const T* Y, const int* row_ptr, const int* col, const T* val, double exag,
+                             double Z, int N, MemBudget* mb, hipStream_t s)
+{
+    const int nb = tsne_step_blocks(N);
+    DevBuf<double> part(nb, mb);
+#define TKA_KL(DM) \
+    hipLaunchKernelGGL((kl_edges<T, DM, 1>), dim3(nb), dim3(256), 0, s, Y, row_ptr, col, val, exag, Z, N, part.get())
+    switch (dims) {
+        case 1: TKA_KL(1); break;
+        case 2: TKA_KL(2); break;
+        case 3: TKA_KL(3); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+#undef TKA_KL
+    TKA_LAUNCH_CHECK();
+    return sum_device(part.get(), nb, mb, s);
+}
+
+template <class T>
+double kl_dense_device(int dims, const T* Y, const T* P, double exag, int N, int dd_mode, MemBudget* mb,
+                       hipStream_t s)
+{
+    const bool compat = dd_mode == TKA_DD_TAPKEE_COMPAT;
+    const int nbz = exact_zblocks(N);
+    DevBuf<double> zp(nbz, mb);
+#define TKA_PZ(DM)                                                                                              \
+    if (compat)                                                                                                 \
+        hipLaunchKernelGGL((pair_z<T, DM, true>), dim3(nbz), dim3(256), 0, s, Y, N, zp.get());                  \
+    else                                                                                                        \
+        hipLaunchKernelGGL((pair_z<T, DM, false>), dim3(nbz), dim3(256), 0, s, Y, N, zp.get());
+    switch (dims) {
+        case 1: TKA_PZ(1); break;
+        case 2: TKA_PZ(2); break;
+        case 3: TKA_PZ(3); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+#undef TKA_PZ
+    TKA_LAUNCH_CHECK();
+    const double sumQ = DBL_MIN + sum_device(zp.get(), nbz, mb, s);
+    const int nb = (int)std::min<size_t>(4096, ceil_div((size_t)N * N, 256));
+    DevBuf<double> part(nb, mb);
+#define TKA_KD(DM)                                                                                                 \
+    if (compat)                                                                                                    \
+        hipLaunchKernelGGL((kl_dense<T, DM, true>), dim3(nb), dim3(256), 0, s, Y, P, exag, N, sumQ, part.get());  \
+    else                                                                                                           \
+        hipLaunchKernelGGL((kl_dense<T, DM, false>), dim3(nb), dim3(256), 0, s, Y, P, exag, N, sumQ, part.get());
+    switch (dims) {
+        case 1: TKA_KD(1); break;
+        case 2: TKA_KD(2); break;
+        case 3: TKA_KD(3); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+#undef TKA_KD
+    TKA_LAUNCH_CHECK();
+    return sum_device(part.get(), nb, mb, s);
+}
+
+#define TKA_INST(T)                                                                                                  \
+    template void center_device<T>(int, T*, int, const double*, int, hipStream_t);                                \
+    template int tsne_step_device<T>(int, const T*, T*, T*, T*, const T*, const T*, const int*, const int*, const T*, \
+                                     const double*, int, int, StepParams, double*, T*, hipStream_t);                 \
+    template void gradient_combine_device<T>(int, const T*, const T*, const T*, const int*, const int*, const T*,    \
+                                             double, const double*, int, int, T*, hipStream_t);                      \
+    template int exact_forces_device<T>(int, const T*, int, int, const T*, double, T*, T*, double*, hipStream_t);    \
+    template double kl_edges_reference_device<T>(int, const T*, const int*, const int*, const T*, double, double,    \
+                                                 int, MemBudget*, hipStream_t);                                       \
+    template double exact_z_device<T>(int, const T*, int, MemBudget*, hipStream_t);                                  \
+    template double kl_edges_exact_device<T>(int, const T*, const int*, const int*, const T*, double, double, int,   \
+                                             MemBudget*, hipStream_t);                                                \
+    template double kl_dense_device<T>(int, const T*, const T*, double, int, int, MemBudget*, hipStream_t);
+TKA_INST(float)
+TKA_INST(double)
+#undef TKA_INST
+
+}  // namespace tka

@@ -1,0 +1,69 @@
+// Gradient pieces other than the Barnes-Hut traversal: the CSR attractive
+// pass fused with the optimiser step, the exact O(N^2) repulsion for
+// theta = 0, and the KL reductions.
+#pragma once
+
+#include "common.hpp"
+
+namespace tka {
+
+struct StepParams {
+    double momentum;
+    double eta;
+    double exag;        // multiplies P (12 while "lying", then 1)
+};
+
+// One optimiser step for all rows (tsne.hpp:152-167, attraction
+// quadtree.hpp:357-384, combine tsne.hpp:344-347):
+//   pos_i = sum_e exag*P_e * (y_i - y_col)/(1 + |y_i - y_col|^2)   (CSR), or
+//           pos_pre (dense theta = 0 form) when row_ptr == nullptr
+//   dC    = pos - neg / Z,  Z = sum(zpart[0..nz))
+//   gains, uY, Ynext = Y + uY.
+// Writes box partials of Ynext (kBoxStride doubles per block) and returns the number
+// of partial blocks.  dC_out (optional) receives dC.
+template <class T>
+int tsne_step_device(int dims, const T* Y, T* Ynext, T* uY, T* gains, const T* neg, const T* pos_pre,
+                     const int* row_ptr, const int* col, const T* val, const double* zpart, int nz, int N,
+                     StepParams sp, double* box_part, T* dC_out, hipStream_t s);
+int tsne_step_blocks(int N);
+
+// zeroMean of Y (N x dims) from the box partials of the step that produced it.
+template <class T>
+void center_device(int dims, T* Y, int N, const double* box_part, int nb, hipStream_t s);
+
+// Gradient only (no update): dC = pos - neg/Z.
+template <class T>
+void gradient_combine_device(int dims, const T* Y, const T* neg, const T* pos_pre, const int* row_ptr,
+                             const int* col, const T* val, double exag, const double* zpart, int nz, int N, T* dC,
+                             hipStream_t s);
+
+// Exact repulsion over all pairs (tsne.hpp:353-407): neg_i = sum_j q^2 (y_i -
+// y_j), zpart = per-block sums of q.  With a dense symmetric P (N x N) also
+// pos_i = sum_j exag*P_ij q (y_i - y_j).  dd_mode as tka_dd_mode.
+template <class T>
+int exact_forces_device(int dims, const T* Y, int N, int dd_mode, const T* P, double exag, T* neg, T* pos,
+                        double* zpart, hipStream_t s);
+inline int exact_zblocks(int N) { return (int)ceil_div(N, 256); }
+
+// Sum of a double array (deterministic order); result copied to the host.
+double sum_device(const double* v, int n, MemBudget* mb, hipStream_t s);
+
+// KL over CSR edges as tsne.hpp:454-486: sum p*log((p + FLT_MIN)/(q/Z + FLT_MIN)),
+// p = exag*val.  Z given.
+template <class T>
+double kl_edges_reference_device(int dims, const T* Y, const int* row_ptr, const int* col, const T* val,
+                                 double exag, double Z, int N, MemBudget* mb, hipStream_t s);
+
+// Exact Z (FP64, all pairs) and KL over CSR edges with it: sum p log(p Z (1+d^2)).
+template <class T>
+double exact_z_device(int dims, const T* Y, int N, MemBudget* mb, hipStream_t s);
+template <class T>
+double kl_edges_exact_device(int dims, const T* Y, const int* row_ptr, const int* col, const T* val, double exag,
+                             double Z, int N, MemBudget* mb, hipStream_t s);
+
+// Dense-P KL as tsne.hpp:409-452 (FP64).
+template <class T>
+double kl_dense_device(int dims, const T* Y, const T* P, double exag, int N, int dd_mode, MemBudget* mb,
+                       hipStream_t s);
+
+}  // namespace tka

@@ -1,0 +1,239 @@
+"""GPU parity: every engine step against the FP64 oracle on the same inputs.
+
+Tolerances (stated per test): integer/index outputs bit-exact; FP64 kernels
+to a few ulps of accumulated rounding; FP32 kernels relative L2 <= 1e-5
+(exact sums) or 1e-4 (Barnes-Hut, same tree).
+"""
+import numpy as np
+import pytest
+
+from tests.conftest import rng
+
+pytestmark = pytest.mark.gpu
+
+
+def mixture(n=600, d=20, k=6, seed=0):
+    g = rng(seed)
+    c = g.normal(0, 5, size=(k, d))
+    return c[np.arange(n) % k] + g.normal(size=(n, d))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(np.asarray(b)), 1e-300))
+
+
+def csr_from_oracle(oracle, X, perp):
+    row, col, val, Xn = oracle.build_p(X, perp, knn_mode=1)
+    return row, col, val, Xn
+
+
+# ---------------------------------------------------------------- normalise
+@pytest.mark.parametrize("shape", [(500, 20), (1001, 50), (64, 784)])
+def test_normalize_bit_exact(engine, oracle, shape):
+    X = rng(1).normal(size=shape) * 3 + 1
+    np.testing.assert_array_equal(engine.normalize(X), oracle.normalize(X))
+
+
+# ---------------------------------------------------------------- kNN
+@pytest.mark.parametrize("dtype", ["fp32", "bf16"])
+@pytest.mark.parametrize("n,d,K", [(700, 20, 30), (1500, 50, 90), (300, 784, 20), (129, 3, 128)])
+def test_knn_bit_exact(engine, oracle, dtype, n, d, K):
+    X = oracle.normalize(mixture(n, d))
+    idx, dist, fails = engine.knn(X, K, dist_dtype=dtype)
+    ridx, rdist = oracle.knn_brute(X, K)
+    np.testing.assert_array_equal(idx, ridx)
+    np.testing.assert_array_equal(dist, rdist)
+
+
+def test_knn_ties_broken_by_index(engine, oracle):
+    # duplicated points and a lattice: many exactly equal distances
+    g = np.stack(np.meshgrid(np.arange(12.0), np.arange(12.0)), -1).reshape(-1, 2)
+    X = np.concatenate([g, g[:40]])
+    for dtype in ("fp32", "bf16"):
+        idx, dist, _ = engine.knn(X, 10, dist_dtype=dtype)
+        ridx, rdist = oracle.knn_brute(X, 10)
+        np.testing.assert_array_equal(idx, ridx)
+        np.testing.assert_array_equal(dist, rdist)
+
+
+# ---------------------------------------------------------------- perplexity
+def test_perplexity_rows(engine, oracle):
+    X = oracle.normalize(mixture(800, 20))
+    _, dist = oracle.knn_brute(X, 90)
+    val, beta, iters, unconv = engine.perplexity(dist, 30.0)
+    rval, rbeta, riters, runconv = oracle.perplexity_sparse(dist, 30.0)
+    # same bisection path: identical iteration counts and betas (dyadic updates)
+    np.testing.assert_array_equal(iters, riters)
+    np.testing.assert_array_equal(beta, rbeta)
+    assert unconv == runconv
+    np.testing.assert_allclose(val, rval, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_perplexity_dense(engine, oracle, compat):
+    X = oracle.normalize(mixture(300, 10))
+    P = engine.perplexity_dense(X, 20.0, dd_mode="tapkee_compat" if compat else "textbook")
+    RP, _, _ = oracle.perplexity_dense(oracle.sqdist(X, compat), 20.0)
+    np.testing.assert_allclose(P, RP, rtol=1e-11, atol=1e-300)
+
+
+# ---------------------------------------------------------------- symmetrise
+def test_symmetrize_matches_reference_as_sets(engine, oracle):
+    X = oracle.normalize(mixture(900, 16))
+    idx, dist = oracle.knn_brute(X, 45)
+    val, *_ = oracle.perplexity_sparse(dist, 15.0)
+    row, col, sval = engine.symmetrize(idx, val)
+    rrow, rcol, rsval = oracle.symmetrize(idx, val)
+    np.testing.assert_array_equal(row, rrow)  # same counts per row
+    for i in range(len(row) - 1):
+        a = slice(row[i], row[i + 1])
+        o = np.argsort(rcol[a], kind="stable")
+        np.testing.assert_array_equal(col[a], rcol[a][o])
+        np.testing.assert_array_equal(sval[a], rsval[a][o])
+
+
+# ---------------------------------------------------------------- gradient
+@pytest.fixture(scope="module")
+def small_problem(oracle):
+    X = mixture(700, 20)
+    row, col, val, _ = csr_from_oracle(oracle, X, 20.0)
+    Y = rng(7).normal(size=(700, 2)) * 3.0
+    return row, col, val, Y
+
+
+def test_gradient_exact_fp64(engine, oracle, small_problem):
+    row, col, val, Y = small_problem
+    dY, z = engine.gradient(Y, row, col, val, 0.0, precision="fp64")
+    rdY, rz = oracle.gradient_exact_sparse(row, col, val, Y)
+    assert rel_l2(dY, rdY) < 1e-12
+    assert abs(z - rz) / rz < 1e-13
+
+
+def test_gradient_exact_fp32(engine, oracle, small_problem):
+    row, col, val, Y = small_problem
+    dY, z = engine.gradient(Y, row, col, val, 0.0, precision="fp32")
+    rdY, rz = oracle.gradient_exact_sparse(row, col, val, Y)
+    assert rel_l2(dY, rdY) < 1e-5
+    assert abs(z - rz) / rz < 1e-5
+
+
+@pytest.mark.parametrize("theta", [0.2, 0.5, 0.8])
+def test_gradient_bh_fp64_matches_reference_tree(engine, oracle, small_problem, theta):
+    row, col, val, Y = small_problem
+    Y = Y - Y.mean(0)
+    dY, z = engine.gradient(Y, row, col, val, theta, precision="fp64")
+    rdY, rz = oracle.gradient_bh(row, col, val, Y, theta)
+    assert rel_l2(dY, rdY) < 1e-9
+    assert abs(z - rz) / rz < 1e-10
+
+
+def test_gradient_bh_fp32(engine, oracle, small_problem):
+    row, col, val, Y = small_problem
+    Y = Y - Y.mean(0)
+    dY, z = engine.gradient(Y, row, col, val, 0.5, precision="fp32")
+    rdY, rz = oracle.gradient_bh(row, col, val, Y, 0.5)
+    assert rel_l2(dY, rdY) < 1e-4
+    assert abs(z - rz) / rz < 1e-5
+
+
+def test_gradient_bh_theta0_is_exact(engine, oracle, small_problem):
+    row, col, val, Y = small_problem
+    # theta = 0 through the tree kernel: only leaves are accepted
+    dY_tree, z_tree = engine.gradient(Y, row, col, val, 1e-300, precision="fp64")
+    rdY, rz = oracle.gradient_exact_sparse(row, col, val, Y)
+    assert rel_l2(dY_tree, rdY) < 1e-12
+    assert abs(z_tree - rz) / rz < 1e-12
+
+
+@pytest.mark.parametrize("compat", [False, True])
+def test_gradient_dense(engine, oracle, compat):
+    X = oracle.normalize(mixture(250, 10))
+    P, _, _ = oracle.perplexity_dense(oracle.sqdist(X), 15.0)
+    P = P + P.T
+    P /= P.sum()
+    Y = rng(3).normal(size=(250, 2)) * (0.05 if compat else 2.0)
+    mode = "tapkee_compat" if compat else "textbook"
+    dY, z = engine.gradient_dense(P, Y, dd_mode=mode, precision="fp64")
+    rdY, rz = oracle.gradient_exact_dense(P, Y, compat=compat)
+    assert rel_l2(dY, rdY) < 1e-11
+    assert abs(z - rz) / abs(rz) < 1e-12
+
+
+# ---------------------------------------------------------------- tree
+def test_bh_tree_matches_reference_tree(engine, oracle):
+    Y = rng(11).normal(size=(400, 2))
+    Y -= Y.mean(0)
+    t = engine.bh_tree(Y)
+    r = oracle.quadtree(Y)
+    # compress the reference tree: drop empty nodes and one-child chain nodes
+    keep = (r["cum"] > 0) & ((r["leaf"] == 1) | (r["nchild"] >= 2))
+    rl = r["level"][keep]
+    rc = r["cum"][keep]
+    rleaf = r["leaf"][keep] == 1
+    # pre-order sequences agree node for node
+    assert len(rl) == len(t["count"])
+    np.testing.assert_array_equal(rc, t["count"])
+    np.testing.assert_array_equal(rleaf, t["leaf_point"] >= 0)
+    inner = ~rleaf
+    np.testing.assert_array_equal(rl[inner], t["level"][inner])
+    np.testing.assert_array_equal(r["index"][keep][rleaf], t["leaf_point"][rleaf])
+    np.testing.assert_allclose(t["comx"], r["comx"][keep], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(t["comy"], r["comy"][keep], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(t["root_box"][2:], r["root_box"][2:], rtol=1e-15)
+
+
+# ---------------------------------------------------------------- KL
+def test_kl_variants(engine, oracle, small_problem):
+    row, col, val, Y = small_problem
+    Y = Y - Y.mean(0)
+    assert abs(engine.kl(Y, row, col, val, mode="exact") -
+               oracle.kl_exact_sparse(row, col, val, Y)) < 1e-11
+    a = engine.kl(Y, row, col, val, theta=0.5, mode="reference")
+    b = oracle.kl_bh(row, col, val, Y, 0.5)
+    assert abs(a - b) < 1e-10 * max(1.0, abs(b))
+
+
+def test_kl_dense(engine, oracle):
+    X = oracle.normalize(mixture(200, 8))
+    P, _, _ = oracle.perplexity_dense(oracle.sqdist(X), 12.0)
+    P = (P + P.T) / (P + P.T).sum()
+    Y = rng(5).normal(size=(200, 2))
+    assert abs(engine.kl_dense(P, Y) - oracle.kl_exact_dense(P, Y)) < 1e-11
+
+
+# ---------------------------------------------------------------- update
+def test_update_recurrence(engine, oracle):
+    g = rng(4)
+    n = 300
+    dY = g.normal(size=(n, 2))
+    dY[::7] = 0.0
+    uY = g.normal(size=(n, 2))
+    uY[::5] = 0.0
+    gains = g.uniform(0.005, 2, size=(n, 2))
+    Y = g.normal(size=(n, 2))
+    a = engine.update(dY, uY, gains, Y, 0.5, precision="fp64")
+    b = oracle.update(dY, uY, gains, Y, 0.5)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(x, y, rtol=1e-14, atol=1e-14)
+
+
+# ---------------------------------------------------------------- full run
+def test_y0_replays_std_rand(engine, oracle):
+    X = mixture(300, 10)
+    # max_iter = 0: the output is the (centred) initial solution
+    oracle.srand(12345)
+    Yr, _ = oracle.tsne_run(X, perplexity=10.0, theta=0.5, max_iter=0)
+    Y, _ = engine.tsne(X, perplexity=10.0, theta=0.5, max_iter=0, seed=12345, loop_precision="fp64")
+    np.testing.assert_allclose(Y, Yr - Yr.mean(0), rtol=0, atol=1e-18)
+
+
+@pytest.mark.parametrize("theta", [0.5, 0.0])
+def test_full_run_kl_within_one_percent(engine, oracle, theta):
+    X = mixture(500, 20)
+    y0 = rng(9).normal(size=(500, 2)) * 1e-4
+    Yr, ir = oracle.tsne_run(X, perplexity=15.0, theta=theta, max_iter=300, knn_mode=1, y0=y0)
+    for prec in ("fp64", "fp32"):
+        Y, st = engine.tsne(X, perplexity=15.0, theta=theta, max_iter=300, y0=y0, loop_precision=prec,
+                            dist_dtype="fp32")
+        kr = ir["kl"][-1]
+        assert abs(st["kl"][-1] - kr) / kr < 0.01, (prec, st["kl"], ir["kl"])
