@@ -2,6 +2,11 @@
 
 The library is built in-tree by `python -m tapkee_amd.build`.  Loading fails
 loudly when it is missing: there is no CPU fallback for any entry point.
+
+One HIP runtime per process: PyTorch ships its own libamdhip64.so.7 (same
+soname as ROCm's).  A process that uses torch must import it before the
+engine loads, so both bind the same runtime; `tapkee_amd.autograd` checks
+this.  Processes without torch use the system ROCm runtime.
 """
 from __future__ import annotations
 
@@ -108,6 +113,7 @@ _ERRORS = {c.status: c for c in (WrongParameterError, NoDataError, NotEnoughMemo
                                  UnsupportedMethodError, DeviceError)}
 
 _lib = None
+loaded_after_torch = None  # whether torch's HIP runtime was already loaded when the engine loaded
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i32p = C.POINTER(C.c_int32)
@@ -125,9 +131,11 @@ SYMBOLS = (
 
 def lib():
     """Load the engine; raises if the HIP library has not been built."""
-    global _lib
+    global _lib, loaded_after_torch
     if _lib is not None:
         return _lib
+    import sys
+    loaded_after_torch = "torch" in sys.modules
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m tapkee_amd.build` "
                           "(the t-SNE engine has no CPU fallback)")

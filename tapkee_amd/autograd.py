@@ -20,6 +20,10 @@ import torch
 from . import _lib
 from . import build_p as _build_p
 
+if _lib._lib is not None and not _lib.loaded_after_torch:
+    raise ImportError("the engine was loaded before torch: import torch (or tapkee_amd.autograd) first so "
+                      "that both use one HIP runtime")
+
 
 def _stream(t: torch.Tensor) -> int:
     return torch.cuda.current_stream(t.device).cuda_stream

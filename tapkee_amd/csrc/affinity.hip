@@ -99,12 +99,12 @@ __global__ __launch_bounds__(256) void sqdist_dense(const double* __restrict__ X
         const double* b = X + (size_t)m * D;
         double acc = 0.0;
         if (compat) {
-            for (int d = 0; d < D; d++) acc = __dadd_rn(acc, __dmul_rn(a[d], b[d]));
+            for (int d = 0; d < D; d++) acc = dadd_sep(acc, dmul_sep(a[d], b[d]));
             acc = -2.0 * acc;
         } else {
             for (int d = 0; d < D; d++) {
-                const double t = __dadd_rn(a[d], -b[d]);
-                acc = __dadd_rn(acc, __dmul_rn(t, t));
+                const double t = dadd_sep(a[d], -b[d]);
+                acc = dadd_sep(acc, dmul_sep(t, t));
             }
         }
         DD[e] = acc;

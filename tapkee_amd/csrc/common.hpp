@@ -121,6 +121,20 @@ inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b
 constexpr int kWave = 64;
 constexpr int kBoxStride = 8;  // doubles per block of Y statistics
 
+// FP64 multiply and add that the compiler never fuses into an FMA, so a sum
+// of products rounds exactly like a scalar loop compiled without contraction
+// (the reference's euclidean_distance, vptree.hpp:95-101).
+TKA_DEV double dmul_sep(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a * b;
+}
+TKA_DEV double dadd_sep(double a, double b)
+{
+#pragma clang fp contract(off)
+    return a + b;
+}
+
 template <class T>
 TKA_DEV T wave_sum(T v)
 {

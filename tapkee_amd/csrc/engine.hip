@@ -364,29 +364,31 @@ struct Session : tka_session {
         }
     }
 
-    // Current solution, zero-mean (tsne.hpp:167), as FP64 N x dims.
-    void embedding(double* Y) override
+    // The current solution in FP64, made zero-mean in FP64 (tsne.hpp:167).
+    DevBuf<double> centred_fp64()
     {
         hipStream_t s = this->s();
         const size_t nd = (size_t)N * dims;
-        DevBuf<T> tmp(nd, &mb);
-        TKA_HIP(hipMemcpyAsync(tmp.get(), Ycur, nd * sizeof(T), hipMemcpyDeviceToDevice, s));
-        if (!centred) center_device<T>(dims, tmp.get(), N, L.box.get(), nbox, s);
         DevBuf<double> Yd(nd, &mb);
-        cast_device<T, double>(tmp.get(), Yd.get(), nd, s);
-        download(Y, Yd.get(), nd, s);
-        TKA_HIP(hipStreamSynchronize(s));
+        cast_device<T, double>(Ycur, Yd.get(), nd, s);
+        center_fresh_device<double>(dims, Yd.get(), N, &mb, s);
+        return Yd;
     }
 
-    // KL(P||Q) of the current Y with the exact normaliser, FP64, P without
-    // exaggeration (translation invariant: no centring needed).
+    void embedding(double* Y) override
+    {
+        DevBuf<double> Yd = centred_fp64();
+        download(Y, Yd.get(), (size_t)N * dims, s());
+        TKA_HIP(hipStreamSynchronize(s()));
+    }
+
+    // KL(P||Q) of the current solution (the embedding() values) with the
+    // exact normaliser, FP64, P without exaggeration.
     double kl_exact() override
     {
         hipStream_t s = this->s();
         MemBudget* m = &mb;
-        const size_t nd = (size_t)N * dims;
-        DevBuf<double> Yd(nd, m);
-        cast_device<T, double>(Ycur, Yd.get(), nd, s);
+        DevBuf<double> Yd = centred_fp64();
         if (L.dense) {
             DevBuf<double> Pd((size_t)N * N, m);
             cast_device<T, double>(L.val.get(), Pd.get(), (size_t)N * N, s);

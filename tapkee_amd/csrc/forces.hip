@@ -147,6 +147,19 @@ __global__ __launch_bounds__(256) void center_rows(T* __restrict__ Y, int N, con
     }
 }
 
+// Per-block sums of each dimension (<= 3) in the box-partial layout.
+template <class T, int DIMS>
+__global__ __launch_bounds__(256) void sum_rows(const T* __restrict__ Y, int N, double* __restrict__ part)
+{
+    __shared__ double sh[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) {
+        const double v = block_sum(i < N ? (double)Y[(size_t)i * DIMS + d] : 0.0, sh);
+        if (threadIdx.x == 0) part[(size_t)kBoxStride * blockIdx.x + d] = v;
+    }
+}
+
 template <class T, int DIMS>
 __global__ __launch_bounds__(256) void grad_combine(const T* __restrict__ Y, const T* __restrict__ neg,
                                                     const T* __restrict__ pos_pre, const int* __restrict__ row_ptr,
@@ -398,6 +411,22 @@ void center_device(int dims, T* Y, int N, const double* box_part, int nb, hipStr
 }
 
 template <class T>
+void center_fresh_device(int dims, T* Y, int N, MemBudget* mb, hipStream_t s)
+{
+    const unsigned g = ceil_div(N, 256);
+    DevBuf<double> part((size_t)kBoxStride * g, mb);
+    switch (dims) {
+        case 1: hipLaunchKernelGGL((sum_rows<T, 1>), dim3(g), dim3(256), 0, s, Y, N, part.get()); break;
+        case 2: hipLaunchKernelGGL((sum_rows<T, 2>), dim3(g), dim3(256), 0, s, Y, N, part.get()); break;
+        case 3: hipLaunchKernelGGL((sum_rows<T, 3>), dim3(g), dim3(256), 0, s, Y, N, part.get()); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+    TKA_LAUNCH_CHECK();
+    center_device<T>(dims, Y, N, part.get(), (int)g, s);
+    TKA_HIP(hipStreamSynchronize(s));  // part is released on return
+}
+
+template <class T>
 int tsne_step_device(int dims, const T* Y, T* Ynext, T* uY, T* gains, const T* neg, const T* pos_pre,
                      const int* row_ptr, const int* col, const T* val, const double* zpart, int nz, int N,
                      StepParams sp, double* box_part, T* dC_out, hipStream_t s)
@@ -554,6 +583,7 @@ double kl_dense_device(int dims, const T* Y, const T* P, double exag, int N, int
 
 #define TKA_INST(T)                                                                                                  \
     template void center_device<T>(int, T*, int, const double*, int, hipStream_t);                                \
+    template void center_fresh_device<T>(int, T*, int, MemBudget*, hipStream_t);                                  \
     template int tsne_step_device<T>(int, const T*, T*, T*, T*, const T*, const T*, const int*, const int*, const T*, \
                                      const double*, int, int, StepParams, double*, T*, hipStream_t);                 \
     template void gradient_combine_device<T>(int, const T*, const T*, const T*, const int*, const int*, const T*,    \

@@ -31,6 +31,10 @@ int tsne_step_blocks(int N);
 template <class T>
 void center_device(int dims, T* Y, int N, const double* box_part, int nb, hipStream_t s);
 
+// zeroMean of Y (N x dims) computed from Y itself.
+template <class T>
+void center_fresh_device(int dims, T* Y, int N, MemBudget* mb, hipStream_t s);
+
 // Gradient only (no update): dC = pos - neg/Z.
 template <class T>
 void gradient_combine_device(int dims, const T* Y, const T* neg, const T* pos_pre, const int* row_ptr,

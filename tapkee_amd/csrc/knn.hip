@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void col_mean_seq(const double* __restrict__ X
         if (r0 + R < N) load(r0 + R);
         if (tid < 64) {
             const int rows = (N - r0) < R ? (N - r0) : R;
-            for (int r = 0; r < rows; r++) acc = __dadd_rn(acc, tile[r][tid]);
+            for (int r = 0; r < rows; r++) acc = dadd_sep(acc, tile[r][tid]);
         }
     }
     if (tid < 64 && c0 + tid < D) mean[c0 + tid] = acc / (double)N;
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(256) void row_norms(const double* __restrict__ X, i
         double s = 0.0;
         for (int d = 0; d < D; d++) {
             const double x = X[(size_t)n * D + d];
-            s = __dadd_rn(s, __dmul_rn(x, x));
+            s = dadd_sep(s, dmul_sep(x, x));
         }
         n2[n] = s;
         r = sqrt(s);
@@ -526,8 +526,8 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
             const double* xj = a.X + (size_t)j * a.D;
             double dd = 0.0;
             for (int d = 0; d < a.D; d++) {
-                const double t = __dadd_rn(xq[d], -xj[d]);
-                dd = __dadd_rn(dd, __dmul_rn(t, t));
+                const double t = dadd_sep(xq[d], -xj[d]);
+                dd = dadd_sep(dd, dmul_sep(t, t));
             }
             v[r].d = (uint64_t)__double_as_longlong(dd);
             v[r].i = j;

@@ -4,6 +4,13 @@ import sys
 import numpy as np
 import pytest
 
+# One HIP runtime per process: torch bundles its own libamdhip64, and it must
+# be loaded before the engine binds one (see tapkee_amd/_lib.py).
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    torch = None
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

@@ -227,13 +227,33 @@ def test_y0_replays_std_rand(engine, oracle):
     np.testing.assert_allclose(Y, Yr - Yr.mean(0), rtol=0, atol=1e-18)
 
 
+def _exact_kl_oracle(oracle, X, Y, perp, theta):
+    if theta > 0:
+        row, col, val, _ = oracle.build_p(X, perp, knn_mode=1)
+        return oracle.kl_exact_sparse(row, col, val, Y)
+    Xn = oracle.normalize(X)
+    P, _, _ = oracle.perplexity_dense(oracle.sqdist(Xn), perp)
+    P = P + P.T
+    P /= P.sum()
+    return oracle.kl_exact_dense(P, Y)
+
+
 @pytest.mark.parametrize("theta", [0.5, 0.0])
-def test_full_run_kl_within_one_percent(engine, oracle, theta):
+def test_full_run_final_kl_within_oracle_spread(engine, oracle, theta):
+    """1000 iterations from the same P and Y0.  The trajectory is chaotic: a
+    1e-15 relative change of Y0 moves the oracle's own KL at iteration 300 by
+    several percent, at 1000 iterations by 0.7% (theta=0) to 2% (theta=0.5).
+    The engine's final exact-Z KL must lie within that spread widened by 1%."""
     X = mixture(500, 20)
     y0 = rng(9).normal(size=(500, 2)) * 1e-4
-    Yr, ir = oracle.tsne_run(X, perplexity=15.0, theta=theta, max_iter=300, knn_mode=1, y0=y0)
+    ref = []
+    for eps in (0.0, 1e-15, 1e-12):
+        Yr, _ = oracle.tsne_run(X, perplexity=15.0, theta=theta, max_iter=1000, knn_mode=1, y0=y0 * (1 + eps))
+        ref.append(_exact_kl_oracle(oracle, X, Yr, 15.0, theta))
+    lo, hi = min(ref) * 0.99, max(ref) * 1.01
     for prec in ("fp64", "fp32"):
-        Y, st = engine.tsne(X, perplexity=15.0, theta=theta, max_iter=300, y0=y0, loop_precision=prec,
-                            dist_dtype="fp32")
-        kr = ir["kl"][-1]
-        assert abs(st["kl"][-1] - kr) / kr < 0.01, (prec, st["kl"], ir["kl"])
+        Y, st = engine.tsne(X, perplexity=15.0, theta=theta, y0=y0, loop_precision=prec, dist_dtype="fp32")
+        assert np.all(np.isfinite(Y)) and abs(Y.mean(0)).max() < 1e-6
+        kl = st["kl_final_exact"]
+        assert abs(kl - _exact_kl_oracle(oracle, X, Y, 15.0, theta)) < 1e-9 * kl
+        assert lo <= kl <= hi, (prec, kl, ref)
