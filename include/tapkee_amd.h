@@ -42,8 +42,8 @@ typedef enum tka_status {
 typedef enum tka_dist_dtype {
     TKA_DIST_FP32 = 0, /* FP32 VALU tiles */
     TKA_DIST_BF16 = 1, /* BF16 MFMA tiles, FP32 accumulate */
-    TKA_DIST_FP8 = 2,  /* FP8 e4m3 MFMA tiles, per-row absmax scaling */
-    TKA_DIST_FP64 = 3  /* FP64 VALU tiles (oracle-grade, slow) */
+    TKA_DIST_FP8 = 2   /* FP8 e4m3 MFMA tiles, per-row absmax scaling; approximate:
+                          rows failing the error bound are counted, not re-solved */
 } tka_dist_dtype;
 
 typedef enum tka_precision { TKA_FP32 = 0, TKA_FP64 = 1 } tka_precision;

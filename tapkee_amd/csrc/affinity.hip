@@ -295,11 +295,13 @@ void symmetrize_knn_device(const int* idx, const double* val, int N, int K, Csr&
     const size_t M = 2 * (size_t)N * K;
     DevBuf<uint64_t> k0(M, mb), v0(M, mb), k1(M, mb), v1(M, mb);
     DevBuf<uint32_t> scratch(radix_sort_scratch_words(M), mb);
+    TKA_HIP(hipMemsetAsync(scratch.get(), 0, scratch.size() * sizeof(uint32_t), s));
     const unsigned g = std::min<size_t>(65535, ceil_div((size_t)N * K, 256));
     hipLaunchKernelGGL(sym_keys, dim3(g), dim3(256), 0, s, idx, val, N, K, nb, k0.get(), v0.get());
     TKA_LAUNCH_CHECK();
     const bool alt = radix_sort_pairs<uint64_t, uint64_t>(k0.get(), v0.get(), k1.get(), v1.get(), M, 2 * nb,
                                                           scratch.get(), s);
+    require(radix_sort_fault(scratch.get(), s) == 0, TKA_E_INTERNAL, "radix sort look-back gave up");
     const uint64_t* ks = alt ? k1.get() : k0.get();
     const uint64_t* vs = alt ? v1.get() : v0.get();
     DevBuf<uint32_t> flags(M, mb), pos(M + 1, mb), sscr(exclusive_scan_scratch_words(M), mb);

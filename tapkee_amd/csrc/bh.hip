@@ -8,16 +8,24 @@
 // leaf or max(hh, hw)/sqrt(D) < theta (:335).
 //
 // Construction (all integer, bit-exact):
-//   keys   point -> 16+16 bit Morton key of its leaf cell, y bit above x bit,
+//   box    one workgroup reduces the bounding-box partials, in a fixed order,
+//          to the root cell.
+//   keys   point -> 32+32 bit Morton key of its leaf cell, y bit above x bit,
 //          so digit order 0..3 is NW, NE, SW, SE and sorting keys gives the
 //          reference's depth-first child order.  Quantisation is
-//          ceil(t * 2^16) - 1, so coordinates on a split line go west/north.
-//   sort   stable LSD radix sort of (key, point).
+//          ceil(t * 2^32) - 1, so coordinates on a split line go west/north;
+//          its top 16 bits equal ceil(t * 2^16) - 1, so the key's high word
+//          is the 16+16 bit key of the level-16 cell.
+//   sort   stable LSD radix sort of (high word, point).
+//   fix    runs of equal high words (points within 2^-16 of the box of each
+//          other: rare) are put in full-key order by a stable insertion sort,
+//          so the result equals a stable sort of the 64-bit keys.
 //   pairs  adjacent sorted keys i, i+1 first differ at quadtree level
-//          l = clz(k_i ^ k_i+1)/2; the level-l cell holding both has >= 2
+//          l = clz64(k_i ^ k_i+1)/2; the level-l cell holding both has >= 2
 //          non-empty children, i.e. it is a node of the compressed tree.
 //          The pair whose keys straddle the end of that cell's first child
-//          owns the node; galloping searches give the cell's key range [a,b).
+//          owns the node; galloping searches (over an LDS window of keys,
+//          global memory beyond it) give the cell's key range [a,b).
 //          Chains of one-child cells are skipped: such a cell has the same
 //          count and centre of mass as its child and a larger size, so the
 //          reference accepts it only if it would accept the child.
@@ -27,7 +35,7 @@
 //          in mask_a, leaf j at P[j] + popcount(mask_j), and a node's subtree
 //          ends at P[b] (its skip pointer).  Centres of mass come from FP64
 //          prefix sums of the sorted coordinates.
-//   Equal keys (points closer than 2^-16 of the box) form one level-16 node
+//   Equal keys (points closer than 2^-32 of the box) form one level-32 node
 //   whose children are the individual points.
 //
 // Traversal is stackless: visit node t; if it is a leaf, interact unless it
@@ -41,8 +49,9 @@ namespace tka {
 namespace {
 
 constexpr int kB = kMortonBits;
-constexpr int kScanItems = 4;
-constexpr int kScanTile = 256 * kScanItems;
+constexpr int kTreeScanItems = 4;
+constexpr int kTreeScanTile = 256 * kTreeScanItems;
+constexpr int kBoxReduceBlock = 1024;
 
 // ---------------------------------------------------------------- bbox
 template <class T>
@@ -90,14 +99,15 @@ __global__ __launch_bounds__(kBoxBlock) void bbox_part(const T* __restrict__ Y, 
     }
 }
 
-// Reduces the box partials (every block does it redundantly, in the same
-// fixed order, so all blocks agree bit for bit).  out6 = mean x, mean y,
-// min x, min y, max x, max y.
-TKA_DEV void reduce_box(const double* __restrict__ part, int nb, int N, double* out6)
+// Root cell of the points from their box partials (quadtree.hpp:104-129):
+// one workgroup, fixed reduction order.  The mean is removed from Y by the
+// key kernel, so the root cell is centred at 0.
+__global__ __launch_bounds__(kBoxReduceBlock) void bh_box(const double* __restrict__ part, int nb, int N,
+                                                          RootBox* __restrict__ root)
 {
-    __shared__ double sh[6][4];
+    __shared__ double sh[6][kBoxReduceBlock / 64];
     double v[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    for (int b = threadIdx.x; b < nb; b += kBoxReduceBlock) {
         const double* p = part + (size_t)kBoxStride * b;
         v[0] += p[0];
         v[1] += p[1];
@@ -112,13 +122,13 @@ TKA_DEV void reduce_box(const double* __restrict__ part, int nb, int N, double* 
     v[3] = wave_min(v[3]);
     v[4] = wave_max(v[4]);
     v[5] = wave_max(v[5]);
-    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    __syncthreads();
+    const int w = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0)
         for (int k = 0; k < 6; k++) sh[k][w] = v[k];
     __syncthreads();
+    if (threadIdx.x != 0) return;
     double r[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-    for (int ww = 0; ww < nw; ww++) {
+    for (int ww = 0; ww < kBoxReduceBlock / 64; ww++) {
         r[0] += sh[0][ww];
         r[1] += sh[1][ww];
         r[2] = fmin(r[2], sh[2][ww]);
@@ -126,9 +136,16 @@ TKA_DEV void reduce_box(const double* __restrict__ part, int nb, int N, double* 
         r[4] = fmax(r[4], sh[4][ww]);
         r[5] = fmax(r[5], sh[5][ww]);
     }
-    out6[0] = r[0] / (double)N;
-    out6[1] = r[1] / (double)N;
-    for (int k = 2; k < 6; k++) out6[k] = r[k];
+    const double mx = r[0] / (double)N, my = r[1] / (double)N;
+    const double hw = fmax(r[4] - mx, mx - r[2]) + 1e-5;
+    const double hh = fmax(r[5] - my, my - r[3]) + 1e-5;
+    RootBox rb;
+    rb.mean_x = mx;
+    rb.mean_y = my;
+    rb.hw = hw;
+    rb.hh = hh;
+    rb.size0 = fmax(hw, hh);
+    *root = rb;
 }
 
 // ---------------------------------------------------------------- keys
@@ -142,58 +159,91 @@ TKA_DEV uint32_t spread16(uint32_t v)
     return v;
 }
 
-// Quantise t in (0, 1) to ceil(t * 2^16) - 1 in [0, 2^16 - 1].
+// Quantise t in (0, 1) to ceil(t * 2^32) - 1 in [0, 2^32 - 1].
 TKA_DEV uint32_t quantise(double t)
 {
     double c = ceil(ldexp(t, kB)) - 1.0;
-    c = c < 0.0 ? 0.0 : (c > 65535.0 ? 65535.0 : c);
+    c = c < 0.0 ? 0.0 : (c > 4294967295.0 ? 4294967295.0 : c);
     return (uint32_t)c;
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void bh_keys(T* __restrict__ Y, int N, const double* __restrict__ box_part, int nb,
-                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ perm,
-                                               uint32_t* __restrict__ mask, RootBox* __restrict__ root)
+__global__ __launch_bounds__(256) void bh_keys(T* __restrict__ Y, int N, const RootBox* __restrict__ root,
+                                               uint32_t* __restrict__ khi, uint32_t* __restrict__ klo,
+                                               uint32_t* __restrict__ perm, unsigned long long* __restrict__ mask)
 {
-    __shared__ double box[6];
-    double r6[6];
-    reduce_box(box_part, nb, N, r6);
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 6; k++) box[k] = r6[k];
-    __syncthreads();
-    const double mx = box[0], my = box[1];
-    // Root cell of the centred points (centre 0): quadtree.hpp:128-129.
-    const double hw = fmax(box[4] - mx, mx - box[2]) + 1e-5;
-    const double hh = fmax(box[5] - my, my - box[3]) + 1e-5;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        root->mean_x = mx;
-        root->mean_y = my;
-        root->hw = hw;
-        root->hh = hh;
-        root->size0 = fmax(hw, hh);
-    }
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= N) return;
+    const double mx = root->mean_x, my = root->mean_y, hw = root->hw, hh = root->hh;
     const T x = (T)((double)Y[2 * (size_t)i] - mx);
     const T y = (T)((double)Y[2 * (size_t)i + 1] - my);
     Y[2 * (size_t)i] = x;
     Y[2 * (size_t)i + 1] = y;
     const uint32_t qx = quantise(((double)x + hw) / (2.0 * hw));
     const uint32_t qy = quantise(((double)y + hh) / (2.0 * hh));
-    keys[i] = (spread16(qy) << 1) | spread16(qx);
+    khi[i] = (spread16(qy >> 16) << 1) | spread16(qx >> 16);
+    klo[i] = (spread16(qy) << 1) | spread16(qx);
     perm[i] = (uint32_t)i;
-    mask[i] = 0u;
+    mask[i] = 0ull;
+}
+
+// Full 64-bit keys in sorted order, the final point order and the sorted
+// coordinates.  A point whose high word differs from both neighbours is
+// already in place; the first point of a run of equal high words sorts the
+// run by low word (stable insertion sort, in place in the outputs).
+template <class T>
+__global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ khi, const uint32_t* __restrict__ perm,
+                                              const uint32_t* __restrict__ klo, const T* __restrict__ Y, int N,
+                                              uint64_t* __restrict__ k64, uint32_t* __restrict__ p2,
+                                              typename Vec2<T>::type* __restrict__ ys)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= N) return;
+    const uint32_t h = khi[j];
+    if (j > 0 && khi[j - 1] == h) return;  // inside a run: its first point handles it
+    int e = j + 1;
+    while (e < N && khi[e] == h) e++;
+    const uint64_t hk = (uint64_t)h << 32;
+    for (int i = j; i < e; i++) {
+        const uint32_t p = perm[i];
+        const uint32_t l = klo[p];
+        int k = i;
+        while (k > j && (uint32_t)k64[k - 1] > l) {
+            k64[k] = k64[k - 1];
+            p2[k] = p2[k - 1];
+            k--;
+        }
+        k64[k] = hk | l;
+        p2[k] = p;
+    }
+    for (int i = j; i < e; i++) {
+        const uint32_t p = p2[i];
+        typename Vec2<T>::type v;
+        v.x = Y[2 * (size_t)p];
+        v.y = Y[2 * (size_t)p + 1];
+        ys[i] = v;
+    }
 }
 
 // ---------------------------------------------------------------- pairs
-TKA_DEV int gallop_left(const uint32_t* __restrict__ k, int i, uint32_t lo)
+// Keys [w0, w0 + kPairWin) of the sorted array are staged in LDS; the short
+// searches of deep cells stay inside it.
+constexpr int kPairWin = 3 * 256;
+
+TKA_DEV uint64_t key_at(const uint64_t* win, int w0, const uint64_t* __restrict__ k, int g)
+{
+    const int o = g - w0;
+    return (o >= 0 && o < kPairWin) ? win[o] : k[g];
+}
+
+TKA_DEV int gallop_left(const uint64_t* win, int w0, const uint64_t* __restrict__ k, int i, uint64_t lo)
 {
     // smallest a <= i with k[a] >= lo (k[i] >= lo)
     int hi = i, lw = -1, step = 1;
     while (true) {
         const int probe = i - step;
         if (probe < 0) break;
-        if (k[probe] < lo) {
+        if (key_at(win, w0, k, probe) < lo) {
             lw = probe;
             break;
         }
@@ -202,7 +252,7 @@ TKA_DEV int gallop_left(const uint32_t* __restrict__ k, int i, uint32_t lo)
     }
     while (hi - lw > 1) {
         const int mid = (hi + lw) >> 1;
-        if (k[mid] >= lo)
+        if (key_at(win, w0, k, mid) >= lo)
             hi = mid;
         else
             lw = mid;
@@ -210,14 +260,14 @@ TKA_DEV int gallop_left(const uint32_t* __restrict__ k, int i, uint32_t lo)
     return hi;
 }
 
-TKA_DEV int gallop_right(const uint32_t* __restrict__ k, int N, int i, uint32_t hi_key)
+TKA_DEV int gallop_right(const uint64_t* win, int w0, const uint64_t* __restrict__ k, int N, int i, uint64_t hi_key)
 {
     // smallest b > i with k[b] > hi_key, or N (k[i] <= hi_key)
     int lw = i, hi = N, step = 1;
     while (true) {
         const int probe = i + step;
         if (probe >= N) break;
-        if (k[probe] > hi_key) {
+        if (key_at(win, w0, k, probe) > hi_key) {
             hi = probe;
             break;
         }
@@ -226,7 +276,7 @@ TKA_DEV int gallop_right(const uint32_t* __restrict__ k, int N, int i, uint32_t 
     }
     while (hi - lw > 1) {
         const int mid = (hi + lw) >> 1;
-        if (k[mid] <= hi_key)
+        if (key_at(win, w0, k, mid) <= hi_key)
             lw = mid;
         else
             hi = mid;
@@ -234,44 +284,40 @@ TKA_DEV int gallop_right(const uint32_t* __restrict__ k, int N, int i, uint32_t 
     return hi;
 }
 
-template <class T>
-__global__ __launch_bounds__(256) void bh_pairs(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ perm,
-                                                const T* __restrict__ Y, int N, uint32_t* __restrict__ mask,
-                                                int* __restrict__ node_a, int* __restrict__ node_b,
-                                                int* __restrict__ node_lvl,
-                                                typename Vec2<T>::type* __restrict__ ys)
+__global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ keys, int N,
+                                                unsigned long long* __restrict__ mask, int* __restrict__ node_a,
+                                                int* __restrict__ node_b, int* __restrict__ node_lvl)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    {
-        const uint32_t p = perm[i];
-        typename Vec2<T>::type v;
-        v.x = Y[2 * (size_t)p];
-        v.y = Y[2 * (size_t)p + 1];
-        ys[i] = v;
+    __shared__ uint64_t win[kPairWin];
+    const int w0 = (int)blockIdx.x * 256 - 256;
+    for (int t = threadIdx.x; t < kPairWin; t += 256) {
+        const int g = w0 + t;
+        win[t] = (g >= 0 && g < N) ? keys[g] : 0ull;
     }
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= N - 1) return;
-    const uint32_t ki = keys[i], kj = keys[i + 1];
-    const uint32_t x = ki ^ kj;
+    const uint64_t ki = win[i - w0], kj = win[i + 1 - w0];
+    const uint64_t x = ki ^ kj;
     int lvl, a, b;
     bool owner;
-    if (x == 0u) {
+    if (x == 0ull) {
         lvl = kB;
-        owner = (i == 0) || (keys[i - 1] != ki);
+        owner = (i == 0) || (key_at(win, w0, keys, i - 1) != ki);
         a = i;
-        b = owner ? gallop_right(keys, N, i + 1, ki) : 0;
+        b = owner ? gallop_right(win, w0, keys, N, i + 1, ki) : 0;
     } else {
-        lvl = __clz(x) >> 1;
-        const int lowbits = 2 * (kB - lvl);  // 2..32
-        const uint32_t lowmask = (lowbits >= 32) ? 0xffffffffu : ((1u << lowbits) - 1u);
-        const uint32_t lo = ki & ~lowmask, hi = ki | lowmask;
-        a = gallop_left(keys, i, lo);
+        lvl = __clzll((long long)x) >> 1;
+        const int lowbits = 2 * (kB - lvl);  // 2..64
+        const uint64_t lowmask = (lowbits >= 64) ? ~0ull : ((1ull << lowbits) - 1ull);
+        const uint64_t lo = ki & ~lowmask, hi = ki | lowmask;
+        a = gallop_left(win, w0, keys, i, lo);
         const int sh = lowbits - 2;
-        owner = (ki >> sh) == (keys[a] >> sh);
-        b = owner ? gallop_right(keys, N, i + 1, hi) : 0;
+        owner = (ki >> sh) == (key_at(win, w0, keys, a) >> sh);
+        b = owner ? gallop_right(win, w0, keys, N, i + 1, hi) : 0;
     }
     if (owner) {
-        atomicOr(&mask[a], 1u << lvl);
+        atomicOr(&mask[a], 1ull << lvl);
         node_a[i] = a;
         node_b[i] = b;
         node_lvl[i] = lvl;
@@ -282,18 +328,18 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint32_t* __restrict__ key
 
 // ---------------------------------------------------------------- scan
 template <class T>
-__global__ __launch_bounds__(256) void bh_scan_part(const uint32_t* __restrict__ mask,
+__global__ __launch_bounds__(256) void bh_scan_part(const unsigned long long* __restrict__ mask,
                                                     const typename Vec2<T>::type* __restrict__ ys, int N,
                                                     double* __restrict__ part)
 {
     __shared__ double sh[4];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    const int base = blockIdx.x * kTreeScanTile + threadIdx.x * kTreeScanItems;
     double c = 0.0, sx = 0.0, sy = 0.0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
+    for (int k = 0; k < kTreeScanItems; k++) {
         const int j = base + k;
         if (j < N) {
-            c += (double)(__popc(mask[j]) + 1);
+            c += (double)(__popcll(mask[j]) + 1);
             sx += (double)ys[j].x;
             sy += (double)ys[j].y;
         }
@@ -320,7 +366,7 @@ TKA_DEV double wave_incl_scan_d(double v)
 }
 
 template <class T>
-__global__ __launch_bounds__(256) void bh_scan_write(const uint32_t* __restrict__ mask,
+__global__ __launch_bounds__(256) void bh_scan_write(const unsigned long long* __restrict__ mask,
                                                      const typename Vec2<T>::type* __restrict__ ys, int N,
                                                      const double* __restrict__ part, uint32_t* __restrict__ pre,
                                                      double* __restrict__ psx, double* __restrict__ psy)
@@ -337,15 +383,15 @@ __global__ __launch_bounds__(256) void bh_scan_write(const uint32_t* __restrict_
     b0 = block_sum(b0, sh);
     b1 = block_sum(b1, sh);
     b2 = block_sum(b2, sh);
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    uint32_t c[kScanItems];
-    double x[kScanItems], y[kScanItems];
+    const int base = blockIdx.x * kTreeScanTile + threadIdx.x * kTreeScanItems;
+    uint32_t c[kTreeScanItems];
+    double x[kTreeScanItems], y[kTreeScanItems];
     double tc = 0.0, tx = 0.0, ty = 0.0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
+    for (int k = 0; k < kTreeScanItems; k++) {
         const int j = base + k;
         const bool ok = j < N;
-        c[k] = ok ? (uint32_t)(__popc(mask[j]) + 1) : 0u;
+        c[k] = ok ? (uint32_t)(__popcll(mask[j]) + 1) : 0u;
         x[k] = ok ? (double)ys[j].x : 0.0;
         y[k] = ok ? (double)ys[j].y : 0.0;
         tc += (double)c[k];
@@ -368,7 +414,7 @@ __global__ __launch_bounds__(256) void bh_scan_write(const uint32_t* __restrict_
     }
     double rc = oc + ic - tc, rx = ox + ix - tx, ry = oy + iy - ty;
 #pragma unroll
-    for (int k = 0; k < kScanItems; k++) {
+    for (int k = 0; k < kTreeScanItems; k++) {
         const int j = base + k;
         if (j < N) {
             pre[j] = (uint32_t)rc;
@@ -388,8 +434,9 @@ __global__ __launch_bounds__(256) void bh_scan_write(const uint32_t* __restrict_
 
 // ---------------------------------------------------------------- emit
 template <class T>
-__global__ __launch_bounds__(256) void bh_emit(const uint32_t* __restrict__ mask, const int* __restrict__ node_a,
-                                               const int* __restrict__ node_b, const int* __restrict__ node_lvl,
+__global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restrict__ mask,
+                                               const int* __restrict__ node_a, const int* __restrict__ node_b,
+                                               const int* __restrict__ node_lvl,
                                                const typename Vec2<T>::type* __restrict__ ys,
                                                const uint32_t* __restrict__ perm, int N,
                                                const uint32_t* __restrict__ pre, const double* __restrict__ psx,
@@ -399,8 +446,8 @@ __global__ __launch_bounds__(256) void bh_emit(const uint32_t* __restrict__ mask
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= N) return;
     {  // leaf j
-        const uint32_t m = mask[j];
-        const uint32_t t = pre[j] + (uint32_t)__popc(m);
+        const unsigned long long m = mask[j];
+        const uint32_t t = pre[j] + (uint32_t)__popcll(m);
         typename Vec4<T>::type nd;
         nd.x = ys[j].x;
         nd.y = ys[j].y;
@@ -413,8 +460,8 @@ __global__ __launch_bounds__(256) void bh_emit(const uint32_t* __restrict__ mask
         const int lvl = node_lvl[j];
         if (lvl >= 0) {
             const int a = node_a[j], b = node_b[j];
-            const uint32_t m = mask[a];
-            const uint32_t t = pre[a] + (uint32_t)__popc(m & ((1u << lvl) - 1u));
+            const unsigned long long m = mask[a];
+            const uint32_t t = pre[a] + (uint32_t)__popcll(m & ((1ull << lvl) - 1ull));
             const double cnt = (double)(b - a);
             const double size = ldexp(root->size0, -lvl);
             typename Vec4<T>::type nd;
@@ -429,6 +476,42 @@ __global__ __launch_bounds__(256) void bh_emit(const uint32_t* __restrict__ mask
 }
 
 // ---------------------------------------------------------------- traverse
+// One node of point p's traversal (quadtree.hpp:317-354): a leaf other than
+// p itself interacts and the walk goes to t+1; an accepted cell
+// (size^2 < theta^2 D) interacts with count*q and the walk jumps past its
+// subtree; otherwise the walk descends to t+1.  Returns the next node.
+template <class T>
+TKA_DEV int bh_visit(const typename Vec4<T>::type& nd, int ax, int t, int p, T px, T py, T theta2, T& fx, T& fy, T& z,
+                     unsigned& cnt)
+{
+    const T dx = px - nd.x, dy = py - nd.y;
+    const T D = dx * dx + dy * dy;
+    if (nd.w < (T)0) {
+        if (ax != p) {
+            const T q = (T)1 / ((T)1 + D);
+            z += q;
+            const T q2 = q * q;
+            fx += q2 * dx;
+            fy += q2 * dy;
+            cnt++;
+        }
+        return t + 1;
+    }
+    if (nd.w < theta2 * D) {
+        const T q = (T)1 / ((T)1 + D);
+        const T cq = nd.z * q;
+        z += cq;
+        const T m = cq * q;
+        fx += m * dx;
+        fy += m * dy;
+        cnt++;
+        return ax > t ? ax : t + 1;  // skip pointers point forward; never loop
+    }
+    return t + 1;
+}
+
+// One thread per point, in Morton order, so the lanes of a wave walk similar
+// paths and their node gathers mostly hit the same cache lines.
 template <class T>
 __global__ __launch_bounds__(256) void bh_traverse(const typename Vec4<T>::type* __restrict__ nodes,
                                                    const int* __restrict__ aux, const uint32_t* __restrict__ pre,
@@ -439,42 +522,22 @@ __global__ __launch_bounds__(256) void bh_traverse(const typename Vec4<T>::type*
 {
     __shared__ double sh[4];
     const int j = blockIdx.x * 256 + threadIdx.x;
+    const bool active = j < N;
     double zd = 0.0;
     unsigned cnt = 0;
-    if (j < N) {
-        const int total = (int)pre[N];
-        const int p = (int)perm[j];
-        const T px = ys[j].x, py = ys[j].y;
-        T fx = 0, fy = 0, z = 0;
+    const int total = (int)pre[N];
+    const int p = active ? (int)perm[j] : -1;
+    T px = (T)0, py = (T)0;
+    if (active) {
+        px = ys[j].x;
+        py = ys[j].y;
+    }
+    T fx = 0, fy = 0, z = 0;
+    if (active) {
         int t = 0;
-        while (t < total) {
-            const typename Vec4<T>::type nd = nodes[t];
-            const int ax = aux[t];
-            const T dx = px - nd.x, dy = py - nd.y;
-            const T D = dx * dx + dy * dy;
-            if (nd.w < (T)0) {
-                if (ax != p) {
-                    const T q = (T)1 / ((T)1 + D);
-                    z += q;
-                    const T q2 = q * q;
-                    fx += q2 * dx;
-                    fy += q2 * dy;
-                    cnt++;
-                }
-                t++;
-            } else if (nd.w < theta2 * D) {
-                const T q = (T)1 / ((T)1 + D);
-                const T cq = nd.z * q;
-                z += cq;
-                const T m = cq * q;
-                fx += m * dx;
-                fy += m * dy;
-                cnt++;
-                t = ax > t ? ax : t + 1;  // skip pointers point forward; never loop
-            } else {
-                t++;
-            }
-        }
+        while (t < total) t = bh_visit<T>(nodes[t], aux[t], t, p, px, py, theta2, fx, fy, z, cnt);
+    }
+    if (active) {
         neg[2 * (size_t)p] = fx;
         neg[2 * (size_t)p + 1] = fy;
         zd = (double)z;
@@ -497,7 +560,11 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     keys_alt.alloc(n, mb);
     perm.alloc(n, mb);
     perm_alt.alloc(n, mb);
+    klo.alloc(n, mb);
+    k64.alloc(n, mb);
+    p2.alloc(n, mb);
     sort_scratch.alloc(radix_sort_scratch_words(n), mb);
+    TKA_HIP(hipMemset(sort_scratch.get(), 0, sort_scratch.size() * sizeof(uint32_t)));
     mask.alloc(n, mb);
     node_a.alloc(n, mb);
     node_b.alloc(n, mb);
@@ -506,7 +573,7 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     pre.alloc(n + 1, mb);
     sx.alloc(n + 1, mb);
     sy.alloc(n + 1, mb);
-    part.alloc(3 * (size_t)ceil_div(n, kScanTile), mb);
+    part.alloc(3 * (size_t)ceil_div(n, kTreeScanTile), mb);
     nodes.alloc(2 * (size_t)n, mb);
     aux.alloc(2 * (size_t)n, mb);
     root.alloc(1, mb);
@@ -524,18 +591,24 @@ void bh_build(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipS
 {
     require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "tree storage too small");
     const unsigned g = ceil_div(N, 256);
-    hipLaunchKernelGGL(bh_keys<T>, dim3(g), dim3(256), 0, s, Y, N, box_part, nbox, tr.keys.get(),
-                       tr.perm.get(), tr.mask.get(), tr.root.get());
+    hipLaunchKernelGGL(bh_box, dim3(1), dim3(kBoxReduceBlock), 0, s, box_part, nbox, N, tr.root.get());
+    TKA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bh_keys<T>, dim3(g), dim3(256), 0, s, Y, N, tr.root.get(), tr.keys.get(), tr.klo.get(),
+                       tr.perm.get(), tr.mask.get());
     TKA_LAUNCH_CHECK();
     const bool alt = radix_sort_pairs<uint32_t, uint32_t>(tr.keys.get(), tr.perm.get(), tr.keys_alt.get(),
-                                                          tr.perm_alt.get(), N, 2 * kB, tr.sort_scratch.get(), s);
-    const uint32_t* k = alt ? tr.keys_alt.get() : tr.keys.get();
-    const uint32_t* p = alt ? tr.perm_alt.get() : tr.perm.get();
-    tr.sorted_perm = p;
-    hipLaunchKernelGGL(bh_pairs<T>, dim3(g), dim3(256), 0, s, k, p, Y, N, tr.mask.get(), tr.node_a.get(),
-                       tr.node_b.get(), tr.node_lvl.get(), tr.ys.get());
+                                                          tr.perm_alt.get(), N, 32, tr.sort_scratch.get(), s);
+    const uint32_t* hk = alt ? tr.keys_alt.get() : tr.keys.get();
+    const uint32_t* hp = alt ? tr.perm_alt.get() : tr.perm.get();
+    hipLaunchKernelGGL(bh_fix<T>, dim3(g), dim3(256), 0, s, hk, hp, tr.klo.get(), Y, N, tr.k64.get(), tr.p2.get(),
+                       tr.ys.get());
     TKA_LAUNCH_CHECK();
-    const unsigned gs = ceil_div(N, kScanTile);
+    const uint32_t* p = tr.p2.get();
+    tr.sorted_perm = p;
+    hipLaunchKernelGGL(bh_pairs, dim3(g), dim3(256), 0, s, tr.k64.get(), N, tr.mask.get(), tr.node_a.get(),
+                       tr.node_b.get(), tr.node_lvl.get());
+    TKA_LAUNCH_CHECK();
+    const unsigned gs = ceil_div(N, kTreeScanTile);
     hipLaunchKernelGGL(bh_scan_part<T>, dim3(gs), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N, tr.part.get());
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(bh_scan_write<T>, dim3(gs), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N, tr.part.get(),
@@ -553,7 +626,7 @@ int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* 
 {
     (void)Y;
     const int g = bh_zblocks(tr.N);
-    hipLaunchKernelGGL(bh_traverse<T>, dim3(g), dim3(256), 0, s, tr.nodes.get(), tr.aux.get(), tr.pre.get(),
+    hipLaunchKernelGGL((bh_traverse<T>), dim3(g), dim3(256), 0, s, tr.nodes.get(), tr.aux.get(), tr.pre.get(),
                        tr.ys.get(), tr.sorted_perm, tr.N, (T)(theta * theta), neg, zpart, interactions);
     TKA_LAUNCH_CHECK();
     return g;

@@ -27,7 +27,7 @@ struct Vec4<double> {
     typedef double4 type;
 };
 
-constexpr int kMortonBits = 16;  // per axis: 32-bit keys, leaf cells of 2^-16 of the root box
+constexpr int kMortonBits = 32;  // per axis: 64-bit keys, leaf cells of 2^-32 of the root box
 
 // Per-block partial statistics of an N x dims array, kBoxStride doubles per
 // block: {sum d0, sum d1, sum d2, min x, min y, max x, max y, 0}.
@@ -44,8 +44,11 @@ struct RootBox {
 template <class T>
 struct BhTree {
     int N = 0;
-    DevBuf<uint32_t> keys, keys_alt, perm, perm_alt, sort_scratch;
-    DevBuf<uint32_t> mask;       // N: levels of internal nodes whose range starts at key j
+    DevBuf<uint32_t> keys, keys_alt, perm, perm_alt, sort_scratch;  // high key words, radix-sorted
+    DevBuf<uint32_t> klo;        // low key word per point
+    DevBuf<uint64_t> k64;        // full keys, sorted
+    DevBuf<uint32_t> p2;         // point at each sorted position
+    DevBuf<unsigned long long> mask;  // N: levels of internal nodes whose range starts at key j
     DevBuf<int> node_a, node_b;  // per pair (owner only)
     DevBuf<int> node_lvl;        // per pair: level if owner, -1 otherwise
     DevBuf<typename Vec2<T>::type> ys;  // sorted coordinates

@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "forces.hpp"
 #include "knn.hpp"
+#include "radix_sort.hpp"
 
 namespace tka {
 namespace {
@@ -264,6 +265,39 @@ struct Session : tka_session {
     bool tree_ready = false;  // tree, neg and zpart already computed for Ycur
     bool centred = false;     // Ycur is zero-mean (else its mean is in the box partials)
 
+    // Points are periodically renumbered in the Morton order of the latest
+    // tree so that CSR neighbours and tree leaves sit close in memory; orig
+    // maps the current numbering back to the caller's.
+    static constexpr int kRenumberEvery = 50;
+    bool renumbered = false;
+    DevBuf<int> orig, orig_tmp, inv, row2, col2;
+    DevBuf<T> val2, state_tmp;
+    DevBuf<uint32_t> rn_scratch;
+
+    void renumber(const uint32_t* perm)
+    {
+        hipStream_t s = this->s();
+        MemBudget* m = &mb;
+        const size_t nd = (size_t)N * dims;
+        if (!renumbered) {
+            orig_tmp.alloc(N, m);
+            inv.alloc(N, m);
+            row2.alloc((size_t)N + 1, m);
+            col2.alloc(L.col.size(), m);
+            val2.alloc(L.val.size(), m);
+            state_tmp.alloc(nd, m);
+            rn_scratch.alloc(renumber_scan_scratch_words(N), m);
+            renumbered = true;
+        }
+        T* state[3] = {Ycur, L.uY.get(), L.gains.get()};
+        renumber_points_device<T>(N, dims, perm, state, 3, state_tmp.get(), L.row.get(), L.col.get(), L.val.get(),
+                                  row2.get(), col2.get(), val2.get(), orig.get(), orig_tmp.get(), inv.get(),
+                                  rn_scratch.get(), s);
+        std::swap(L.row, row2);
+        std::swap(L.col, col2);
+        std::swap(L.val, val2);
+    }
+
     void init(const std::vector<double>& Y0)
     {
         hipStream_t s = this->s();
@@ -295,14 +329,16 @@ struct Session : tka_session {
         momentum = o.momentum;
         interactions.alloc(1, m);
         TKA_HIP(hipMemsetAsync(interactions.get(), 0, sizeof(unsigned long long), s));
+        std::vector<int> id(N);
+        for (int i = 0; i < N; i++) id[i] = i;
+        upload(orig, id.data(), (size_t)N, s, m);
+        TKA_HIP(hipStreamSynchronize(s));
     }
 
     void step(int n) override
     {
         hipStream_t s = this->s();
         MemBudget* m = &mb;
-        const int* row = L.dense ? nullptr : L.row.get();
-        const int* col = L.dense ? nullptr : L.col.get();
         for (int k = 0; k < n; k++, iter++) {
             if (exact) {
                 if (profile) timers[kTimExact].begin(s);
@@ -310,6 +346,8 @@ struct Session : tka_session {
                                        L.dense ? L.pos.get() : nullptr, L.zpart.get(), s);
                 if (profile) timers[kTimExact].end(s);
             } else if (!tree_ready) {
+                if (iter > 0 && iter % kRenumberEvery == 0 && tree.sorted_perm != nullptr)
+                    renumber(tree.sorted_perm);  // order of the previous iteration's tree
                 if (profile) timers[kTimBuild].begin(s);
                 bh_build<T>(Ycur, N, L.box.get(), nbox, tree, s);
                 if (profile) timers[kTimBuild].end(s);
@@ -319,6 +357,9 @@ struct Session : tka_session {
                 if (profile) timers[kTimRepulsion].end(s);
             }
             tree_ready = false;
+            // after renumbering: the CSR arrays were swapped for their renumbered copies
+            const int* row = L.dense ? nullptr : L.row.get();
+            const int* col = L.dense ? nullptr : L.col.get();
             StepParams sp{momentum, o.eta, exag};
             if (profile) timers[kTimStep].begin(s);
             nbox = tsne_step_device<T>(dims, Ycur, Ynext, L.uY.get(), L.gains.get(), L.neg.get(),
@@ -375,9 +416,23 @@ struct Session : tka_session {
         return Yd;
     }
 
+    // The Morton sort's tile look-back is bounded; a sort that gave up has
+    // left a fault word (never expected: every wait is on a running tile).
+    void check_sort()
+    {
+        if (!exact && radix_sort_fault(tree.sort_scratch.get(), s()))
+            fail(TKA_E_INTERNAL, "radix sort look-back gave up; the tree of some iteration was wrong");
+    }
+
     void embedding(double* Y) override
     {
+        check_sort();
         DevBuf<double> Yd = centred_fp64();
+        if (renumbered) {
+            DevBuf<double> out((size_t)N * dims, &mb);
+            unpermute_rows_device<double>(N, dims, Yd.get(), orig.get(), out.get(), s());
+            Yd = std::move(out);
+        }
         download(Y, Yd.get(), (size_t)N * dims, s());
         TKA_HIP(hipStreamSynchronize(s()));
     }
@@ -388,6 +443,7 @@ struct Session : tka_session {
     {
         hipStream_t s = this->s();
         MemBudget* m = &mb;
+        check_sort();
         DevBuf<double> Yd = centred_fp64();
         if (L.dense) {
             DevBuf<double> Pd((size_t)N * N, m);

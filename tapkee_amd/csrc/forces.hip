@@ -3,6 +3,8 @@
 
 #include <cfloat>
 
+#include "radix_sort.hpp"
+
 namespace tka {
 namespace {
 
@@ -367,6 +369,57 @@ __global__ __launch_bounds__(256) void sum_fixed(const double* __restrict__ v, i
     if (threadIdx.x == 0) *out = s;
 }
 
+// ---------------------------------------------------------------- renumbering
+__global__ __launch_bounds__(256) void rn_inverse(const uint32_t* __restrict__ perm, int N, int* __restrict__ inv)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < N) inv[perm[j]] = j;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void rn_gather(const uint32_t* __restrict__ perm, int N, int dims,
+                                                 const T* __restrict__ src, T* __restrict__ dst)
+{
+    const size_t total = (size_t)N * dims;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const size_t j = e / dims, d = e % dims;
+        dst[e] = src[(size_t)perm[j] * dims + d];
+    }
+}
+
+__global__ __launch_bounds__(256) void rn_lengths(const uint32_t* __restrict__ perm, const int* __restrict__ row,
+                                                  int N, uint32_t* __restrict__ len)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < N) {
+        const uint32_t p = perm[j];
+        len[j] = (uint32_t)(row[p + 1] - row[p]);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void rn_copy_rows(const uint32_t* __restrict__ perm, const int* __restrict__ inv,
+                                                    const int* __restrict__ row, const int* __restrict__ col,
+                                                    const T* __restrict__ val, const uint32_t* __restrict__ start,
+                                                    int N, int* __restrict__ row2, int* __restrict__ col2,
+                                                    T* __restrict__ val2)
+{
+    const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
+    const int j = blockIdx.x * kRowsPerBlock + g;
+    if (j >= N) return;
+    const uint32_t p = perm[j];
+    const int b = row[p], n = row[p + 1] - b;
+    const uint32_t o = start[j];
+    for (int k = gl; k < n; k += kG) {
+        col2[o + k] = inv[col[b + k]];
+        val2[o + k] = val[b + k];
+    }
+    if (gl == 0) {
+        row2[j] = (int)o;
+        if (j == N - 1) row2[N] = (int)start[N];
+    }
+}
+
 template <class T, int DIMS, bool kCompat, bool kDense>
 void launch_exact(const T* Y, int N, const T* P, double exag, T* neg, T* pos, double* zpart, hipStream_t s)
 {
@@ -396,6 +449,57 @@ void dispatch_exact(const T* Y, int N, int dd_mode, const T* P, double exag, T* 
 }  // namespace
 
 int tsne_step_blocks(int N) { return (int)ceil_div(N, kRowsPerBlock); }
+
+size_t renumber_scan_scratch_words(int N) { return (size_t)N + (size_t)N + 1 + exclusive_scan_scratch_words(N); }
+
+template <class T>
+void renumber_points_device(int N, int dims, const uint32_t* perm, T* const* state, int nstate, T* tmp,
+                            const int* row, const int* col, const T* val, int* row2, int* col2, T* val2,
+                            int* orig, int* orig_tmp, int* inv, uint32_t* scan_scratch, hipStream_t s)
+{
+    const unsigned g = ceil_div(N, 256);
+    const size_t nd = (size_t)N * dims;
+    const unsigned gn = (unsigned)std::min<size_t>(8192, ceil_div(nd, 256));
+    hipLaunchKernelGGL(rn_inverse, dim3(g), dim3(256), 0, s, perm, N, inv);
+    TKA_LAUNCH_CHECK();
+    for (int k = 0; k < nstate; k++) {
+        hipLaunchKernelGGL((rn_gather<T>), dim3(gn), dim3(256), 0, s, perm, N, dims, state[k], tmp);
+        TKA_LAUNCH_CHECK();
+        TKA_HIP(hipMemcpyAsync(state[k], tmp, nd * sizeof(T), hipMemcpyDeviceToDevice, s));
+    }
+    hipLaunchKernelGGL((rn_gather<int>), dim3(g), dim3(256), 0, s, perm, N, 1, orig, orig_tmp);
+    TKA_LAUNCH_CHECK();
+    TKA_HIP(hipMemcpyAsync(orig, orig_tmp, (size_t)N * sizeof(int), hipMemcpyDeviceToDevice, s));
+    uint32_t* len = scan_scratch;
+    uint32_t* start = scan_scratch + N;
+    uint32_t* tmpw = start + N + 1;
+    hipLaunchKernelGGL(rn_lengths, dim3(g), dim3(256), 0, s, perm, row, N, len);
+    TKA_LAUNCH_CHECK();
+    exclusive_scan_u32(len, start, N, tmpw, s);
+    hipLaunchKernelGGL((rn_copy_rows<T>), dim3(tsne_step_blocks(N)), dim3(256), 0, s, perm, inv, row, col, val, start,
+                       N, row2, col2, val2);
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void unpermute_rows(int N, int dims, const T* __restrict__ Y,
+                                                      const int* __restrict__ orig, T* __restrict__ out)
+{
+    const size_t total = (size_t)N * dims;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const size_t j = e / dims, d = e % dims;
+        out[(size_t)orig[j] * dims + d] = Y[e];
+    }
+}
+
+template <class T>
+void unpermute_rows_device(int N, int dims, const T* Y, const int* orig, T* out, hipStream_t s)
+{
+    const size_t nd = (size_t)N * dims;
+    hipLaunchKernelGGL((unpermute_rows<T>), dim3((unsigned)std::min<size_t>(8192, ceil_div(nd, 256))), dim3(256), 0,
+                       s, N, dims, Y, orig, out);
+    TKA_LAUNCH_CHECK();
+}
 
 template <class T>
 void center_device(int dims, T* Y, int N, const double* box_part, int nb, hipStream_t s)
@@ -584,6 +688,9 @@ double kl_dense_device(int dims, const T* Y, const T* P, double exag, int N, int
 #define TKA_INST(T)                                                                                                  \
     template void center_device<T>(int, T*, int, const double*, int, hipStream_t);                                \
     template void center_fresh_device<T>(int, T*, int, MemBudget*, hipStream_t);                                  \
+    template void renumber_points_device<T>(int, int, const uint32_t*, T* const*, int, T*, const int*, const int*,  \
+                                            const T*, int*, int*, T*, int*, int*, int*, uint32_t*, hipStream_t);     \
+    template void unpermute_rows_device<T>(int, int, const T*, const int*, T*, hipStream_t);                        \
     template int tsne_step_device<T>(int, const T*, T*, T*, T*, const T*, const T*, const int*, const int*, const T*, \
                                      const double*, int, int, StepParams, double*, T*, hipStream_t);                 \
     template void gradient_combine_device<T>(int, const T*, const T*, const T*, const int*, const int*, const T*,    \

@@ -308,6 +308,12 @@ TKA_DEV void filter_tile(const float (&v)[16], int t, int cb, int q, int slot_lo
                          int* s_cnt, uint64_t* qbuf)
 {
     const int lane = threadIdx.x & 63;
+    // Once the threshold has settled almost no value passes: one min per lane
+    // decides whether the per-value tests are needed at all.
+    float mn = v[0];
+#pragma unroll
+    for (int i = 1; i < 16; i++) mn = fminf(mn, v[i]);
+    if (!(mn < tau)) return;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
         const int m = 32 * t + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
@@ -490,6 +496,100 @@ __global__ __launch_bounds__(256) void knn_tiles_bf16(const __bf16* __restrict__
     }
 }
 
+// FP8 (e4m3) MFMA tiles with per-row absmax scaling: A8 rows hold
+// fp8(x / s_x), s_x = absmax(x) / 224, so q.c ~= s_q s_c sum(q8 c8).
+// The squared norms come from FP32 copies of the FP64 norms.
+__global__ __launch_bounds__(256) void knn_tiles_fp8(const uint8_t* __restrict__ A8, const float* __restrict__ scale,
+                                                     const float* __restrict__ nrm, int Kp, TileArgs a)
+{
+    constexpr int KC = 256;          // bytes (fp8 values) per LDS chunk
+    constexpr int KV = KC / 8;       // 8-byte vectors per row in a chunk
+    __shared__ uint64_t sA[kCB][KV + 1];
+    __shared__ float sN[kCB], sS[kCB];
+    __shared__ int s_cnt[kQB];
+    __shared__ float s_tau[kQB];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
+    const int slot0 = blockIdx.x * kQB;
+    const int sl = 32 * w + (lane & 31);
+    const int q = query_row(a, slot0 + sl);
+    const float sq = q >= 0 ? scale[q] : 0.f;
+    if (tid < kQB) {
+        s_cnt[tid] = 0;
+        s_tau[tid] = HUGE_VALF;
+    }
+    uint64_t* qbuf = a.buf + (size_t)(slot0 + sl) * kCap;
+    for (int cb = 0; cb < a.N; cb += kCB) {
+        f32x16 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[t] = f32x16{};
+        for (int kc = 0; kc < Kp; kc += KC) {
+            const int kn = (Kp - kc) < KC ? (Kp - kc) : KC;  // multiple of 16
+            const int nv = kn / 8;
+            __syncthreads();
+            for (int e = tid; e < kCB * KV; e += 256) {
+                const int m = e / KV, v = e % KV;
+                const int row = cb + m;
+                sA[m][v] = (row < a.N && v < nv) ? *reinterpret_cast<const uint64_t*>(A8 + (size_t)row * Kp + kc + 8 * v)
+                                                 : 0ull;
+            }
+            if (kc == 0 && tid < kCB) {
+                const int row = cb + tid;
+                sN[tid] = row < a.N ? nrm[row] : 0.f;
+                sS[tid] = row < a.N ? scale[row] : 0.f;
+            }
+            __syncthreads();
+            for (int ks = 0; ks < kn / 16; ks++) {
+                const long b = (q >= 0) ? *reinterpret_cast<const long*>(A8 + (size_t)q * Kp + kc + 16 * ks + 8 * half)
+                                        : 0l;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const long av = (long)sA[32 * t + (lane & 31)][2 * ks + half];
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(av, b, acc[t], 0, 0, 0);
+                }
+            }
+        }
+        const float tau = s_tau[sl];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            float v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = 32 * t + 8 * (i >> 2) + 4 * half + (i & 3);
+                v[i] = fmaf(-2.f * sq * sS[m], acc[t][i], sN[m]);
+            }
+            filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
+        }
+        compact_if_needed(w, a, slot0, s_cnt, s_tau);
+    }
+    if (lane < 32 && slot0 + sl < a.nq) {
+        a.cnt_out[slot0 + sl] = s_cnt[sl];
+        a.tau_out[slot0 + sl] = s_tau[sl];
+    }
+}
+
+__global__ __launch_bounds__(256) void prep_fp8(const double* __restrict__ X, const double* __restrict__ n2, int N,
+                                                int D, int Kp, uint8_t* __restrict__ A8, float* __restrict__ scale,
+                                                float* __restrict__ nrm)
+{
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    const double* x = X + (size_t)n * D;
+    double amax = 0.0;
+    for (int d = 0; d < D; d++) amax = fmax(amax, fabs(x[d]));
+    const double s = amax > 0.0 ? amax / 224.0 : 1.0;
+    scale[n] = (float)s;
+    nrm[n] = (float)n2[n];
+    uint32_t* row = reinterpret_cast<uint32_t*>(A8 + (size_t)n * Kp);
+    for (int d0 = 0; d0 < Kp; d0 += 4) {
+        float f[4];
+        for (int k = 0; k < 4; k++) f[k] = (d0 + k < D) ? (float)(x[d0 + k] / s) : 0.f;
+        int wv = 0;
+        wv = __builtin_amdgcn_cvt_pk_fp8_f32(f[0], f[1], wv, false);
+        wv = __builtin_amdgcn_cvt_pk_fp8_f32(f[2], f[3], wv, true);
+        row[d0 / 4] = (uint32_t)wv;
+    }
+}
+
 // ---------------------------------------------------------------- rerank
 struct RerankArgs {
     const double* X;      // N x D normalised FP64
@@ -508,6 +608,46 @@ struct RerankArgs {
     int* fail;            // per slot: 1 if the bound failed
 };
 
+// Exact FP64 squared distance, summed in index order without contraction
+// (the reference's euclidean_distance, vptree.hpp:95-101).
+TKA_DEV double exact_sqdist(const double* __restrict__ xq, const double* __restrict__ xj, int D)
+{
+    double dd = 0.0;
+    for (int d = 0; d < D; d++) {
+        const double t = dadd_sep(xq[d], -xj[d]);
+        dd = dadd_sep(dd, dmul_sep(t, t));
+    }
+    return dd;
+}
+
+// Sorts the first R registers' worth of keys (R in {1, 2, 4, 8}).
+TKA_DEV void sort_prefix(DKey (&v)[8], int R)
+{
+    if (R <= 1) {
+        DKey u[1] = {v[0]};
+        wave_bitonic_sort<1>(u);
+        v[0] = u[0];
+    } else if (R <= 2) {
+        DKey u[2] = {v[0], v[1]};
+        wave_bitonic_sort<2>(u);
+        v[0] = u[0];
+        v[1] = u[1];
+    } else if (R <= 4) {
+        DKey u[4] = {v[0], v[1], v[2], v[3]};
+        wave_bitonic_sort<4>(u);
+        for (int r = 0; r < 4; r++) v[r] = u[r];
+    } else {
+        wave_bitonic_sort<8>(v);
+    }
+}
+
+// One wave per row.  The kept candidates are sorted by their approximate
+// values v_j (|v_j - e_j| <= eps, e_j = ||c_j||^2 - 2 q.c_j); with v_(K) the
+// K-th smallest, any candidate with v_j > v_(K) + 2 eps has e_j > e_K and
+// cannot be in the top K, so only the prefix up to that threshold gets exact
+// FP64 distances.  The row is certified when its K-th exact distance lies
+// below (final threshold + ||q||^2 - eps): no candidate dropped during the
+// stream can then belong to the top K.
 __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
 {
     const int lane = threadIdx.x & 63;
@@ -517,50 +657,66 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     const int c = a.cnt[slot] < kCap ? a.cnt[slot] : kCap;
     const uint64_t* qbuf = a.buf + (size_t)slot * kCap;
     const double* xq = a.X + (size_t)q * a.D;
+    const double qn2 = a.n2[q];
+    const double eps = a.eps_a * sqrt(qn2) * a.rmax + a.eps_b * a.rmax * a.rmax + 1e-12 * (qn2 + a.rmax * a.rmax);
+    const int K = a.K;
+
+    // approximate order
+    uint64_t u[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int p = r * 64 + lane;
+        u[r] = p < c ? qbuf[p] : ~0ull;
+    }
+    wave_bitonic_sort<8>(u);
+    uint32_t vk_bits = 0;
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+        if (r * 64 + lane == K - 1) vk_bits = (uint32_t)(u[r] >> 32);
+    vk_bits = __shfl(vk_bits, (K - 1) & 63, 64);
+    const double thr = (double)ord2f(vk_bits) + 2.0 * eps;
+    int m = 0;  // candidates within the threshold (a prefix of the sorted list)
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int p = r * 64 + lane;
+        const bool in = p < c && (double)ord2f((uint32_t)(u[r] >> 32)) <= thr;
+        m += __popcll(__ballot(in));
+    }
+    if (c < K) m = c;
+
     DKey v[8];
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         const int p = r * 64 + lane;
-        if (p < c) {
-            const uint32_t j = (uint32_t)qbuf[p];
-            const double* xj = a.X + (size_t)j * a.D;
-            double dd = 0.0;
-            for (int d = 0; d < a.D; d++) {
-                const double t = dadd_sep(xq[d], -xj[d]);
-                dd = dadd_sep(dd, dmul_sep(t, t));
-            }
-            v[r].d = (uint64_t)__double_as_longlong(dd);
+        if (p < m) {
+            const uint32_t j = (uint32_t)u[r];
+            v[r].d = (uint64_t)__double_as_longlong(exact_sqdist(xq, a.X + (size_t)j * a.D, a.D));
             v[r].i = j;
         } else {
             v[r].d = ~0ull;
             v[r].i = 0xffffffffu;
         }
     }
-    wave_bitonic_sort<8>(v);
-    __shared__ double s_dk[4];
+    const int R = m <= 64 ? 1 : m <= 128 ? 2 : m <= 256 ? 4 : 8;
+    sort_prefix(v, R);
+    double dk = 0.0;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         const int p = r * 64 + lane;
-        if (p < a.K) {
-            a.idx[(size_t)q * a.K + p] = (int)v[r].i;
-            a.dist[(size_t)q * a.K + p] = __longlong_as_double((long long)v[r].d);
+        if (p < K) {
+            a.idx[(size_t)q * K + p] = (int)v[r].i;
+            a.dist[(size_t)q * K + p] = __longlong_as_double((long long)v[r].d);
         }
-        if (p == a.K - 1) s_dk[threadIdx.x >> 6] = __longlong_as_double((long long)v[r].d);
+        if (p == K - 1) dk = __longlong_as_double((long long)v[r].d);
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    dk = __shfl(dk, (K - 1) & 63, 64);
     if (lane == 0) {
         const float tau = a.tau[slot];
         int bad = 0;
-        if (c < a.K) {
+        if (c < K) {
             bad = 1;  // cannot happen with K <= N-1 unless values were non-finite
         } else if (isfinite(tau)) {
-            const double qn2 = a.n2[q];
-            const double bound = a.eps_a * sqrt(qn2) * a.rmax + a.eps_b * a.rmax * a.rmax +
-                                 1e-12 * (qn2 + a.rmax * a.rmax);
-            const double dk = s_dk[threadIdx.x >> 6];
-            bad = !(dk < (double)tau + qn2 - bound);
+            bad = !(dk < (double)tau + qn2 - eps);
         }
         a.fail[slot] = bad;
     }
@@ -617,6 +773,9 @@ void run_tiles(int dtype, const void* op_a, const void* op_b, const float* nrm, 
     const dim3 grid(ceil_div(nq, kQB));
     if (dtype == TKA_DIST_BF16) {
         hipLaunchKernelGGL(knn_tiles_bf16, grid, dim3(256), 0, s, (const __bf16*)op_a, (const __bf16*)op_b, Kp, ta);
+    } else if (dtype == TKA_DIST_FP8) {
+        hipLaunchKernelGGL(knn_tiles_fp8, grid, dim3(256), 0, s, (const uint8_t*)op_a, (const float*)op_b, nrm, Kp,
+                           ta);
     } else {
         hipLaunchKernelGGL(knn_tiles_fp32, grid, dim3(256), 0, s, (const float*)op_a, nrm, D, ta);
     }
@@ -632,7 +791,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     require(K <= kMaxKprime, TKA_E_WRONG_PARAMETER,
             "K = " + std::to_string(K) + " exceeds the engine's limit of " + std::to_string(kMaxKprime) +
                 " neighbours (perplexity <= 128)");
-    require(dtype == TKA_DIST_FP32 || dtype == TKA_DIST_BF16, TKA_E_UNSUPPORTED,
+    require(dtype == TKA_DIST_FP32 || dtype == TKA_DIST_BF16 || dtype == TKA_DIST_FP8, TKA_E_UNSUPPORTED,
             "distance dtype " + std::to_string(dtype) + " is not available");
     if (kprime <= 0) kprime = 2 * K;
     kprime = std::max(kprime, K);
@@ -652,6 +811,8 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     // Operands.
     DevBuf<float> Xf, nrm;
     DevBuf<__bf16> A, B;
+    DevBuf<uint8_t> A8;
+    DevBuf<float> scale8, nrm8;
     int Kp = 0;
     auto make_fp32 = [&]() {
         if (Xf.get()) return;
@@ -683,6 +844,20 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         const double gam = Kp * u32;
         eps_a = 2.0 * (6.2 * ub2 + 2.1 * gam);
         eps_b = 2.0 * (1.1 * ub2 + 1.1 * gam);
+    } else if (dtype == TKA_DIST_FP8) {
+        Kp = (D + 15) / 16 * 16;
+        A8.alloc((size_t)N * Kp, mb);
+        scale8.alloc(N, mb);
+        nrm8.alloc(N, mb);
+        hipLaunchKernelGGL(prep_fp8, dim3(ceil_div(N, 256)), dim3(256), 0, s, Xn, n2.get(), N, D, Kp, A8.get(),
+                           scale8.get(), nrm8.get());
+        TKA_LAUNCH_CHECK();
+        run_tiles(dtype, A8.get(), scale8.get(), nrm8.get(), N, D, Kp, kprime, nullptr, N, tr, mb, s);
+        // e4m3 keeps 4 significant bits (unit roundoff 2^-4): the bound is
+        // loose at any useful k'; FP8 rows are not re-solved (approximate mode)
+        const double u8 = std::ldexp(1.0, -4);
+        eps_a = 2.0 * (2.0 * (2 * u8 + u8 * u8) + 2.0 * D * u32);
+        eps_b = 2.0 * (2.0 * u32);
     } else {
         make_fp32();
         run_tiles(dtype, Xf.get(), nullptr, nrm.get(), N, D, 0, kprime, nullptr, N, tr, mb, s);
@@ -704,7 +879,8 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     TKA_HIP(hipMemcpyAsync(&hf, nfail.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     TKA_HIP(hipStreamSynchronize(s));
     if (st) st->certificate_failures = hf;
-    if (hf > 0) {
+    if (st) st->unresolved = hf;
+    if (hf > 0 && dtype != TKA_DIST_FP8) {
         // Deterministic order of the re-solved rows does not matter: each row
         // is solved independently and written to its own output slot.
         make_fp32();

@@ -1,8 +1,49 @@
 // Radix sort and exclusive scan kernels (wave64, LDS-ranked, stable).
+//
+// Sort = one counting launch + one launch per 8-bit digit ("one sweep"):
+//   rs_count     every tile histograms all digits of its keys at once and adds
+//                them to global per-pass counts; the last tile to arrive turns
+//                the counts into per-pass digit bases (and clears them for the
+//                next sort).  It also clears the tile status words.
+//   rs_onesweep  takes a tile number in arrival order, ranks its keys by the
+//                pass's digit (wave ballots + per-wave LDS counters), publishes
+//                its per-digit counts, sums its predecessors' counts by
+//                decoupled look-back (Merrill & Garland 2016; Adinets & Merrill
+//                2022) and scatters.  Tiles are numbered in the order their
+//                workgroups start, so a tile only waits on tiles that are
+//                already running: the wait always ends.  It is bounded anyway;
+//                a tile that gives up sets a fault word the host can read.
 #include "radix_sort.hpp"
 
 namespace tka {
 namespace {
+
+constexpr uint32_t kFlagAgg = 1u << 30;   // tile's own count published
+constexpr uint32_t kFlagInc = 2u << 30;   // inclusive prefix through this tile published
+constexpr uint32_t kValMask = (1u << 30) - 1;
+constexpr int kSpinLimit = 1 << 22;
+constexpr int kLookback = 16;             // predecessor tiles read per look-back round
+constexpr size_t kRsHeader = 16 + 2 * (size_t)kRsMaxPasses * 256;
+
+// Scratch layout (words): [0] arrival ticket, [1] fault, [2, 2+8) tile counters
+// per pass, then global counts and bases (8 x 256 each), then tile status
+// words (passes x tiles x 256).
+struct RsLayout {
+    uint32_t* ticket;
+    uint32_t* fault;
+    uint32_t* tile_ctr;
+    uint32_t* gcnt;
+    uint32_t* gbase;
+    uint32_t* status;
+};
+
+RsLayout rs_layout(uint32_t* s)
+{
+    return {s, s + 1, s + 2, s + 16, s + 16 + kRsMaxPasses * 256, s + kRsHeader};
+}
+
+TKA_DEV uint32_t ld_acq(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+TKA_DEV void st_rel(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
 TKA_DEV uint32_t wave_inclusive_scan(uint32_t v)
 {
@@ -34,80 +75,74 @@ TKA_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total)
     return before + inc - v;
 }
 
-template <class K>
-__global__ __launch_bounds__(kRsBlock) void rs_hist(const K* __restrict__ keys, size_t n, int shift,
-                                                    uint32_t* __restrict__ hist, int tiles)
+template <class K, int ITEMS>
+__global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys, size_t n, int passes, RsLayout L,
+                                                     int tiles)
 {
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0;
+    __shared__ uint32_t h[kRsMaxPasses][256];
+    __shared__ uint32_t sh[kRsBlock / 64];
+    __shared__ int is_last;
+    const int t = threadIdx.x;
+    for (int p = 0; p < passes; p++) {
+        h[p][t] = 0;
+        L.status[((size_t)p * tiles + blockIdx.x) * 256 + t] = 0;
+    }
+    if (blockIdx.x == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
     __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kRsTile;
+    const size_t base = (size_t)blockIdx.x * kRsBlock * ITEMS;
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
-        const size_t i = base + (size_t)k * kRsBlock + threadIdx.x;
-        if (i < n) atomicAdd(&h[(uint32_t)(keys[i] >> shift) & 255u], 1u);
+    for (int k = 0; k < ITEMS; k++) {
+        const size_t i = base + (size_t)k * kRsBlock + t;
+        if (i < n) {
+            const K key = keys[i];
+            for (int p = 0; p < passes; p++) atomicAdd(&h[p][(uint32_t)(key >> (8 * p)) & 255u], 1u);
+        }
     }
     __syncthreads();
-    hist[(size_t)threadIdx.x * tiles + blockIdx.x] = h[threadIdx.x];
-}
-
-// Exclusive scan of L words by one 1024-thread block (digit-major histogram).
-// `in` and `out` may alias (each thread reads its chunk before writing it).
-__global__ __launch_bounds__(1024) void rs_scan_single(const uint32_t* in, uint32_t* out, size_t L)
-{
-    __shared__ uint32_t sh[16];
-    const size_t per = (L + 1023) / 1024;
-    const size_t beg = threadIdx.x * per;
-    const size_t end = beg + per < L ? beg + per : L;
-    uint32_t s = 0;
-    for (size_t i = beg; i < end; i++) s += in[i];
-    uint32_t run = block_exclusive_scan(s, sh, nullptr);
-    for (size_t i = beg; i < end; i++) {
-        const uint32_t v = in[i];
-        out[i] = run;
-        run += v;
+    for (int p = 0; p < passes; p++) {
+        const uint32_t c = h[p][t];
+        if (c) atomicAdd(&L.gcnt[p * 256 + t], c);
     }
+    __threadfence();
+    __syncthreads();
+    if (t == 0) is_last = (atomicAdd(L.ticket, 1u) == (uint32_t)(tiles - 1));
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();  // acquire: every tile's counts are in
+    for (int p = 0; p < passes; p++) {
+        const uint32_t c = ld_acq(&L.gcnt[p * 256 + t]);
+        L.gbase[p * 256 + t] = block_exclusive_scan(c, sh, nullptr);
+        L.gcnt[p * 256 + t] = 0;
+    }
+    if (t == 0) atomicExch(L.ticket, 0u);
 }
 
 // Stable scatter of one tile.  Item k of lane l in wave w sits at tile
-// position w*64*kRsItems + k*64 + l, so ranking items in (w, k, l) order keeps
+// position w*64*ITEMS + k*64 + l, so ranking items in (w, k, l) order keeps
 // the input order among equal digits.
-template <class K, class V, bool kFused>
-__global__ __launch_bounds__(kRsBlock) void rs_scatter(const K* __restrict__ kin, const V* __restrict__ vin,
-                                                       K* __restrict__ kout, V* __restrict__ vout, size_t n, int shift,
-                                                       const uint32_t* __restrict__ hist,
-                                                       const uint32_t* __restrict__ offs, int tiles)
+template <class K, class V, int ITEMS>
+__global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ kin, const V* __restrict__ vin,
+                                                        K* __restrict__ kout, V* __restrict__ vout, size_t n,
+                                                        int pass, RsLayout L, int tiles)
 {
     constexpr int kWaves = kRsBlock / 64;
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t base[256];
-    __shared__ uint32_t sh[kWaves];
+    __shared__ uint32_t tile_sh;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int t = threadIdx.x; t < kWaves * 256; t += kRsBlock) (&wcnt[0][0])[t] = 0;
-    {
-        const int d = threadIdx.x;
-        if (kFused) {
-            uint32_t tot = 0, before = 0;
-            for (int b = 0; b < tiles; b++) {
-                const uint32_t v = hist[(size_t)d * tiles + b];
-                tot += v;
-                before += (b < (int)blockIdx.x) ? v : 0u;
-            }
-            const uint32_t ex = block_exclusive_scan(tot, sh, nullptr);
-            base[d] = ex + before;
-        } else {
-            base[d] = offs[(size_t)d * tiles + blockIdx.x];
-        }
-    }
+    if (threadIdx.x == 0) tile_sh = atomicAdd(&L.tile_ctr[pass], 1u);
     __syncthreads();
+    const uint32_t tile = tile_sh;
+    const int shift = 8 * pass;
 
-    K key[kRsItems];
-    V val[kRsItems];
-    uint32_t rank[kRsItems];
-    int dig[kRsItems];
-    const size_t tile0 = (size_t)blockIdx.x * kRsTile + (size_t)w * 64 * kRsItems + lane;
+    K key[ITEMS];
+    V val[ITEMS];
+    uint32_t rank[ITEMS];
+    int dig[ITEMS];
+    const size_t tile0 = (size_t)tile * kRsBlock * ITEMS + (size_t)w * 64 * ITEMS + lane;
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const size_t i = tile0 + (size_t)k * 64;
         const bool ok = i < n;
         key[k] = ok ? kin[i] : K(0);
@@ -116,7 +151,7 @@ __global__ __launch_bounds__(kRsBlock) void rs_scatter(const K* __restrict__ kin
     }
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const int d = dig[k];
         const bool ok = d >= 0;
         uint64_t m = __ballot(ok);
@@ -136,17 +171,59 @@ __global__ __launch_bounds__(kRsBlock) void rs_scatter(const K* __restrict__ kin
     __syncthreads();
     {
         const int d = threadIdx.x;
-        uint32_t run = 0;
+        uint32_t cnt = 0;
 #pragma unroll
         for (int ww = 0; ww < kWaves; ww++) {
             const uint32_t c = wcnt[ww][d];
-            wcnt[ww][d] = run;
-            run += c;
+            wcnt[ww][d] = cnt;
+            cnt += c;
         }
+        uint32_t* st = L.status + (size_t)pass * tiles * 256 + d;
+        uint32_t excl = 0;
+        if (tile == 0) {
+            st_rel(st, kFlagInc | cnt);
+        } else {
+            st_rel(st + (size_t)tile * 256, kFlagAgg | cnt);
+            // Look back over a window of kLookback predecessors per round
+            // (independent loads in flight together), nearest first: add
+            // published counts until an inclusive prefix; stop the round at
+            // a tile that has not published yet and read it again.
+            int j = (int)tile - 1, spins = 0;
+            bool done = false;
+            while (!done) {
+                uint32_t v[kLookback];
+#pragma unroll
+                for (int w = 0; w < kLookback; w++) v[w] = (j - w >= 0) ? ld_acq(st + (size_t)(j - w) * 256) : 0u;
+                // branch-free over the window (keeps v[] in registers); tile 0
+                // is always inclusive, so entries past it are never taken
+                int used = 0;
+                bool stop = false;
+#pragma unroll
+                for (int w = 0; w < kLookback; w++) {
+                    const uint32_t f = v[w] & ~kValMask;
+                    const bool take = !stop && f != 0;
+                    excl += take ? (v[w] & kValMask) : 0u;
+                    used = take ? w + 1 : used;
+                    done = done || (take && f == kFlagInc);
+                    stop = stop || f == 0 || f == kFlagInc;
+                }
+                if (done) break;
+                j -= used;
+                if (used == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(L.fault, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+            st_rel(st + (size_t)tile * 256, kFlagInc | (excl + cnt));
+        }
+        base[d] = L.gbase[pass * 256 + d] + excl;
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const int d = dig[k];
         if (d >= 0) {
             const uint32_t pos = base[d] + wcnt[w][d] + rank[k];
@@ -157,77 +234,108 @@ __global__ __launch_bounds__(kRsBlock) void rs_scatter(const K* __restrict__ kin
 }
 
 // ---------------------------------------------------------------- scan
+constexpr int kScanItems = kScanTile / kRsBlock;
+
 __global__ __launch_bounds__(kRsBlock) void scan_tile_sums(const uint32_t* __restrict__ in, size_t n,
                                                            uint32_t* __restrict__ sums)
 {
     __shared__ uint32_t sh[kRsBlock / 64];
-    const size_t base = (size_t)blockIdx.x * kRsTile;
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
     uint32_t s = 0;
-    for (int k = 0; k < kRsItems; k++) {
-        const size_t i = base + (size_t)threadIdx.x * kRsItems + k;
-        if (i < n) s += in[i];
-    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++)
+        if (base + k < n) s += in[base + k];
     uint32_t tot;
     block_exclusive_scan(s, sh, &tot);
     if (threadIdx.x == 0) sums[blockIdx.x] = tot;
 }
 
+// Exclusive scan of L words by one 1024-thread block; `in` and `out` may
+// alias (each thread reads its chunk before writing it).
+__global__ __launch_bounds__(1024) void scan_single(const uint32_t* in, uint32_t* out, size_t L)
+{
+    __shared__ uint32_t sh[16];
+    const size_t per = (L + 1023) / 1024;
+    const size_t beg = threadIdx.x * per;
+    const size_t end = beg + per < L ? beg + per : L;
+    uint32_t s = 0;
+    for (size_t i = beg; i < end; i++) s += in[i];
+    uint32_t run = block_exclusive_scan(s, sh, nullptr);
+    for (size_t i = beg; i < end; i++) {
+        const uint32_t v = in[i];
+        out[i] = run;
+        run += v;
+    }
+}
+
 __global__ __launch_bounds__(kRsBlock) void scan_tiles(const uint32_t* __restrict__ in, size_t n,
                                                        const uint32_t* __restrict__ tile_off,
-                                                       uint32_t* __restrict__ out)
+                                                       uint32_t* __restrict__ out, int write_total)
 {
     __shared__ uint32_t sh[kRsBlock / 64];
-    const size_t base = (size_t)blockIdx.x * kRsTile + (size_t)threadIdx.x * kRsItems;
-    uint32_t v[kRsItems];
+    const size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
+    uint32_t v[kScanItems];
     uint32_t s = 0;
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
+    for (int k = 0; k < kScanItems; k++) {
         v[k] = (base + k < n) ? in[base + k] : 0u;
         s += v[k];
     }
     uint32_t run = block_exclusive_scan(s, sh, nullptr) + tile_off[blockIdx.x];
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
+    for (int k = 0; k < kScanItems; k++) {
         if (base + k < n) out[base + k] = run;
         run += v[k];
     }
-    const unsigned last = (unsigned)((n + kRsTile - 1) / kRsTile) - 1;
-    if (blockIdx.x == last && threadIdx.x == kRsBlock - 1) out[n] = run;
+    const unsigned last = (unsigned)((n + kScanTile - 1) / kScanTile) - 1;
+    if (write_total && blockIdx.x == last && threadIdx.x == kRsBlock - 1) out[n] = run;
+}
+
+void scan_impl(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch, bool write_total, hipStream_t s)
+{
+    const int tiles = (int)ceil_div(n, kScanTile);
+    hipLaunchKernelGGL(scan_tile_sums, dim3(tiles), dim3(kRsBlock), 0, s, in, n, scratch);
+    TKA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scan_single, dim3(1), dim3(1024), 0, s, scratch, scratch, (size_t)tiles);
+    TKA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(scan_tiles, dim3(tiles), dim3(kRsBlock), 0, s, in, n, scratch, out, write_total ? 1 : 0);
+    TKA_LAUNCH_CHECK();
+}
+
+template <class K, class V, int ITEMS>
+bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch, hipStream_t s)
+{
+    const int tiles = (int)ceil_div(n, (size_t)kRsBlock * ITEMS);
+    const int passes = (end_bit + 7) / 8;
+    const RsLayout L = rs_layout(scratch);
+    hipLaunchKernelGGL((rs_count<K, ITEMS>), dim3(tiles), dim3(kRsBlock), 0, s, keys, n, passes, L, tiles);
+    TKA_LAUNCH_CHECK();
+    K* ka = keys;
+    V* va = vals;
+    K* kb = keys_alt;
+    V* vb = vals_alt;
+    for (int p = 0; p < passes; p++) {
+        hipLaunchKernelGGL((rs_onesweep<K, V, ITEMS>), dim3(tiles), dim3(kRsBlock), 0, s, ka, va, kb, vb, n, p, L,
+                           tiles);
+        TKA_LAUNCH_CHECK();
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
+    return (passes & 1) != 0;
 }
 
 }  // namespace
+
+size_t radix_sort_scratch_words(size_t n) { return kRsHeader + (size_t)kRsMaxPasses * 256 * radix_tiles(n ? n : 1); }
 
 template <class K, class V>
 bool radix_sort_pairs(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch,
                       hipStream_t stream)
 {
-    if (n <= 1) return false;
-    const int tiles = (int)ceil_div(n, kRsTile);
-    uint32_t* hist = scratch;
-    uint32_t* offs = scratch + (size_t)256 * tiles;
-    K* ka = keys;
-    V* va = vals;
-    K* kb = keys_alt;
-    V* vb = vals_alt;
-    bool in_alt = false;
-    for (int shift = 0; shift < end_bit; shift += 8) {
-        hipLaunchKernelGGL(rs_hist<K>, dim3(tiles), dim3(kRsBlock), 0, stream, ka, n, shift, hist, tiles);
-        TKA_LAUNCH_CHECK();
-        if (tiles <= kRsFusedMaxTiles) {
-            hipLaunchKernelGGL((rs_scatter<K, V, true>), dim3(tiles), dim3(kRsBlock), 0, stream, ka, va, kb, vb, n,
-                               shift, hist, nullptr, tiles);
-        } else {
-            hipLaunchKernelGGL(rs_scan_single, dim3(1), dim3(1024), 0, stream, hist, offs, (size_t)256 * tiles);
-            TKA_LAUNCH_CHECK();
-            hipLaunchKernelGGL((rs_scatter<K, V, false>), dim3(tiles), dim3(kRsBlock), 0, stream, ka, va, kb, vb, n,
-                               shift, hist, offs, tiles);
-        }
-        TKA_LAUNCH_CHECK();
-        std::swap(ka, kb);
-        std::swap(va, vb);
-        in_alt = !in_alt;
-    }
-    return in_alt;
+    if (n <= 1 || end_bit <= 0) return false;
+    require(n <= kRsMaxItems, TKA_E_UNSUPPORTED, "radix sort: more than 2^30 - 1 items");
+    require(end_bit <= 8 * (int)sizeof(K), TKA_E_INTERNAL, "radix sort: end_bit beyond the key width");
+    return sort_impl<K, V, kRsItems>(keys, vals, keys_alt, vals_alt, n, end_bit, scratch, stream);
 }
 
 template bool radix_sort_pairs<uint32_t, uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, size_t, int, uint32_t*,
@@ -237,20 +345,21 @@ template bool radix_sort_pairs<uint64_t, uint64_t>(uint64_t*, uint64_t*, uint64_
 template bool radix_sort_pairs<uint64_t, uint32_t>(uint64_t*, uint32_t*, uint64_t*, uint32_t*, size_t, int, uint32_t*,
                                                    hipStream_t);
 
+uint32_t radix_sort_fault(const uint32_t* scratch, hipStream_t stream)
+{
+    uint32_t f = 0;
+    TKA_HIP(hipMemcpyAsync(&f, scratch + 1, sizeof(f), hipMemcpyDeviceToHost, stream));
+    TKA_HIP(hipStreamSynchronize(stream));
+    return f;
+}
+
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch, hipStream_t stream)
 {
     if (n == 0) {
         TKA_HIP(hipMemsetAsync(out, 0, sizeof(uint32_t), stream));
         return;
     }
-    const int tiles = (int)ceil_div(n, kRsTile);
-    uint32_t* sums = scratch;  // tiles words; scanned in place into sums (exclusive)
-    hipLaunchKernelGGL(scan_tile_sums, dim3(tiles), dim3(kRsBlock), 0, stream, in, n, sums);
-    TKA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(rs_scan_single, dim3(1), dim3(1024), 0, stream, sums, sums, (size_t)tiles);
-    TKA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(scan_tiles, dim3(tiles), dim3(kRsBlock), 0, stream, in, n, sums, out);
-    TKA_LAUNCH_CHECK();
+    scan_impl(in, out, n, scratch, true, stream);
 }
 
 }  // namespace tka
