@@ -144,7 +144,7 @@ typedef struct tka_session tka_session;
 typedef struct tka_kernel_timing {
     int64_t launches_build;      /* tree builds (keys, sort, nodes) timed */
     int64_t launches_repulsion;  /* Barnes-Hut traversals timed */
-    int64_t launches_step;       /* fused attraction + update launches timed */
+    int64_t launches_step;       /* optimiser updates timed (with the attraction fused when theta = 0) */
     int64_t launches_exact;      /* exact all-pairs repulsion launches timed */
     double ms_build;             /* summed HIP-event durations on the session stream */
     double ms_repulsion;
@@ -154,6 +154,9 @@ typedef struct tka_kernel_timing {
     int64_t nnz;                 /* entries of P (edges per attraction pass) */
     int32_t n;
     int32_t dims;
+    int64_t launches_attraction; /* Barnes-Hut loop: attraction passes timed (second stream,
+                                    concurrent with the tree build and traversal) */
+    double ms_attraction;
 } tka_kernel_timing;
 
 TKA_API tka_status tka_session_create(const double* X, int32_t N, int32_t D, int32_t no_dims, double perplexity,

@@ -67,7 +67,7 @@ class KernelTiming(C.Structure):
         ("launches_build", C.c_int64), ("launches_repulsion", C.c_int64), ("launches_step", C.c_int64),
         ("launches_exact", C.c_int64), ("ms_build", C.c_double), ("ms_repulsion", C.c_double),
         ("ms_step", C.c_double), ("ms_exact", C.c_double), ("interactions", C.c_int64), ("nnz", C.c_int64),
-        ("n", C.c_int32), ("dims", C.c_int32),
+        ("n", C.c_int32), ("dims", C.c_int32), ("launches_attraction", C.c_int64), ("ms_attraction", C.c_double),
     ]
 
 

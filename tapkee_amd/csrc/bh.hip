@@ -8,8 +8,8 @@
 // leaf or max(hh, hw)/sqrt(D) < theta (:335).
 //
 // Construction (all integer, bit-exact):
-//   box    one workgroup reduces the bounding-box partials, in a fixed order,
-//          to the root cell.
+//   box    every key workgroup reduces the bounding-box partials, in a fixed
+//          order, to the root cell.
 //   keys   point -> 32+32 bit Morton key of its leaf cell, y bit above x bit,
 //          so digit order 0..3 is NW, NE, SW, SE and sorting keys gives the
 //          reference's depth-first child order.  Quantisation is
@@ -51,7 +51,7 @@ namespace {
 constexpr int kB = kMortonBits;
 constexpr int kTreeScanItems = 4;
 constexpr int kTreeScanTile = 256 * kTreeScanItems;
-constexpr int kBoxReduceBlock = 1024;
+constexpr int kHiPasses = 4;  // radix passes over the 32-bit high key words
 
 // ---------------------------------------------------------------- bbox
 template <class T>
@@ -99,55 +99,6 @@ __global__ __launch_bounds__(kBoxBlock) void bbox_part(const T* __restrict__ Y, 
     }
 }
 
-// Root cell of the points from their box partials (quadtree.hpp:104-129):
-// one workgroup, fixed reduction order.  The mean is removed from Y by the
-// key kernel, so the root cell is centred at 0.
-__global__ __launch_bounds__(kBoxReduceBlock) void bh_box(const double* __restrict__ part, int nb, int N,
-                                                          RootBox* __restrict__ root)
-{
-    __shared__ double sh[6][kBoxReduceBlock / 64];
-    double v[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-    for (int b = threadIdx.x; b < nb; b += kBoxReduceBlock) {
-        const double* p = part + (size_t)kBoxStride * b;
-        v[0] += p[0];
-        v[1] += p[1];
-        v[2] = fmin(v[2], p[3]);
-        v[3] = fmin(v[3], p[4]);
-        v[4] = fmax(v[4], p[5]);
-        v[5] = fmax(v[5], p[6]);
-    }
-    v[0] = wave_sum(v[0]);
-    v[1] = wave_sum(v[1]);
-    v[2] = wave_min(v[2]);
-    v[3] = wave_min(v[3]);
-    v[4] = wave_max(v[4]);
-    v[5] = wave_max(v[5]);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
-        for (int k = 0; k < 6; k++) sh[k][w] = v[k];
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    double r[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
-    for (int ww = 0; ww < kBoxReduceBlock / 64; ww++) {
-        r[0] += sh[0][ww];
-        r[1] += sh[1][ww];
-        r[2] = fmin(r[2], sh[2][ww]);
-        r[3] = fmin(r[3], sh[3][ww]);
-        r[4] = fmax(r[4], sh[4][ww]);
-        r[5] = fmax(r[5], sh[5][ww]);
-    }
-    const double mx = r[0] / (double)N, my = r[1] / (double)N;
-    const double hw = fmax(r[4] - mx, mx - r[2]) + 1e-5;
-    const double hh = fmax(r[5] - my, my - r[3]) + 1e-5;
-    RootBox rb;
-    rb.mean_x = mx;
-    rb.mean_y = my;
-    rb.hw = hw;
-    rb.hh = hh;
-    rb.size0 = fmax(hw, hh);
-    *root = rb;
-}
-
 // ---------------------------------------------------------------- keys
 TKA_DEV uint32_t spread16(uint32_t v)
 {
@@ -167,24 +118,43 @@ TKA_DEV uint32_t quantise(double t)
     return (uint32_t)c;
 }
 
+// One radix-sort tile of points per workgroup: root cell (every workgroup
+// reduces the box partials itself; workgroup 0 stores it), centre, key, then
+// the sort's counting step on the high words (so the sort needs no counting
+// launch).
 template <class T>
-__global__ __launch_bounds__(256) void bh_keys(T* __restrict__ Y, int N, const RootBox* __restrict__ root,
-                                               uint32_t* __restrict__ khi, uint32_t* __restrict__ klo,
-                                               uint32_t* __restrict__ perm, unsigned long long* __restrict__ mask)
+__global__ __launch_bounds__(kRsBlock) void bh_keys(T* __restrict__ Y, int N, const double* __restrict__ box_part,
+                                                    int nbox, RootBox* __restrict__ root,
+                                                    uint32_t* __restrict__ khi, uint32_t* __restrict__ klo,
+                                                    uint32_t* __restrict__ perm,
+                                                    unsigned long long* __restrict__ mask, RsLayout L, int tiles)
 {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= N) return;
-    const double mx = root->mean_x, my = root->mean_y, hw = root->hw, hh = root->hh;
-    const T x = (T)((double)Y[2 * (size_t)i] - mx);
-    const T y = (T)((double)Y[2 * (size_t)i + 1] - my);
-    Y[2 * (size_t)i] = x;
-    Y[2 * (size_t)i + 1] = y;
-    const uint32_t qx = quantise(((double)x + hw) / (2.0 * hw));
-    const uint32_t qy = quantise(((double)y + hh) / (2.0 * hh));
-    khi[i] = (spread16(qy >> 16) << 1) | spread16(qx >> 16);
-    klo[i] = (spread16(qy) << 1) | spread16(qx);
-    perm[i] = (uint32_t)i;
-    mask[i] = 0ull;
+    const RootBox rb = reduce_root_box<kRsBlock>(box_part, nbox, N);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *root = rb;
+    const double mx = rb.mean_x, my = rb.mean_y, hw = rb.hw, hh = rb.hh;
+    uint32_t hk[kRsItems];
+    bool ok[kRsItems];
+    const int base = blockIdx.x * kRsBlock * kRsItems + threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < kRsItems; k++) {
+        const int i = base + k * kRsBlock;
+        ok[k] = i < N;
+        hk[k] = 0u;
+        if (ok[k]) {
+            const T x = (T)((double)Y[2 * (size_t)i] - mx);
+            const T y = (T)((double)Y[2 * (size_t)i + 1] - my);
+            Y[2 * (size_t)i] = x;
+            Y[2 * (size_t)i + 1] = y;
+            const uint32_t qx = quantise(((double)x + hw) / (2.0 * hw));
+            const uint32_t qy = quantise(((double)y + hh) / (2.0 * hh));
+            hk[k] = (spread16(qy >> 16) << 1) | spread16(qx >> 16);
+            khi[i] = hk[k];
+            klo[i] = (spread16(qy) << 1) | spread16(qx);
+            perm[i] = (uint32_t)i;
+            mask[i] = 0ull;
+        }
+    }
+    rs_count_tile<uint32_t>(hk, ok, kHiPasses, L, (int)blockIdx.x, tiles);
 }
 
 // Full 64-bit keys in sorted order, the final point order and the sorted
@@ -440,21 +410,19 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
                                                const typename Vec2<T>::type* __restrict__ ys,
                                                const uint32_t* __restrict__ perm, int N,
                                                const uint32_t* __restrict__ pre, const double* __restrict__ psx,
-                                               const double* __restrict__ psy, const RootBox* __restrict__ root,
-                                               typename Vec4<T>::type* __restrict__ nodes, int* __restrict__ aux)
+                                               const double* __restrict__ psy, BhNode<T>* __restrict__ nodes)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= N) return;
     {  // leaf j
         const unsigned long long m = mask[j];
         const uint32_t t = pre[j] + (uint32_t)__popcll(m);
-        typename Vec4<T>::type nd;
+        BhNode<T> nd;
         nd.x = ys[j].x;
         nd.y = ys[j].y;
-        nd.z = (T)1;
-        nd.w = (T)-1;
+        nd.cl = 1u;
+        nd.link = kLeafBit | perm[j];
         nodes[t] = nd;
-        aux[t] = (int)perm[j];
     }
     if (j < N - 1) {
         const int lvl = node_lvl[j];
@@ -463,14 +431,12 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
             const unsigned long long m = mask[a];
             const uint32_t t = pre[a] + (uint32_t)__popcll(m & ((1ull << lvl) - 1ull));
             const double cnt = (double)(b - a);
-            const double size = ldexp(root->size0, -lvl);
-            typename Vec4<T>::type nd;
+            BhNode<T> nd;
             nd.x = (T)((psx[b] - psx[a]) / cnt);
             nd.y = (T)((psy[b] - psy[a]) / cnt);
-            nd.z = (T)cnt;
-            nd.w = (T)(size * size);
+            nd.cl = (uint32_t)(b - a) | ((uint32_t)lvl << kCountBits);
+            nd.link = pre[b];
             nodes[t] = nd;
-            aux[t] = (int)pre[b];
         }
     }
 }
@@ -481,13 +447,13 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
 // (size^2 < theta^2 D) interacts with count*q and the walk jumps past its
 // subtree; otherwise the walk descends to t+1.  Returns the next node.
 template <class T>
-TKA_DEV int bh_visit(const typename Vec4<T>::type& nd, int ax, int t, int p, T px, T py, T theta2, T& fx, T& fy, T& z,
+TKA_DEV int bh_visit(const BhNode<T>& nd, int t, uint32_t p, T px, T py, T theta2, T size0sq, T& fx, T& fy, T& z,
                      unsigned& cnt)
 {
     const T dx = px - nd.x, dy = py - nd.y;
     const T D = dx * dx + dy * dy;
-    if (nd.w < (T)0) {
-        if (ax != p) {
+    if (nd.link & kLeafBit) {
+        if ((nd.link & ~kLeafBit) != p) {
             const T q = (T)1 / ((T)1 + D);
             z += q;
             const T q2 = q * q;
@@ -497,15 +463,18 @@ TKA_DEV int bh_visit(const typename Vec4<T>::type& nd, int ax, int t, int p, T p
         }
         return t + 1;
     }
-    if (nd.w < theta2 * D) {
+    const int lvl = (int)(nd.cl >> kCountBits);
+    const T size2 = ldexp(size0sq, -2 * lvl);  // (size0 2^-lvl)^2, exact scaling
+    if (size2 < theta2 * D) {
         const T q = (T)1 / ((T)1 + D);
-        const T cq = nd.z * q;
+        const T cq = (T)(nd.cl & ((1u << kCountBits) - 1u)) * q;
         z += cq;
         const T m = cq * q;
         fx += m * dx;
         fy += m * dy;
         cnt++;
-        return ax > t ? ax : t + 1;  // skip pointers point forward; never loop
+        const int sk = (int)nd.link;
+        return sk > t ? sk : t + 1;  // skip pointers point forward; never loop
     }
     return t + 1;
 }
@@ -513,11 +482,12 @@ TKA_DEV int bh_visit(const typename Vec4<T>::type& nd, int ax, int t, int p, T p
 // One thread per point, in Morton order, so the lanes of a wave walk similar
 // paths and their node gathers mostly hit the same cache lines.
 template <class T>
-__global__ __launch_bounds__(256) void bh_traverse(const typename Vec4<T>::type* __restrict__ nodes,
-                                                   const int* __restrict__ aux, const uint32_t* __restrict__ pre,
+__global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__ nodes,
+                                                   const uint32_t* __restrict__ pre,
                                                    const typename Vec2<T>::type* __restrict__ ys,
                                                    const uint32_t* __restrict__ perm, int N, T theta2,
-                                                   T* __restrict__ neg, double* __restrict__ zpart,
+                                                   const RootBox* __restrict__ root, T* __restrict__ neg,
+                                                   double* __restrict__ zpart,
                                                    unsigned long long* __restrict__ interactions)
 {
     __shared__ double sh[4];
@@ -526,7 +496,8 @@ __global__ __launch_bounds__(256) void bh_traverse(const typename Vec4<T>::type*
     double zd = 0.0;
     unsigned cnt = 0;
     const int total = (int)pre[N];
-    const int p = active ? (int)perm[j] : -1;
+    const T size0sq = (T)(root->size0 * root->size0);
+    const uint32_t p = active ? perm[j] : 0u;
     T px = (T)0, py = (T)0;
     if (active) {
         px = ys[j].x;
@@ -535,7 +506,7 @@ __global__ __launch_bounds__(256) void bh_traverse(const typename Vec4<T>::type*
     T fx = 0, fy = 0, z = 0;
     if (active) {
         int t = 0;
-        while (t < total) t = bh_visit<T>(nodes[t], aux[t], t, p, px, py, theta2, fx, fy, z, cnt);
+        while (t < total) t = bh_visit<T>(nodes[t], t, p, px, py, theta2, size0sq, fx, fy, z, cnt);
     }
     if (active) {
         neg[2 * (size_t)p] = fx;
@@ -555,6 +526,7 @@ __global__ __launch_bounds__(256) void bh_traverse(const typename Vec4<T>::type*
 template <class T>
 void BhTree<T>::alloc(int n, MemBudget* mb)
 {
+    require(n <= kMaxTreePoints, TKA_E_UNSUPPORTED, "Barnes-Hut tree: at most 2^26 - 1 points");
     N = n;
     keys.alloc(n, mb);
     keys_alt.alloc(n, mb);
@@ -575,7 +547,6 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     sy.alloc(n + 1, mb);
     part.alloc(3 * (size_t)ceil_div(n, kTreeScanTile), mb);
     nodes.alloc(2 * (size_t)n, mb);
-    aux.alloc(2 * (size_t)n, mb);
     root.alloc(1, mb);
 }
 
@@ -587,17 +558,29 @@ void bbox_partials_device(const T* Y, int N, double* part, hipStream_t s)
 }
 
 template <class T>
-void bh_build(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s)
+void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s)
 {
     require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "tree storage too small");
+    const int tiles = (int)radix_tiles((size_t)N);
+    hipLaunchKernelGGL(bh_keys<T>, dim3(tiles), dim3(kRsBlock), 0, s, Y, N, box_part, nbox, tr.root.get(),
+                       tr.keys.get(),
+                       tr.klo.get(), tr.perm.get(), tr.mask.get(), rs_layout(tr.sort_scratch.get()), tiles);
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T>
+void bh_build(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s)
+{
+    bh_build_keys<T>(Y, N, box_part, nbox, tr, s);
+    bh_build_tree<T>(Y, N, tr, s);
+}
+
+template <class T>
+void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s)
+{
     const unsigned g = ceil_div(N, 256);
-    hipLaunchKernelGGL(bh_box, dim3(1), dim3(kBoxReduceBlock), 0, s, box_part, nbox, N, tr.root.get());
-    TKA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bh_keys<T>, dim3(g), dim3(256), 0, s, Y, N, tr.root.get(), tr.keys.get(), tr.klo.get(),
-                       tr.perm.get(), tr.mask.get());
-    TKA_LAUNCH_CHECK();
-    const bool alt = radix_sort_pairs<uint32_t, uint32_t>(tr.keys.get(), tr.perm.get(), tr.keys_alt.get(),
-                                                          tr.perm_alt.get(), N, 32, tr.sort_scratch.get(), s);
+    const bool alt = radix_sort_pairs_counted<uint32_t, uint32_t>(
+        tr.keys.get(), tr.perm.get(), tr.keys_alt.get(), tr.perm_alt.get(), N, 8 * kHiPasses, tr.sort_scratch.get(), s);
     const uint32_t* hk = alt ? tr.keys_alt.get() : tr.keys.get();
     const uint32_t* hp = alt ? tr.perm_alt.get() : tr.perm.get();
     hipLaunchKernelGGL(bh_fix<T>, dim3(g), dim3(256), 0, s, hk, hp, tr.klo.get(), Y, N, tr.k64.get(), tr.p2.get(),
@@ -615,8 +598,7 @@ void bh_build(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipS
                        tr.pre.get(), tr.sx.get(), tr.sy.get());
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(bh_emit<T>, dim3(g), dim3(256), 0, s, tr.mask.get(), tr.node_a.get(), tr.node_b.get(),
-                       tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(), tr.root.get(),
-                       tr.nodes.get(), tr.aux.get());
+                       tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(), tr.nodes.get());
     TKA_LAUNCH_CHECK();
 }
 
@@ -626,8 +608,8 @@ int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* 
 {
     (void)Y;
     const int g = bh_zblocks(tr.N);
-    hipLaunchKernelGGL((bh_traverse<T>), dim3(g), dim3(256), 0, s, tr.nodes.get(), tr.aux.get(), tr.pre.get(),
-                       tr.ys.get(), tr.sorted_perm, tr.N, (T)(theta * theta), neg, zpart, interactions);
+    hipLaunchKernelGGL((bh_traverse<T>), dim3(g), dim3(256), 0, s, tr.nodes.get(), tr.pre.get(), tr.ys.get(),
+                       tr.sorted_perm, tr.N, (T)(theta * theta), tr.root.get(), neg, zpart, interactions);
     TKA_LAUNCH_CHECK();
     return g;
 }
@@ -638,6 +620,10 @@ template void bbox_partials_device<float>(const float*, int, double*, hipStream_
 template void bbox_partials_device<double>(const double*, int, double*, hipStream_t);
 template void bh_build<float>(float*, int, const double*, int, BhTree<float>&, hipStream_t);
 template void bh_build<double>(double*, int, const double*, int, BhTree<double>&, hipStream_t);
+template void bh_build_keys<float>(float*, int, const double*, int, BhTree<float>&, hipStream_t);
+template void bh_build_keys<double>(double*, int, const double*, int, BhTree<double>&, hipStream_t);
+template void bh_build_tree<float>(const float*, int, BhTree<float>&, hipStream_t);
+template void bh_build_tree<double>(const double*, int, BhTree<double>&, hipStream_t);
 template int bh_repulsion<float>(const BhTree<float>&, const float*, double, float*, double*, unsigned long long*,
                                  hipStream_t);
 template int bh_repulsion<double>(const BhTree<double>&, const double*, double, double*, double*,

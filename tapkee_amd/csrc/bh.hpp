@@ -40,6 +40,76 @@ struct RootBox {
     double size0;           // max(hw, hh): half-width used by the acceptance test at level 0
 };
 
+// Root cell of the points from nb box partials (quadtree.hpp:104-129),
+// computed by all BLOCK threads of a workgroup in a fixed order, so every
+// workgroup that calls it gets the same bits, and returned to all threads.
+// The mean is removed from Y by the key kernel, so the root cell is centred
+// at 0.
+template <int BLOCK>
+TKA_DEV RootBox reduce_root_box(const double* part, int nb, int N)
+{
+    __shared__ double sh[6][BLOCK / 64];
+    __shared__ RootBox out;
+    double v[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    for (int b = threadIdx.x; b < nb; b += BLOCK) {
+        const double* p = part + (size_t)kBoxStride * b;
+        v[0] += p[0];
+        v[1] += p[1];
+        v[2] = fmin(v[2], p[3]);
+        v[3] = fmin(v[3], p[4]);
+        v[4] = fmax(v[4], p[5]);
+        v[5] = fmax(v[5], p[6]);
+    }
+    v[0] = wave_sum(v[0]);
+    v[1] = wave_sum(v[1]);
+    v[2] = wave_min(v[2]);
+    v[3] = wave_min(v[3]);
+    v[4] = wave_max(v[4]);
+    v[5] = wave_max(v[5]);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) sh[k][w] = v[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[6] = {0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+        for (int ww = 0; ww < BLOCK / 64; ww++) {
+            r[0] += sh[0][ww];
+            r[1] += sh[1][ww];
+            r[2] = fmin(r[2], sh[2][ww]);
+            r[3] = fmin(r[3], sh[3][ww]);
+            r[4] = fmax(r[4], sh[4][ww]);
+            r[5] = fmax(r[5], sh[5][ww]);
+        }
+        const double mx = r[0] / (double)N, my = r[1] / (double)N;
+        const double hw = fmax(r[4] - mx, mx - r[2]) + 1e-5;
+        const double hh = fmax(r[5] - my, my - r[3]) + 1e-5;
+        out.mean_x = mx;
+        out.mean_y = my;
+        out.hw = hw;
+        out.hh = hh;
+        out.size0 = fmax(hw, hh);
+    }
+    __syncthreads();
+    return out;
+}
+
+// One node of the pre-order tree, read with one vector load per step of the
+// walk.  cl: point count (low kCountBits bits) and quadtree level (high
+// bits) of an internal node, 1 for a leaf; link: skip index (end of the
+// subtree) of an internal node, kLeafBit | point index for a leaf.  An
+// internal node's cell half-width is size0 * 2^-level.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr int kCountBits = 26;
+constexpr int kMaxTreePoints = (1 << kCountBits) - 1;
+
+template <class T>
+struct alignas(4 * sizeof(T)) BhNode {
+    T x, y;  // centre of mass (leaf: the point)
+    uint32_t cl;
+    uint32_t link;
+};
+
 // Device storage of one tree build; capacity for N points.
 template <class T>
 struct BhTree {
@@ -55,8 +125,7 @@ struct BhTree {
     DevBuf<uint32_t> pre;        // N+1 exclusive prefix of (popcount(mask)+1)
     DevBuf<double> sx, sy;       // N+1 exclusive prefix sums of sorted coordinates
     DevBuf<double> part;         // scan partials
-    DevBuf<typename Vec4<T>::type> nodes;  // pre-order: com x, com y, count, size^2 (<0: leaf)
-    DevBuf<int> aux;             // internal: skip index; leaf: point index
+    DevBuf<BhNode<T>> nodes;     // pre-order, 2N - 1 at most
     DevBuf<RootBox> root;
     const uint32_t* sorted_perm = nullptr;  // valid after build
     void alloc(int n, MemBudget* mb);
@@ -68,9 +137,17 @@ void bbox_partials_device(const T* Y, int N, double* part, hipStream_t s);
 inline int bbox_blocks(int N) { return (int)ceil_div(N, kBoxBlock); }
 
 // Builds the tree for Y (N x 2 row-major, T).  Y is centred in place using the
-// mean from `box_part` (nbox blocks of partials of this Y).
+// mean from `box_part` (nbox blocks of partials of this Y); every key
+// workgroup reduces the partials itself, so keep nbox small (hundreds).
 template <class T>
 void bh_build(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s);
+
+// bh_build in two halves: root cell, centring of Y and keys (after which Y
+// is final for this iteration); then sort and nodes.
+template <class T>
+void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s);
+template <class T>
+void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s);
 
 // Repulsive forces: neg (N x 2) = sum over accepted cells of
 // count*q^2*(y - com) and zpart[block] = partial sums of count*q

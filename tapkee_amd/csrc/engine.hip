@@ -162,6 +162,8 @@ void validate(int N, int D, int no_dims, double perplexity, double theta)
     // exact path supports 1..3 dims, and K = (int)(3*perplexity) must be >= 1.
     if (theta > 0.0 && no_dims != 2)
         fail(TKA_E_WRONG_PARAMETER, "Barnes-Hut t-SNE (theta > 0) needs target dimension 2");
+    if (theta > 0.0 && N > kMaxTreePoints)
+        fail(TKA_E_UNSUPPORTED, "Barnes-Hut t-SNE: at most 2^26 - 1 points (tree node counts are 26-bit)");
     if (no_dims > 3) fail(TKA_E_WRONG_PARAMETER, "target dimension above 3 is not supported");
 }
 
@@ -214,7 +216,7 @@ struct EventTimer {
     }
 };
 
-enum { kTimBuild = 0, kTimRepulsion = 1, kTimStep = 2, kTimExact = 3, kTimCount = 4 };
+enum { kTimBuild = 0, kTimRepulsion = 1, kTimStep = 2, kTimExact = 3, kTimAttract = 4, kTimCount = 5 };
 
 }  // namespace
 }  // namespace tka
@@ -274,6 +276,32 @@ struct Session : tka_session {
     DevBuf<T> val2, state_tmp;
     DevBuf<uint32_t> rn_scratch;
 
+    // Barnes-Hut loop: the attraction pass runs on a second stream beside
+    // the tree build and traversal.  ev_y marks "Y centred and final for this
+    // iteration"; the update waits for ev_attr.
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_y = nullptr, ev_attr = nullptr;
+
+    ~Session() override
+    {
+        if (s2) {
+            (void)hipStreamSynchronize(s2);
+            (void)hipStreamDestroy(s2);
+        }
+        if (ev_y) (void)hipEventDestroy(ev_y);
+        if (ev_attr) (void)hipEventDestroy(ev_attr);
+    }
+
+    void launch_attraction()
+    {
+        TKA_HIP(hipEventRecord(ev_y, s()));
+        TKA_HIP(hipStreamWaitEvent(s2, ev_y, 0));
+        if (profile) timers[kTimAttract].begin(s2);
+        attraction_device<T>(dims, Ycur, L.row.get(), L.col.get(), L.val.get(), exag, N, L.pos.get(), s2);
+        if (profile) timers[kTimAttract].end(s2);
+        TKA_HIP(hipEventRecord(ev_attr, s2));
+    }
+
     void renumber(const uint32_t* perm)
     {
         hipStream_t s = this->s();
@@ -318,10 +346,15 @@ struct Session : tka_session {
         L.box.alloc((size_t)kBoxStride * std::max(nstep, nbox0), m);
         nz = exact ? exact_zblocks(N) : bh_zblocks(N);
         L.zpart.alloc(nz, m);
-        if (exact && L.dense) L.pos.alloc(nd, m);
+        if (L.dense || !exact) L.pos.alloc(nd, m);
         // tsne.hpp:75-80: uY = 0, gains = 1; Y0 is not centred before iteration 0
         bbox_partials_device<T>(L.Ya.get(), N, L.box.get(), s);
-        if (!exact) tree.alloc(N, m);
+        if (!exact) {
+            tree.alloc(N, m);
+            TKA_HIP(hipStreamCreate(&s2));
+            TKA_HIP(hipEventCreateWithFlags(&ev_y, hipEventDisableTiming));
+            TKA_HIP(hipEventCreateWithFlags(&ev_attr, hipEventDisableTiming));
+        }
         nbox = nbox0;
         Ycur = L.Ya.get();
         Ynext = L.Yb.get();
@@ -349,23 +382,39 @@ struct Session : tka_session {
                 if (iter > 0 && iter % kRenumberEvery == 0 && tree.sorted_perm != nullptr)
                     renumber(tree.sorted_perm);  // order of the previous iteration's tree
                 if (profile) timers[kTimBuild].begin(s);
-                bh_build<T>(Ycur, N, L.box.get(), nbox, tree, s);
+                bh_build_keys<T>(Ycur, N, L.box.get(), nbox, tree, s);
+                bh_build_tree<T>(Ycur, N, tree, s);
                 if (profile) timers[kTimBuild].end(s);
+                // Y is final after the key kernel; starting the attraction
+                // here overlaps it with the (latency-bound) traversal rather
+                // than with the sort
+                launch_attraction();
                 if (profile) timers[kTimRepulsion].begin(s);
                 bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(),
                                 profile ? interactions.get() : nullptr, s);
                 if (profile) timers[kTimRepulsion].end(s);
+            } else {
+                launch_attraction();  // tree and repulsion came with the last KL evaluation
             }
             tree_ready = false;
             // after renumbering: the CSR arrays were swapped for their renumbered copies
             const int* row = L.dense ? nullptr : L.row.get();
             const int* col = L.dense ? nullptr : L.col.get();
             StepParams sp{momentum, o.eta, exag};
-            if (profile) timers[kTimStep].begin(s);
-            nbox = tsne_step_device<T>(dims, Ycur, Ynext, L.uY.get(), L.gains.get(), L.neg.get(),
-                                       L.dense ? L.pos.get() : nullptr, row, col, L.dense ? nullptr : L.val.get(),
-                                       L.zpart.get(), nz, N, sp, L.box.get(), nullptr, s);
-            if (profile) timers[kTimStep].end(s);
+            if (exact) {
+                if (profile) timers[kTimStep].begin(s);
+                nbox = tsne_step_device<T>(dims, Ycur, Ynext, L.uY.get(), L.gains.get(), L.neg.get(),
+                                           L.dense ? L.pos.get() : nullptr, row, col,
+                                           L.dense ? nullptr : L.val.get(), L.zpart.get(), nz, N, sp, L.box.get(),
+                                           nullptr, s);
+                if (profile) timers[kTimStep].end(s);
+            } else {
+                TKA_HIP(hipStreamWaitEvent(s, ev_attr, 0));
+                if (profile) timers[kTimStep].begin(s);
+                nbox = tsne_update_device<T>(dims, Ycur, Ynext, L.uY.get(), L.gains.get(), L.neg.get(), L.pos.get(),
+                                             L.zpart.get(), nz, N, sp, L.box.get(), nullptr, s);
+                if (profile) timers[kTimStep].end(s);
+            }
             std::swap(Ycur, Ynext);
             centred = false;
             if (exact) {
@@ -808,6 +857,8 @@ TKA_API tka_status tka_session_kernel_timing(tka_session* s, tka_kernel_timing* 
         t->ms_repulsion = s->timers[kTimRepulsion].ms;
         t->ms_step = s->timers[kTimStep].ms;
         t->ms_exact = s->timers[kTimExact].ms;
+        t->launches_attraction = s->timers[kTimAttract].launches;
+        t->ms_attraction = s->timers[kTimAttract].ms;
         unsigned long long inter = 0;
         if (s->interactions.get()) {
             TKA_HIP(hipMemcpy(&inter, s->interactions.get(), sizeof(inter), hipMemcpyDeviceToHost));
@@ -1085,7 +1136,6 @@ TKA_API tka_status tka_bh_tree(const double* Y, int32_t N, int32_t precision, in
         check_finite(Y, (size_t)N * 2, "Y");
         auto run = [&](auto tag) {
             using T = decltype(tag);
-            using T4 = typename Vec4<T>::type;
             MemBudget mb;
             Stream st;
             hipStream_t s = st.s;
@@ -1103,18 +1153,16 @@ TKA_API tka_status tka_bh_tree(const double* Y, int32_t N, int32_t precision, in
             download(&total, tree.pre.get() + N, 1, s);
             download(&rb, tree.root.get(), 1, s);
             st.sync();
-            std::vector<T4> nodes(total);
-            std::vector<int> aux(total);
+            std::vector<BhNode<T>> nodes(total);
             download(nodes.data(), tree.nodes.get(), total, s);
-            download(aux.data(), tree.aux.get(), total, s);
             st.sync();
             *n_nodes = (int32_t)total;
             for (uint32_t t = 0; t < total; t++) {
-                const bool leaf = nodes[t].w < (T)0;
-                if (level) level[t] = leaf ? -1 : (int)std::lround(std::log2(rb.size0 / std::sqrt((double)nodes[t].w)));
-                if (count) count[t] = (int)std::lround((double)nodes[t].z);
-                if (leaf_point) leaf_point[t] = leaf ? aux[t] : -1;
-                if (skip) skip[t] = leaf ? (int)t + 1 : aux[t];
+                const bool leaf = (nodes[t].link & kLeafBit) != 0;
+                if (level) level[t] = leaf ? -1 : (int)(nodes[t].cl >> kCountBits);
+                if (count) count[t] = leaf ? 1 : (int)(nodes[t].cl & ((1u << kCountBits) - 1u));
+                if (leaf_point) leaf_point[t] = leaf ? (int)(nodes[t].link & ~kLeafBit) : -1;
+                if (skip) skip[t] = leaf ? (int)t + 1 : (int)nodes[t].link;
                 if (com_x) com_x[t] = (double)nodes[t].x;
                 if (com_y) com_y[t] = (double)nodes[t].y;
             }

@@ -130,6 +130,128 @@ __global__ __launch_bounds__(256) void tsne_step(const T* __restrict__ Y, T* __r
     }
 }
 
+// Attraction alone (quadtree.hpp:357-384), for the Barnes-Hut loop where it
+// runs beside the tree build: pos_i = sum_e exag*P_e (y_i - y_col)/(1 + d^2).
+// Each lane takes its row's edges e, e+kG, e+2kG, ... as before, loading
+// kBatch column indices, then kBatch rows of Y, before the arithmetic, so a
+// lane waits on two memory latencies per kBatch edges rather than per edge.
+// Padding entries use the row itself with P = 0 and add exactly zero.
+constexpr int kBatch = 4;
+
+template <class T, int DIMS>
+__global__ __launch_bounds__(256) void attract(const T* __restrict__ Y, const int* __restrict__ row_ptr,
+                                               const int* __restrict__ col, const T* __restrict__ val, T exag,
+                                               int N, T* __restrict__ pos)
+{
+    const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
+    const int i = blockIdx.x * kRowsPerBlock + g;
+    if (i >= N) return;  // whole row groups leave together; group sums stay inside a group
+    T yi[DIMS], acc[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) {
+        yi[d] = Y[(size_t)i * DIMS + d];
+        acc[d] = (T)0;
+    }
+    const int e1 = row_ptr[i + 1];
+    for (int base = row_ptr[i] + gl; base < e1; base += kBatch * kG) {
+        int j[kBatch];
+        T v[kBatch];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            const int e = base + u * kG;
+            const bool ok = e < e1;
+            j[u] = ok ? col[e] : i;
+            v[u] = ok ? val[e] : (T)0;
+        }
+        T yj[kBatch][DIMS];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++)
+#pragma unroll
+            for (int d = 0; d < DIMS; d++) yj[u][d] = Y[(size_t)j[u] * DIMS + d];
+#pragma unroll
+        for (int u = 0; u < kBatch; u++) {
+            T diff[DIMS];
+            T D = (T)0;
+#pragma unroll
+            for (int d = 0; d < DIMS; d++) {
+                diff[d] = yi[d] - yj[u][d];
+                D += diff[d] * diff[d];
+            }
+            const T m = (exag * v[u]) / ((T)1 + D);
+#pragma unroll
+            for (int d = 0; d < DIMS; d++) acc[d] += m * diff[d];
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) acc[d] = group_sum(acc[d]);
+    if (gl == 0)
+#pragma unroll
+        for (int d = 0; d < DIMS; d++) pos[(size_t)i * DIMS + d] = acc[d];
+}
+
+// The optimiser step from precomputed attraction and repulsion, one thread
+// per row (tsne.hpp:152-167 and :344-347); also the box partials of Ynext.
+constexpr int kUpdBlock = 1024;
+
+template <class T, int DIMS>
+__global__ __launch_bounds__(kUpdBlock) void tsne_update(const T* __restrict__ Y, T* __restrict__ Ynext,
+                                                         T* __restrict__ uY, T* __restrict__ gains,
+                                                         const T* __restrict__ neg, const T* __restrict__ pos,
+                                                         const double* __restrict__ zpart, int nz, int N, T momentum,
+                                                         T eta, double* __restrict__ box_part, T* __restrict__ dC_out)
+{
+    constexpr int kWaves = kUpdBlock / 64;
+    __shared__ double sh[kWaves];
+    __shared__ double sbox[7][kWaves];
+    const double Z = z_total(zpart, nz, sh);
+    const int i = blockIdx.x * kUpdBlock + threadIdx.x;
+    double bx[7] = {0.0, 0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+    if (i < N) {
+        T yn[DIMS];
+#pragma unroll
+        for (int d = 0; d < DIMS; d++) {
+            const size_t o = (size_t)i * DIMS + d;
+            const T dC = pos[o] - (T)((double)neg[o] / Z);
+            if (dC_out) dC_out[o] = dC;
+            T gn = gains[o];
+            T u = uY[o];
+            gn = (sgn(dC) != sgn(u)) ? (gn + (T)0.2) : (gn * (T)0.8);
+            if (gn < (T)0.01) gn = (T)0.01;
+            u = momentum * u - eta * gn * dC;
+            gains[o] = gn;
+            uY[o] = u;
+            yn[d] = Y[o] + u;
+            Ynext[o] = yn[d];
+            if (d < 3) bx[d] = (double)yn[d];
+        }
+        bx[3] = bx[5] = (double)yn[0];
+        bx[4] = bx[6] = (double)yn[DIMS > 1 ? 1 : 0];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) bx[k] = wave_sum(bx[k]);
+    bx[3] = wave_min(bx[3]);
+    bx[4] = wave_min(bx[4]);
+    bx[5] = wave_max(bx[5]);
+    bx[6] = wave_max(bx[6]);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 7; k++) sbox[k][w] = bx[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r[7] = {0.0, 0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
+        for (int ww = 0; ww < kWaves; ww++) {
+            for (int k = 0; k < 3; k++) r[k] += sbox[k][ww];
+            r[3] = fmin(r[3], sbox[3][ww]);
+            r[4] = fmin(r[4], sbox[4][ww]);
+            r[5] = fmax(r[5], sbox[5][ww]);
+            r[6] = fmax(r[6], sbox[6][ww]);
+        }
+        double* o = box_part + (size_t)kBoxStride * blockIdx.x;
+        for (int k = 0; k < 7; k++) o[k] = r[k];
+        o[7] = 0.0;
+    }
+}
+
 // zeroMean (tsne.hpp:488-518) from box partials: Y -= sum/N per dimension.
 template <class T, int DIMS>
 __global__ __launch_bounds__(256) void center_rows(T* __restrict__ Y, int N, const double* __restrict__ part, int nb)
@@ -550,6 +672,44 @@ int tsne_step_device(int dims, const T* Y, T* Ynext, T* uY, T* gains, const T* n
     return nb;
 }
 
+int tsne_update_blocks(int N) { return (int)ceil_div(N, kUpdBlock); }
+
+template <class T>
+void attraction_device(int dims, const T* Y, const int* row_ptr, const int* col, const T* val, double exag, int N,
+                       T* pos, hipStream_t s)
+{
+    const int nb = tsne_step_blocks(N);
+#define TKA_ATT(DM) \
+    hipLaunchKernelGGL((attract<T, DM>), dim3(nb), dim3(256), 0, s, Y, row_ptr, col, val, (T)exag, N, pos)
+    switch (dims) {
+        case 1: TKA_ATT(1); break;
+        case 2: TKA_ATT(2); break;
+        case 3: TKA_ATT(3); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+#undef TKA_ATT
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T>
+int tsne_update_device(int dims, const T* Y, T* Ynext, T* uY, T* gains, const T* neg, const T* pos,
+                       const double* zpart, int nz, int N, StepParams sp, double* box_part, T* dC_out, hipStream_t s)
+{
+    const int nb = tsne_update_blocks(N);
+#define TKA_UPD(DM)                                                                                                  \
+    hipLaunchKernelGGL((tsne_update<T, DM>), dim3(nb), dim3(kUpdBlock), 0, s, Y, Ynext, uY, gains, neg, pos, zpart, \
+                       nz, N, (T)sp.momentum, (T)sp.eta, box_part, dC_out)
+    switch (dims) {
+        case 1: TKA_UPD(1); break;
+        case 2: TKA_UPD(2); break;
+        case 3: TKA_UPD(3); break;
+        default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
+    }
+#undef TKA_UPD
+    TKA_LAUNCH_CHECK();
+    return nb;
+}
+
 template <class T>
 void gradient_combine_device(int dims, const T* Y, const T* neg, const T* pos_pre, const int* row_ptr,
                              const int* col, const T* val, double exag, const double* zpart, int nz, int N, T* dC,
@@ -693,6 +853,10 @@ double kl_dense_device(int dims, const T* Y, const T* P, double exag, int N, int
     template void unpermute_rows_device<T>(int, int, const T*, const int*, T*, hipStream_t);                        \
     template int tsne_step_device<T>(int, const T*, T*, T*, T*, const T*, const T*, const int*, const int*, const T*, \
                                      const double*, int, int, StepParams, double*, T*, hipStream_t);                 \
+    template void attraction_device<T>(int, const T*, const int*, const int*, const T*, double, int, T*,           \
+                                       hipStream_t);                                                                 \
+    template int tsne_update_device<T>(int, const T*, T*, T*, T*, const T*, const T*, const double*, int, int,       \
+                                       StepParams, double*, T*, hipStream_t);                                        \
     template void gradient_combine_device<T>(int, const T*, const T*, const T*, const int*, const int*, const T*,    \
                                              double, const double*, int, int, T*, hipStream_t);                      \
     template int exact_forces_device<T>(int, const T*, int, int, const T*, double, T*, T*, double*, hipStream_t);    \

@@ -27,6 +27,19 @@ int tsne_step_device(int dims, const T* Y, T* Ynext, T* uY, T* gains, const T* n
                      StepParams sp, double* box_part, T* dC_out, hipStream_t s);
 int tsne_step_blocks(int N);
 
+// The same step in two launches, so that the attraction can run on a second
+// stream beside the tree build: pos (N x dims) from the CSR, then the update
+// from pos, neg and the zpart partials of Z.  The update returns the number of
+// box partials it wrote (tsne_update_blocks(N), a few per thousand points, so
+// that the next tree build can reduce them cheaply).
+template <class T>
+void attraction_device(int dims, const T* Y, const int* row_ptr, const int* col, const T* val, double exag, int N,
+                       T* pos, hipStream_t s);
+template <class T>
+int tsne_update_device(int dims, const T* Y, T* Ynext, T* uY, T* gains, const T* neg, const T* pos,
+                       const double* zpart, int nz, int N, StepParams sp, double* box_part, T* dC_out, hipStream_t s);
+int tsne_update_blocks(int N);
+
 // zeroMean of Y (N x dims) from the box partials of the step that produced it.
 template <class T>
 void center_device(int dims, T* Y, int N, const double* box_part, int nb, hipStream_t s);

@@ -4,7 +4,8 @@
 //   rs_count     every tile histograms all digits of its keys at once and adds
 //                them to global per-pass counts; the last tile to arrive turns
 //                the counts into per-pass digit bases (and clears them for the
-//                next sort).  It also clears the tile status words.
+//                next sort).  It also clears the tile status words.  A kernel
+//                that produces the keys can do this itself (rs_count_tile).
 //   rs_onesweep  takes a tile number in arrival order, ranks its keys by the
 //                pass's digit (wave ballots + per-wave LDS counters), publishes
 //                its per-digit counts, sums its predecessors' counts by
@@ -23,98 +24,21 @@ constexpr uint32_t kFlagInc = 2u << 30;   // inclusive prefix through this tile 
 constexpr uint32_t kValMask = (1u << 30) - 1;
 constexpr int kSpinLimit = 1 << 22;
 constexpr int kLookback = 16;             // predecessor tiles read per look-back round
-constexpr size_t kRsHeader = 16 + 2 * (size_t)kRsMaxPasses * 256;
 
-// Scratch layout (words): [0] arrival ticket, [1] fault, [2, 2+8) tile counters
-// per pass, then global counts and bases (8 x 256 each), then tile status
-// words (passes x tiles x 256).
-struct RsLayout {
-    uint32_t* ticket;
-    uint32_t* fault;
-    uint32_t* tile_ctr;
-    uint32_t* gcnt;
-    uint32_t* gbase;
-    uint32_t* status;
-};
-
-RsLayout rs_layout(uint32_t* s)
-{
-    return {s, s + 1, s + 2, s + 16, s + 16 + kRsMaxPasses * 256, s + kRsHeader};
-}
-
-TKA_DEV uint32_t ld_acq(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-TKA_DEV void st_rel(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
-
-TKA_DEV uint32_t wave_inclusive_scan(uint32_t v)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
-}
-
-// Exclusive scan over the threads of a block (blockDim.x multiple of 64,
-// <= 1024).  `sh` needs blockDim.x/64 words.  *total receives the block sum.
-TKA_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total)
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    uint32_t inc = wave_inclusive_scan(v);
-    __syncthreads();
-    if (lane == 63) sh[w] = inc;
-    __syncthreads();
-    uint32_t before = 0, all = 0;
-    for (int i = 0; i < nw; i++) {
-        const uint32_t s = sh[i];
-        before += (i < w) ? s : 0u;
-        all += s;
-    }
-    if (total) *total = all;
-    return before + inc - v;
-}
-
-template <class K, int ITEMS>
+template <class K>
 __global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys, size_t n, int passes, RsLayout L,
                                                      int tiles)
 {
-    __shared__ uint32_t h[kRsMaxPasses][256];
-    __shared__ uint32_t sh[kRsBlock / 64];
-    __shared__ int is_last;
-    const int t = threadIdx.x;
-    for (int p = 0; p < passes; p++) {
-        h[p][t] = 0;
-        L.status[((size_t)p * tiles + blockIdx.x) * 256 + t] = 0;
-    }
-    if (blockIdx.x == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
-    __syncthreads();
-    const size_t base = (size_t)blockIdx.x * kRsBlock * ITEMS;
+    K key[kRsItems];
+    bool ok[kRsItems];
+    const size_t base = (size_t)blockIdx.x * kRsBlock * kRsItems;
 #pragma unroll
-    for (int k = 0; k < ITEMS; k++) {
-        const size_t i = base + (size_t)k * kRsBlock + t;
-        if (i < n) {
-            const K key = keys[i];
-            for (int p = 0; p < passes; p++) atomicAdd(&h[p][(uint32_t)(key >> (8 * p)) & 255u], 1u);
-        }
+    for (int k = 0; k < kRsItems; k++) {
+        const size_t i = base + (size_t)k * kRsBlock + threadIdx.x;
+        ok[k] = i < n;
+        key[k] = ok[k] ? keys[i] : K(0);
     }
-    __syncthreads();
-    for (int p = 0; p < passes; p++) {
-        const uint32_t c = h[p][t];
-        if (c) atomicAdd(&L.gcnt[p * 256 + t], c);
-    }
-    __threadfence();
-    __syncthreads();
-    if (t == 0) is_last = (atomicAdd(L.ticket, 1u) == (uint32_t)(tiles - 1));
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();  // acquire: every tile's counts are in
-    for (int p = 0; p < passes; p++) {
-        const uint32_t c = ld_acq(&L.gcnt[p * 256 + t]);
-        L.gbase[p * 256 + t] = block_exclusive_scan(c, sh, nullptr);
-        L.gcnt[p * 256 + t] = 0;
-    }
-    if (t == 0) atomicExch(L.ticket, 0u);
+    rs_count_tile<K>(key, ok, passes, L, (int)blockIdx.x, tiles);
 }
 
 // Stable scatter of one tile.  Item k of lane l in wave w sits at tile
@@ -181,9 +105,9 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
         uint32_t* st = L.status + (size_t)pass * tiles * 256 + d;
         uint32_t excl = 0;
         if (tile == 0) {
-            st_rel(st, kFlagInc | cnt);
+            rs_st(st, kFlagInc | cnt);
         } else {
-            st_rel(st + (size_t)tile * 256, kFlagAgg | cnt);
+            rs_st(st + (size_t)tile * 256, kFlagAgg | cnt);
             // Look back over a window of kLookback predecessors per round
             // (independent loads in flight together), nearest first: add
             // published counts until an inclusive prefix; stop the round at
@@ -193,17 +117,17 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
             while (!done) {
                 uint32_t v[kLookback];
 #pragma unroll
-                for (int w = 0; w < kLookback; w++) v[w] = (j - w >= 0) ? ld_acq(st + (size_t)(j - w) * 256) : 0u;
+                for (int q = 0; q < kLookback; q++) v[q] = (j - q >= 0) ? rs_ld(st + (size_t)(j - q) * 256) : 0u;
                 // branch-free over the window (keeps v[] in registers); tile 0
                 // is always inclusive, so entries past it are never taken
                 int used = 0;
                 bool stop = false;
 #pragma unroll
-                for (int w = 0; w < kLookback; w++) {
-                    const uint32_t f = v[w] & ~kValMask;
+                for (int q = 0; q < kLookback; q++) {
+                    const uint32_t f = v[q] & ~kValMask;
                     const bool take = !stop && f != 0;
-                    excl += take ? (v[w] & kValMask) : 0u;
-                    used = take ? w + 1 : used;
+                    excl += take ? (v[q] & kValMask) : 0u;
+                    used = take ? q + 1 : used;
                     done = done || (take && f == kFlagInc);
                     stop = stop || f == 0 || f == kFlagInc;
                 }
@@ -217,7 +141,7 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
-            st_rel(st + (size_t)tile * 256, kFlagInc | (excl + cnt));
+            rs_st(st + (size_t)tile * 256, kFlagInc | (excl + cnt));
         }
         base[d] = L.gbase[pass * 256 + d] + excl;
     }
@@ -303,13 +227,16 @@ void scan_impl(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch, b
 }
 
 template <class K, class V, int ITEMS>
-bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch, hipStream_t s)
+bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch, bool counted,
+               hipStream_t s)
 {
     const int tiles = (int)ceil_div(n, (size_t)kRsBlock * ITEMS);
     const int passes = (end_bit + 7) / 8;
     const RsLayout L = rs_layout(scratch);
-    hipLaunchKernelGGL((rs_count<K, ITEMS>), dim3(tiles), dim3(kRsBlock), 0, s, keys, n, passes, L, tiles);
-    TKA_LAUNCH_CHECK();
+    if (!counted) {
+        hipLaunchKernelGGL((rs_count<K>), dim3(tiles), dim3(kRsBlock), 0, s, keys, n, passes, L, tiles);
+        TKA_LAUNCH_CHECK();
+    }
     K* ka = keys;
     V* va = vals;
     K* kb = keys_alt;
@@ -335,8 +262,21 @@ bool radix_sort_pairs(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int 
     if (n <= 1 || end_bit <= 0) return false;
     require(n <= kRsMaxItems, TKA_E_UNSUPPORTED, "radix sort: more than 2^30 - 1 items");
     require(end_bit <= 8 * (int)sizeof(K), TKA_E_INTERNAL, "radix sort: end_bit beyond the key width");
-    return sort_impl<K, V, kRsItems>(keys, vals, keys_alt, vals_alt, n, end_bit, scratch, stream);
+    return sort_impl<K, V, kRsItems>(keys, vals, keys_alt, vals_alt, n, end_bit, scratch, false, stream);
 }
+
+template <class K, class V>
+bool radix_sort_pairs_counted(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch,
+                              hipStream_t stream)
+{
+    if (n <= 1 || end_bit <= 0) return false;
+    require(n <= kRsMaxItems, TKA_E_UNSUPPORTED, "radix sort: more than 2^30 - 1 items");
+    require(end_bit <= 8 * (int)sizeof(K), TKA_E_INTERNAL, "radix sort: end_bit beyond the key width");
+    return sort_impl<K, V, kRsItems>(keys, vals, keys_alt, vals_alt, n, end_bit, scratch, true, stream);
+}
+
+template bool radix_sort_pairs_counted<uint32_t, uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, size_t, int,
+                                                           uint32_t*, hipStream_t);
 
 template bool radix_sort_pairs<uint32_t, uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, size_t, int, uint32_t*,
                                                    hipStream_t);

@@ -26,6 +26,107 @@ template <class K, class V>
 bool radix_sort_pairs(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch,
                       hipStream_t stream);
 
+// The same sort without its counting launch, for a caller whose own kernel
+// wrote the keys and ran rs_count_tile for every tile (see below).
+template <class K, class V>
+bool radix_sort_pairs_counted(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch,
+                              hipStream_t stream);
+
+// ---------------------------------------------------------------- device side
+// Scratch layout (words): [0] arrival ticket, [1] fault, [2, 2+8) tile counters
+// per pass, then global digit counts and bases (8 x 256 each), then tile
+// status words (passes x tiles x 256).
+constexpr size_t kRsHeader = 16 + 2 * (size_t)kRsMaxPasses * 256;
+
+struct RsLayout {
+    uint32_t* ticket;
+    uint32_t* fault;
+    uint32_t* tile_ctr;
+    uint32_t* gcnt;
+    uint32_t* gbase;
+    uint32_t* status;
+};
+
+inline RsLayout rs_layout(uint32_t* s)
+{
+    return {s, s + 1, s + 2, s + 16, s + 16 + kRsMaxPasses * 256, s + kRsHeader};
+}
+
+TKA_DEV uint32_t rs_ld(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+TKA_DEV void rs_st(uint32_t* p, uint32_t v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+TKA_DEV uint32_t wave_inclusive_scan(uint32_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Exclusive scan over the threads of a block (blockDim.x multiple of 64,
+// <= 1024).  `sh` needs blockDim.x/64 words.  *total receives the block sum.
+TKA_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t inc = wave_inclusive_scan(v);
+    __syncthreads();
+    if (lane == 63) sh[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int i = 0; i < nw; i++) {
+        const uint32_t s = sh[i];
+        before += (i < w) ? s : 0u;
+        all += s;
+    }
+    if (total) *total = all;
+    return before + inc - v;
+}
+
+// Counting step of the sort for one tile, called by all kRsBlock threads of
+// the workgroup that holds tile `tile` (keys tile*kRsBlock*kRsItems + k*kRsBlock
+// + threadIdx.x, k < kRsItems; ok[k] false past n).  Adds the tile's digit
+// histograms of all passes to the global counts, clears the tile's status
+// words and, in the last tile to arrive, turns the counts into digit bases.
+template <class K>
+TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems], int passes, RsLayout L, int tile,
+                           int tiles)
+{
+    __shared__ uint32_t h[kRsMaxPasses][256];
+    __shared__ uint32_t sh[kRsBlock / 64];
+    __shared__ int is_last;
+    const int t = threadIdx.x;
+    for (int p = 0; p < passes; p++) {
+        h[p][t] = 0;
+        L.status[((size_t)p * tiles + tile) * 256 + t] = 0;
+    }
+    if (tile == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRsItems; k++)
+        if (ok[k])
+            for (int p = 0; p < passes; p++) atomicAdd(&h[p][(uint32_t)(key[k] >> (8 * p)) & 255u], 1u);
+    __syncthreads();
+    for (int p = 0; p < passes; p++) {
+        const uint32_t c = h[p][t];
+        if (c) atomicAdd(&L.gcnt[p * 256 + t], c);
+    }
+    __threadfence();
+    __syncthreads();
+    if (t == 0) is_last = (atomicAdd(L.ticket, 1u) == (uint32_t)(tiles - 1));
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();  // acquire: every tile's counts are in
+    for (int p = 0; p < passes; p++) {
+        const uint32_t c = rs_ld(&L.gcnt[p * 256 + t]);
+        L.gbase[p * 256 + t] = block_exclusive_scan(c, sh, nullptr);
+        L.gcnt[p * 256 + t] = 0;
+    }
+    if (t == 0) atomicExch(L.ticket, 0u);
+}
+
 // Nonzero if a sort using this scratch gave up waiting on a predecessor tile
 // (its output is then wrong); reads one word, synchronising the stream.
 uint32_t radix_sort_fault(const uint32_t* scratch, hipStream_t stream);
