@@ -19,6 +19,7 @@
 //              no dropped candidate can belong to the top K.
 //   4. rows failing the bound are re-run with FP32 tiles and k' = 384.
 #include "knn.hpp"
+#include "radix_sort.hpp"
 
 #include <vector>
 
@@ -48,9 +49,14 @@ TKA_DEV uint64_t shfl_xor_u64(uint64_t v, int m)
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Ascending bitonic sort of 64*R keys held as v[r] at position r*64 + lane.
+// Ascending wave bitonic sort of 64*R (distance bits, index) pairs held as
+// d[r], ix[r] at position r*64 + lane (the first R of 8 registers; the pair
+// order is lexicographic).  Two plain register arrays rather than an array
+// of structs: the compiler keeps these in registers.
+TKA_DEV bool pair_less(uint64_t ad, uint32_t ai, uint64_t bd, uint32_t bi) { return ad < bd || (ad == bd && ai < bi); }
+
 template <int R>
-TKA_DEV void wave_bitonic_sort(uint64_t (&v)[R])
+TKA_DEV void wave_sort_pairs(uint64_t (&d)[8], uint32_t (&ix)[8])
 {
     const int lane = threadIdx.x & 63;
 #pragma unroll
@@ -62,11 +68,15 @@ TKA_DEV void wave_bitonic_sort(uint64_t (&v)[R])
 #pragma unroll
                 for (int r = 0; r < R; r++) {
                     if ((r & jr) == 0) {
+                        const int r2 = r | jr;
                         const bool up = (((r * 64 + lane) & k) == 0);
-                        const uint64_t a = v[r], b = v[r | jr];
-                        const bool sw = up ? (a > b) : (a < b);
-                        v[r] = sw ? b : a;
-                        v[r | jr] = sw ? a : b;
+                        const bool sw = up ? pair_less(d[r2], ix[r2], d[r], ix[r]) : pair_less(d[r], ix[r], d[r2], ix[r2]);
+                        const uint64_t d0 = d[r], d1 = d[r2];
+                        const uint32_t i0 = ix[r], i1 = ix[r2];
+                        d[r] = sw ? d1 : d0;
+                        d[r2] = sw ? d0 : d1;
+                        ix[r] = sw ? i1 : i0;
+                        ix[r2] = sw ? i0 : i1;
                     }
                 }
             } else {
@@ -75,56 +85,12 @@ TKA_DEV void wave_bitonic_sort(uint64_t (&v)[R])
                     const int p = r * 64 + lane;
                     const bool up = (p & k) == 0;
                     const bool lower = (p & j) == 0;
-                    const uint64_t o = shfl_xor_u64(v[r], j);
-                    const uint64_t mn = o < v[r] ? o : v[r];
-                    const uint64_t mx = o < v[r] ? v[r] : o;
-                    v[r] = (lower == up) ? mn : mx;
-                }
-            }
-        }
-    }
-}
-
-// Lexicographic (distance bits, index) keys for the FP64 re-rank.
-struct DKey {
-    uint64_t d;
-    uint32_t i;
-};
-TKA_DEV bool dkey_less(const DKey& a, const DKey& b) { return a.d < b.d || (a.d == b.d && a.i < b.i); }
-
-template <int R>
-TKA_DEV void wave_bitonic_sort(DKey (&v)[R])
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int k = 2; k <= 64 * R; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 64) {
-                const int jr = j >> 6;
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    if ((r & jr) == 0) {
-                        const bool up = (((r * 64 + lane) & k) == 0);
-                        const DKey a = v[r], b = v[r | jr];
-                        const bool sw = up ? dkey_less(b, a) : dkey_less(a, b);
-                        v[r] = sw ? b : a;
-                        v[r | jr] = sw ? a : b;
-                    }
-                }
-            } else {
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int p = r * 64 + lane;
-                    const bool up = (p & k) == 0;
-                    const bool lower = (p & j) == 0;
-                    DKey o;
-                    o.d = shfl_xor_u64(v[r].d, j);
-                    o.i = __shfl_xor(v[r].i, j, 64);
-                    const bool oless = dkey_less(o, v[r]);
-                    const DKey mn = oless ? o : v[r];
-                    const DKey mx = oless ? v[r] : o;
-                    v[r] = (lower == up) ? mn : mx;
+                    const uint64_t od = shfl_xor_u64(d[r], j);
+                    const uint32_t oi = __shfl_xor(ix[r], j, 64);
+                    const bool oless = pair_less(od, oi, d[r], ix[r]);
+                    const bool take = (lower == up) ? oless : !oless;  // keep the min in the lower slot when ascending
+                    d[r] = take ? od : d[r];
+                    ix[r] = take ? oi : ix[r];
                 }
             }
         }
@@ -291,6 +257,7 @@ __global__ __launch_bounds__(256) void prep_bf16x3(const double* __restrict__ X,
 struct TileArgs {
     int N;
     int kprime;
+    int rotate;         // 1: block b streams candidate chunks b, b+1, ... (mod), starting with its own rows
     const int* qlist;   // optional list of query rows (nq entries)
     int nq;
     uint64_t* buf;      // row-slot x kCap
@@ -301,41 +268,151 @@ struct TileArgs {
 // Row-slot of the wave's query: blocks cover query slots [0, nq).
 TKA_DEV int query_row(const TileArgs& a, int slot) { return slot < a.nq ? (a.qlist ? a.qlist[slot] : slot) : -1; }
 
-// Appends the 16 values a lane holds for its query (tile t of the chunk).
-// Lane l holds C[m][n] with n = l%32 (query) and
-// m = 8*(i/4) + 4*(l/32) + i%4 (candidate within the 32-wide tile).
+TKA_DEV void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+TKA_DEV int wave_incl_scan_i(int v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Threshold selection over the ordered keys (high words) of up to 64*R
+// entries u[r] at position r*64 + lane (empty entries are ~0): the smallest
+// T, to 8*rounds leading bits (the remaining low bits of T set), with at
+// least kth entries whose key is <= T; all ones if there are fewer than kth
+// entries.  With rounds = 4, T is exactly the kth smallest key.  Radix
+// select on 8-bit digits with a 256-bin histogram in the wave's LDS (hist).
+template <int R>
+TKA_DEV uint32_t wave_select(const uint64_t (&u)[R], int kth, int rounds, int* hist)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t prefix = 0;
+    int below = 0;  // entries in bins below the selected ones so far
+    for (int rd = 0; rd < rounds; rd++) {
+        const int sh = 24 - 8 * rd;
+        const uint32_t hmask = rd == 0 ? 0u : (0xffffffffu << (sh + 8));
+#pragma unroll
+        for (int b = 0; b < 4; b++) hist[4 * lane + b] = 0;
+        wave_lds_sync();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const uint32_t key = (uint32_t)(u[r] >> 32);
+            if (u[r] != ~0ull && (key & hmask) == prefix) atomicAdd(&hist[(key >> sh) & 255u], 1);
+        }
+        wave_lds_sync();
+        int h[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) h[b] = hist[4 * lane + b];
+        const int s = h[0] + h[1] + h[2] + h[3];
+        const int incl = wave_incl_scan_i(s), excl = incl - s;
+        const int need = kth - below;
+        const uint64_t owner = __ballot(excl < need && incl >= need);
+        if (owner == 0) return 0xffffffffu;  // fewer than kth entries
+        const int ol = __ffsll((long long)owner) - 1;
+        int bin = 0, before = 0;
+        if (lane == ol) {
+            // first of the lane's bins at which the running count reaches need
+            int c = excl, b = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (b == k && c + h[k] < need) {
+                    c += h[k];
+                    b = k + 1;
+                }
+            }
+            bin = 4 * lane + b;
+            before = c;
+        }
+        bin = __shfl(bin, ol, 64);
+        before = __shfl(before, ol, 64);
+        below += before;
+        prefix |= (uint32_t)bin << sh;
+        wave_lds_sync();
+    }
+    const int low = 32 - 8 * rounds;
+    return low >= 32 ? 0xffffffffu : (prefix | (low ? ((1u << low) - 1u) : 0u));
+}
+
+// v[i] for a per-lane index i, by a tree of bit-mask merges (register arrays
+// indexed by a divergent value, or selects the compiler folds into such an
+// index, go to scratch memory).
+TKA_DEV uint32_t merge_bit(uint32_t lo, uint32_t hi, int i, int b)
+{
+    const uint32_t m = 0u - (uint32_t)((i >> b) & 1);
+    return (lo & ~m) | (hi & m);
+}
+TKA_DEV float pick16(const float (&v)[16], int i)
+{
+    uint32_t a[8], b[4], c[2];
+#pragma unroll
+    for (int k = 0; k < 8; k++) a[k] = merge_bit(__float_as_uint(v[2 * k]), __float_as_uint(v[2 * k + 1]), i, 0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) b[k] = merge_bit(a[2 * k], a[2 * k + 1], i, 1);
+#pragma unroll
+    for (int k = 0; k < 2; k++) c[k] = merge_bit(b[2 * k], b[2 * k + 1], i, 2);
+    return __uint_as_float(merge_bit(c[0], c[1], i, 3));
+}
+
+// Filter of the 16 values a lane holds for its query (tile t of the chunk):
+// lane l holds C[m][n] with n = l%32 (query) and m = 8*(i/4) + 4*(l/32) + i%4
+// (candidate within the 32-wide tile).  Values below the row's threshold are
+// appended to the row buffer: one min and one compare per lane when nothing
+// passes; otherwise one LDS atomic reserves the lane's slots and a loop over
+// the lane's passing values stores them (a uniform loop over all 16 values
+// with predicated stores measured slower).
 TKA_DEV void filter_tile(const float (&v)[16], int t, int cb, int q, int slot_local, float tau, const TileArgs& a,
                          int* s_cnt, uint64_t* qbuf)
 {
-    const int lane = threadIdx.x & 63;
-    // Once the threshold has settled almost no value passes: one min per lane
-    // decides whether the per-value tests are needed at all.
+    const int half = (threadIdx.x & 63) >> 5;
     float mn = v[0];
 #pragma unroll
     for (int i = 1; i < 16; i++) mn = fminf(mn, v[i]);
-    if (!(mn < tau)) return;
+    if (!(mn < tau) || q < 0) return;
+    const int base = cb + 32 * t + 4 * half;  // candidate of value i: base + 8*(i/4) + i%4
+    const bool edge = base + 27 >= a.N || (q >= base && q <= base + 27);
+    uint32_t m = 0;
 #pragma unroll
     for (int i = 0; i < 16; i++) {
-        const int m = 32 * t + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
-        const int j = cb + m;
-        if (q >= 0 && j < a.N && j != q && v[i] < tau) {
-            const int s = atomicAdd(&s_cnt[slot_local], 1);
-            qbuf[s] = ((uint64_t)f2ord(v[i]) << 32) | (uint32_t)j;
-        }
+        const int j = base + 8 * (i >> 2) + (i & 3);
+        const bool ok = v[i] < tau && !(edge && (j >= a.N || j == q));
+        m |= (uint32_t)ok << i;
+    }
+    if (m == 0) return;
+    int s = atomicAdd(&s_cnt[slot_local], __popc(m));
+    while (m) {
+        const int i = __ffs(m) - 1;
+        m &= m - 1;
+        const int j = base + 8 * (i >> 2) + (i & 3);
+        qbuf[s++] = ((uint64_t)f2ord(pick16(v, i)) << 32) | (uint32_t)j;
     }
 }
 
 // After a chunk: every row of this wave whose buffer could overflow on the
-// next chunk is sorted and cut to kprime entries.
-TKA_DEV void compact_if_needed(int w, const TileArgs& a, int slot0, int* s_cnt, float* s_tau)
+// next chunk keeps only its entries up to a selected threshold (>= kprime of
+// them, 16-bit resolution), which becomes the row's new threshold.  If that
+// leaves too many, the threshold T is the exact kprime-th key: entries below
+// it are kept, and entries equal to it only up to the capacity, the new
+// threshold being T itself (a dropped entry then equals the threshold, which
+// the re-rank's strict certificate treats as possibly closer: such a row is
+// re-solved if it matters).
+TKA_DEV void compact_if_needed(int w, const TileArgs& a, int slot0, int* s_cnt, float* s_tau, int* hist)
 {
     const int lane = threadIdx.x & 63;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_lds_sync();
     const int my = 32 * w + (lane & 31);
     const bool need = lane < 32 && s_cnt[my] > kCap - kCB;
     uint64_t mask = __ballot(need);
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     while (mask) {
         const int l = __ffsll((long long)mask) - 1;
         mask &= mask - 1;
@@ -348,18 +425,42 @@ TKA_DEV void compact_if_needed(int w, const TileArgs& a, int slot0, int* s_cnt, 
             const int p = r * 64 + lane;
             v[r] = p < c ? qbuf[p] : ~0ull;
         }
-        wave_bitonic_sort<8>(v);
         const int kp = a.kprime;
+        uint32_t T = wave_select<8>(v, kp, 2, hist);
+        int kept = 0;
+#pragma unroll
+        for (int r = 0; r < 8; r++) kept += __popcll(__ballot(v[r] != ~0ull && (uint32_t)(v[r] >> 32) <= T));
+        bool exact = false;
+        if (kept > kCap - kCB) {
+            T = wave_select<8>(v, kp, 4, hist);
+            exact = true;
+        }
+        // keep key < T, and key == T (all of them, or with an exact T up to
+        // the capacity left), in buffer order
+        int pos = 0;
 #pragma unroll
         for (int r = 0; r < 8; r++) {
-            const int p = r * 64 + lane;
-            if (p < kp) qbuf[p] = v[r];
-            if (p == kp - 1) s_tau[sl] = ord2f((uint32_t)(v[r] >> 32));
+            const uint32_t key = (uint32_t)(v[r] >> 32);
+            const bool keep = v[r] != ~0ull && key < T;
+            const uint64_t km = __ballot(keep);
+            if (keep) qbuf[pos + __popcll(km & lt)] = v[r];
+            pos += __popcll(km);
         }
-        if (lane == 0) s_cnt[sl] = kp;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+            const uint32_t key = (uint32_t)(v[r] >> 32);
+            const bool tie = v[r] != ~0ull && key == T;
+            const uint64_t tm = __ballot(tie);
+            const int p = pos + __popcll(tm & lt);
+            if (tie && p < kCap - kCB) qbuf[p] = v[r];
+            pos += __popcll(tm);
+        }
+        if (pos > kCap - kCB) pos = kCap - kCB;
+        if (lane == 0) {
+            s_cnt[sl] = pos;
+            s_tau[sl] = exact ? ord2f(T) : ((T >= 0xff7fffffu) ? HUGE_VALF : ord2f(T + 1u));
+        }
+        wave_lds_sync();
     }
 }
 
@@ -373,6 +474,7 @@ __global__ __launch_bounds__(256) void knn_tiles_fp32(const float* __restrict__ 
     __shared__ float sN[kCB];
     __shared__ int s_cnt[kQB];
     __shared__ float s_tau[kQB];
+    __shared__ int s_hist[4][256];      // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * kQB;
     const int sl = 32 * w + (lane & 31);
@@ -427,7 +529,7 @@ __global__ __launch_bounds__(256) void knn_tiles_fp32(const float* __restrict__ 
             }
             filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
         }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau);
+        compact_if_needed(w, a, slot0, s_cnt, s_tau, s_hist[w]);
     }
     if (lane < 32 && slot0 + sl < a.nq) {
         a.cnt_out[slot0 + sl] = s_cnt[sl];
@@ -444,6 +546,7 @@ __global__ __launch_bounds__(256) void knn_tiles_bf16(const __bf16* __restrict__
     __shared__ bf16x8 sA[kCB][KV + 1];      // +1 vector of padding per row
     __shared__ int s_cnt[kQB];
     __shared__ float s_tau[kQB];
+    __shared__ int s_hist[4][256];      // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * kQB;
     const int sl = 32 * w + (lane & 31);
@@ -488,7 +591,7 @@ __global__ __launch_bounds__(256) void knn_tiles_bf16(const __bf16* __restrict__
             for (int i = 0; i < 16; i++) v[i] = acc[t][i];
             filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
         }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau);
+        compact_if_needed(w, a, slot0, s_cnt, s_tau, s_hist[w]);
     }
     if (lane < 32 && slot0 + sl < a.nq) {
         a.cnt_out[slot0 + sl] = s_cnt[sl];
@@ -508,6 +611,7 @@ __global__ __launch_bounds__(256) void knn_tiles_fp8(const uint8_t* __restrict__
     __shared__ float sN[kCB], sS[kCB];
     __shared__ int s_cnt[kQB];
     __shared__ float s_tau[kQB];
+    __shared__ int s_hist[4][256];      // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * kQB;
     const int sl = 32 * w + (lane & 31);
@@ -559,7 +663,7 @@ __global__ __launch_bounds__(256) void knn_tiles_fp8(const uint8_t* __restrict__
             }
             filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
         }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau);
+        compact_if_needed(w, a, slot0, s_cnt, s_tau, s_hist[w]);
     }
     if (lane < 32 && slot0 + sl < a.nq) {
         a.cnt_out[slot0 + sl] = s_cnt[sl];
@@ -590,6 +694,219 @@ __global__ __launch_bounds__(256) void prep_fp8(const double* __restrict__ X, co
     }
 }
 
+// ---------------------------------------------------------------- register tiles
+// Operand rows in half-major order for the MFMA layouts: a row is 2*NV
+// vectors plus one pad vector, and vector h*NV + g holds the k values that
+// lane half h feeds to step group g.  The pad carries the FP32 norm ||c||^2
+// and (FP8) the row scale.  Rows are allocated for a whole number of kCB-row
+// chunks (zero rows past N).  A wave keeps its query operand in registers
+// for the whole candidate stream; candidate chunks are copied global -> LDS
+// by the direct LDS loads of CDNA (no VGPR staging), double-buffered, the
+// copy of chunk c+1 running while chunk c is multiplied and filtered.  The
+// odd row stride (2*NV + 1 vectors) spreads consecutive rows over the LDS
+// banks.
+constexpr int kNVLimit = 16;  // widest operand row (vectors per half) kept in registers
+
+struct OpBF16 {  // v_mfma_f32_32x32x16_bf16: one k-step of 16 per vector pair
+    typedef bf16x8 V;
+    TKA_DEV static f32x16 mma(V a, V b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0); }
+};
+struct OpFP8 {  // v_mfma_f32_32x32x16_fp8_fp8
+    typedef long V;
+    TKA_DEV static f32x16 mma(V a, V b, f32x16 c) { return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, b, c, 0, 0, 0); }
+};
+struct OpF32 {  // four k-steps of v_mfma_f32_32x32x2_f32 per vector
+    typedef float4 V;
+    TKA_DEV static f32x16 mma(V a, V b, f32x16 c)
+    {
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(a.x, b.x, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(a.y, b.y, c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_32x32x2f32(a.z, b.z, c, 0, 0, 0);
+        return __builtin_amdgcn_mfma_f32_32x32x2f32(a.w, b.w, c, 0, 0, 0);
+    }
+};
+
+typedef __attribute__((address_space(3))) void lds_void;
+
+// Copies `bytes` (a multiple of 1024) from global src to LDS dst, 16 bytes
+// per lane, each wave instruction filling one contiguous KiB.
+TKA_DEV void dma_to_lds(const unsigned char* src, unsigned char* dst, int bytes)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int e = w; e < bytes / 1024; e += 4)
+        __builtin_amdgcn_global_load_lds(src + (size_t)e * 1024 + 16 * lane, (lds_void*)(dst + e * 1024), 16, 0, 0);
+}
+
+// kMode 0: the value is the accumulator (BF16: ||c||^2 folded into the operands)
+//       1: ||c||^2 - 2 acc (FP32)
+//       2: ||c||^2 - 2 s_q s_c acc (FP8 with per-row scales)
+template <class Op, int kMode, int kNVMax>
+__global__ __launch_bounds__(256) void knn_tiles_reg(const typename Op::V* __restrict__ A,
+                                                     const typename Op::V* __restrict__ B, int NV, TileArgs a)
+{
+    typedef typename Op::V V;
+    extern __shared__ __align__(16) unsigned char dyn_lds[];  // two chunks of kCB rows x (2*NV + 1) vectors
+    __shared__ int s_cnt[kQB];
+    __shared__ float s_tau[kQB];
+    __shared__ int s_hist[4][256];  // per-wave selection histogram
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
+    const int slot0 = blockIdx.x * kQB;
+    const int sl = 32 * w + (lane & 31);
+    const int q = query_row(a, slot0 + sl);
+    const int RS = 2 * NV + 1;
+    const int chunk_bytes = kCB * RS * (int)sizeof(V);
+    if (tid < kQB) {
+        s_cnt[tid] = 0;
+        s_tau[tid] = HUGE_VALF;
+    }
+    V bq[kNVMax];
+    float sq = 0.f;
+    {
+        const V* qr = B + (size_t)(q >= 0 ? q : 0) * RS;
+#pragma unroll
+        for (int g = 0; g < kNVMax; g++) bq[g] = (g < NV && q >= 0) ? qr[half * NV + g] : V{};
+        if (kMode == 2 && q >= 0) sq = reinterpret_cast<const float*>(qr + 2 * NV)[1];
+    }
+    uint64_t* qbuf = a.buf + (size_t)(slot0 + sl) * kCap;
+    const unsigned char* src = reinterpret_cast<const unsigned char*>(A);
+    const int nch = (a.N + kCB - 1) / kCB;
+    const int c0 = a.rotate ? (int)blockIdx.x % nch : 0;
+    dma_to_lds(src + (size_t)c0 * kCB * RS * sizeof(V), dyn_lds, chunk_bytes);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int c = 0; c < nch; c++) {
+        const int cb = ((c0 + c) % nch) * kCB;
+        unsigned char* cur = dyn_lds + (c & 1) * chunk_bytes;
+        if (c + 1 < nch)
+            dma_to_lds(src + (size_t)(((c0 + c + 1) % nch) * kCB) * RS * sizeof(V), dyn_lds + ((c + 1) & 1) * chunk_bytes,
+                       chunk_bytes);
+        const V* sA = reinterpret_cast<const V*>(cur);
+        f32x16 acc[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[t] = f32x16{};
+        const V* rowp = sA + (lane & 31) * RS + half * NV;
+#pragma unroll
+        for (int g = 0; g < kNVMax; g++) {
+            if (g < NV) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[t] = Op::mma(rowp[32 * t * RS + g], bq[g], acc[t]);
+            }
+        }
+        const float tau = s_tau[sl];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            float v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const int m = 32 * t + 8 * (i >> 2) + 4 * half + (i & 3);
+                if (kMode == 0) {
+                    v[i] = acc[t][i];
+                } else {
+                    const float* pad = reinterpret_cast<const float*>(sA + m * RS + 2 * NV);
+                    v[i] = kMode == 1 ? fmaf(-2.f, acc[t][i], pad[0]) : fmaf(-2.f * sq * pad[1], acc[t][i], pad[0]);
+                }
+            }
+            filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
+        }
+        compact_if_needed(w, a, slot0, s_cnt, s_tau, s_hist[w]);
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copy of the next chunk has landed
+        __syncthreads();                // every wave's has, and the current chunk is free
+    }
+    if (lane < 32 && slot0 + sl < a.nq) {
+        a.cnt_out[slot0 + sl] = s_cnt[sl];
+        a.tau_out[slot0 + sl] = s_tau[sl];
+    }
+}
+
+// Padded half-major BF16 operands (hi/lo splits as prep_bf16x3): vector
+// (h, g) of a row holds k = 16 g + 8 h + 0..7; the pad vector and the rows
+// past N are zero.  A and B share the layout (B: the query-side factors).
+__global__ __launch_bounds__(256) void prep_bf16x3_hm(const double* __restrict__ X, const double* __restrict__ n2,
+                                                      int N, int Npad, int D, int NV, __bf16* __restrict__ A,
+                                                      __bf16* __restrict__ B)
+{
+    const int RSe = 8 * (2 * NV + 1);  // bf16 per padded row
+    const size_t total = (size_t)Npad * RSe;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const int n = (int)(e / RSe), p = (int)(e % RSe);
+        float av = 0.f, bv = 0.f;
+        if (n < N && p < 16 * NV) {
+            const int vi = p >> 3, el = p & 7, h = vi / NV, g = vi - h * NV;
+            const int k = 16 * g + 8 * h + el;
+            if (k < 3 * D) {
+                const int seg = k / D, d = k % D;
+                const double x = X[(size_t)n * D + d];
+                const __bf16 hi = (__bf16)x;
+                const __bf16 lo = (__bf16)(x - (double)(float)hi);
+                if (seg == 0) {
+                    av = (float)hi;
+                    bv = -2.f * (float)hi;
+                } else if (seg == 1) {
+                    av = (float)lo;
+                    bv = -2.f * (float)hi;
+                } else {
+                    av = (float)hi;
+                    bv = -2.f * (float)lo;
+                }
+            } else if (k == 3 * D || k == 3 * D + 1) {
+                const double nn = n2[n];
+                const __bf16 hi = (__bf16)nn;
+                const __bf16 lo = (__bf16)(nn - (double)(float)hi);
+                av = (k == 3 * D) ? (float)hi : (float)lo;
+                bv = 1.f;
+            }
+        }
+        A[e] = (__bf16)av;
+        B[e] = (__bf16)bv;
+    }
+}
+
+// Padded half-major FP32 operand: S k-steps of 2 per row (S = 4 NV);
+// position h*S + s holds k = 2 s + h; the pad vector is {||x||^2, 0, 0, 0}.
+__global__ __launch_bounds__(256) void prep_fp32_hm(const double* __restrict__ X, const double* __restrict__ n2, int N,
+                                                    int Npad, int D, int NV, float* __restrict__ Xh)
+{
+    const int S = 4 * NV, RSe = 4 * (2 * NV + 1);
+    const size_t total = (size_t)Npad * RSe;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const int n = (int)(e / RSe), p = (int)(e % RSe);
+        float v = 0.f;
+        if (n < N) {
+            if (p < 2 * S) {
+                const int h = p / S, k = 2 * (p - h * S) + h;
+                v = k < D ? (float)X[(size_t)n * D + k] : 0.f;
+            } else if (p == 2 * S) {
+                v = (float)n2[n];
+            }
+        }
+        Xh[e] = v;
+    }
+}
+
+// Padded half-major FP8 operand from the k-ordered rows written by prep_fp8
+// (vector (h, g) = 8 bytes holding k = 16 g + 8 h + 0..7); the pad vector
+// holds the floats {||x||^2, scale}.
+__global__ __launch_bounds__(256) void fp8_to_hm(const uint8_t* __restrict__ src, const float* __restrict__ nrm,
+                                                 const float* __restrict__ scale, int N, int Npad, int NV,
+                                                 uint64_t* __restrict__ dst)
+{
+    const int RS = 2 * NV + 1;
+    const size_t total = (size_t)Npad * RS;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const int n = (int)(e / RS), vi = (int)(e % RS);
+        uint64_t v = 0;
+        if (n < N) {
+            if (vi < 2 * NV) {
+                const int h = vi / NV, g = vi - h * NV;
+                v = *reinterpret_cast<const uint64_t*>(src + (size_t)n * 16 * NV + 16 * g + 8 * h);
+            } else {
+                v = ((uint64_t)__float_as_uint(scale[n]) << 32) | __float_as_uint(nrm[n]);
+            }
+        }
+        dst[e] = v;
+    }
+}
+
 // ---------------------------------------------------------------- rerank
 struct RerankArgs {
     const double* X;      // N x D normalised FP64
@@ -606,6 +923,7 @@ struct RerankArgs {
     int* idx;             // N x K (indexed by row)
     double* dist;
     int* fail;            // per slot: 1 if the bound failed
+    const uint32_t* perm; // optional: X rows are the caller's rows perm[0..N) (outputs and ties use those)
 };
 
 // Exact FP64 squared distance, summed in index order without contraction
@@ -613,32 +931,38 @@ struct RerankArgs {
 TKA_DEV double exact_sqdist(const double* __restrict__ xq, const double* __restrict__ xj, int D)
 {
     double dd = 0.0;
-    for (int d = 0; d < D; d++) {
+    int d = 0;
+    for (; d + 8 <= D; d += 8) {  // loads of a group issued together; sums stay in index order
+        double a[8], b[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            a[k] = xq[d + k];
+            b[k] = xj[d + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const double t = dadd_sep(a[k], -b[k]);
+            dd = dadd_sep(dd, dmul_sep(t, t));
+        }
+    }
+    for (; d < D; d++) {
         const double t = dadd_sep(xq[d], -xj[d]);
         dd = dadd_sep(dd, dmul_sep(t, t));
     }
     return dd;
 }
 
-// Sorts the first R registers' worth of keys (R in {1, 2, 4, 8}).
-TKA_DEV void sort_prefix(DKey (&v)[8], int R)
+// Sorts the first 64*R entries (R in {1, 2, 4, 8}: the smallest that holds them).
+TKA_DEV void sort_prefix(uint64_t (&d)[8], uint32_t (&ix)[8], int R)
 {
-    if (R <= 1) {
-        DKey u[1] = {v[0]};
-        wave_bitonic_sort<1>(u);
-        v[0] = u[0];
-    } else if (R <= 2) {
-        DKey u[2] = {v[0], v[1]};
-        wave_bitonic_sort<2>(u);
-        v[0] = u[0];
-        v[1] = u[1];
-    } else if (R <= 4) {
-        DKey u[4] = {v[0], v[1], v[2], v[3]};
-        wave_bitonic_sort<4>(u);
-        for (int r = 0; r < 4; r++) v[r] = u[r];
-    } else {
-        wave_bitonic_sort<8>(v);
-    }
+    if (R <= 1)
+        wave_sort_pairs<1>(d, ix);
+    else if (R <= 2)
+        wave_sort_pairs<2>(d, ix);
+    else if (R <= 4)
+        wave_sort_pairs<4>(d, ix);
+    else
+        wave_sort_pairs<8>(d, ix);
 }
 
 // One wave per row.  The kept candidates are sorted by their approximate
@@ -654,6 +978,7 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     const int slot = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (slot >= a.nq) return;
     const int q = a.qlist ? a.qlist[slot] : slot;
+    const int qo = a.perm ? (int)a.perm[q] : q;  // the caller's row
     const int c = a.cnt[slot] < kCap ? a.cnt[slot] : kCap;
     const uint64_t* qbuf = a.buf + (size_t)slot * kCap;
     const double* xq = a.X + (size_t)q * a.D;
@@ -661,53 +986,54 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     const double eps = a.eps_a * sqrt(qn2) * a.rmax + a.eps_b * a.rmax * a.rmax + 1e-12 * (qn2 + a.rmax * a.rmax);
     const int K = a.K;
 
-    // approximate order
+    __shared__ int s_hist[4][256];
+    __shared__ uint64_t s_list[4][kCap];
+    const int w = threadIdx.x >> 6;
     uint64_t u[8];
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         const int p = r * 64 + lane;
         u[r] = p < c ? qbuf[p] : ~0ull;
     }
-    wave_bitonic_sort<8>(u);
-    uint32_t vk_bits = 0;
-#pragma unroll
-    for (int r = 0; r < 8; r++)
-        if (r * 64 + lane == K - 1) vk_bits = (uint32_t)(u[r] >> 32);
-    vk_bits = __shfl(vk_bits, (K - 1) & 63, 64);
-    const double thr = (double)ord2f(vk_bits) + 2.0 * eps;
-    int m = 0;  // candidates within the threshold (a prefix of the sorted list)
+    // v_(K) exactly (radix select), then the candidates within thr, packed
+    const uint32_t vk_bits = wave_select<8>(u, K, 4, s_hist[w]);
+    const double thr = c >= K ? (double)ord2f(vk_bits) + 2.0 * eps : HUGE_VAL;
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int m = 0;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
-        const int p = r * 64 + lane;
-        const bool in = p < c && (double)ord2f((uint32_t)(u[r] >> 32)) <= thr;
-        m += __popcll(__ballot(in));
+        const bool in = u[r] != ~0ull && (double)ord2f((uint32_t)(u[r] >> 32)) <= thr;
+        const uint64_t im = __ballot(in);
+        if (in) s_list[w][m + __popcll(im & lt)] = u[r];
+        m += __popcll(im);
     }
-    if (c < K) m = c;
+    wave_lds_sync();
 
-    DKey v[8];
+    uint64_t vd[8];
+    uint32_t vi[8];
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         const int p = r * 64 + lane;
         if (p < m) {
-            const uint32_t j = (uint32_t)u[r];
-            v[r].d = (uint64_t)__double_as_longlong(exact_sqdist(xq, a.X + (size_t)j * a.D, a.D));
-            v[r].i = j;
+            const uint32_t j = (uint32_t)s_list[w][p];
+            vd[r] = (uint64_t)__double_as_longlong(exact_sqdist(xq, a.X + (size_t)j * a.D, a.D));
+            vi[r] = a.perm ? a.perm[j] : j;
         } else {
-            v[r].d = ~0ull;
-            v[r].i = 0xffffffffu;
+            vd[r] = ~0ull;
+            vi[r] = 0xffffffffu;
         }
     }
     const int R = m <= 64 ? 1 : m <= 128 ? 2 : m <= 256 ? 4 : 8;
-    sort_prefix(v, R);
+    sort_prefix(vd, vi, R);
     double dk = 0.0;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         const int p = r * 64 + lane;
         if (p < K) {
-            a.idx[(size_t)q * K + p] = (int)v[r].i;
-            a.dist[(size_t)q * K + p] = __longlong_as_double((long long)v[r].d);
+            a.idx[(size_t)qo * K + p] = (int)vi[r];
+            a.dist[(size_t)qo * K + p] = __longlong_as_double((long long)vd[r]);
         }
-        if (p == K - 1) dk = __longlong_as_double((long long)v[r].d);
+        if (p == K - 1) dk = __longlong_as_double((long long)vd[r]);
     }
     dk = __shfl(dk, (K - 1) & 63, 64);
     if (lane == 0) {
@@ -722,10 +1048,103 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     }
 }
 
-__global__ void collect_failures(const int* __restrict__ fail, int n, int* __restrict__ list, int* __restrict__ count)
+__global__ void collect_failures(const int* __restrict__ fail, int n, const uint32_t* __restrict__ perm,
+                                 int* __restrict__ list, int* __restrict__ count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && fail[i]) list[atomicAdd(count, 1)] = i;
+    if (i < n && fail[i]) list[atomicAdd(count, 1)] = perm ? (int)perm[i] : i;
+}
+
+// ---------------------------------------------------------------- locality order
+// Rows are streamed in an order where nearby rows sit in nearby chunks, so a
+// block's first chunks (its own rows, rotate = 1) already hold most of its
+// neighbours and the thresholds fall early: few values pass the filter
+// afterwards.  Order: Morton code of a random 3-D projection of the rows.
+// The kNN result does not depend on it (ties are broken by the caller's row
+// index in the re-rank).
+__global__ __launch_bounds__(256) void proj3(const double* __restrict__ X, int N, int D, const float* __restrict__ R,
+                                             float* __restrict__ P, float* __restrict__ part)
+{
+    __shared__ float sh[6][4];
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    float v[6] = {HUGE_VALF, HUGE_VALF, HUGE_VALF, -HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
+    if (n < N) {
+        float p[3] = {0.f, 0.f, 0.f};
+        for (int d = 0; d < D; d++) {
+            const float x = (float)X[(size_t)n * D + d];
+#pragma unroll
+            for (int k = 0; k < 3; k++) p[k] = fmaf(x, R[3 * d + k], p[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            P[3 * (size_t)n + k] = p[k];
+            v[k] = v[k + 3] = p[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        for (int o = 32; o > 0; o >>= 1) {
+            v[k] = fminf(v[k], __shfl_xor(v[k], o, 64));
+            v[k + 3] = fmaxf(v[k + 3], __shfl_xor(v[k + 3], o, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) sh[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float r = sh[threadIdx.x][0];
+        for (int w = 1; w < 4; w++) r = threadIdx.x < 3 ? fminf(r, sh[threadIdx.x][w]) : fmaxf(r, sh[threadIdx.x][w]);
+        part[6 * blockIdx.x + threadIdx.x] = r;
+    }
+}
+
+TKA_DEV uint64_t spread21(uint64_t v)
+{
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
+    return v;
+}
+
+__global__ __launch_bounds__(256) void morton3(const float* __restrict__ P, int N, const float* __restrict__ part,
+                                               int nb, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+{
+    __shared__ float lim[6];
+    if (threadIdx.x < 6) {
+        float r = part[threadIdx.x];
+        for (int b = 1; b < nb; b++)
+            r = threadIdx.x < 3 ? fminf(r, part[6 * b + threadIdx.x]) : fmaxf(r, part[6 * b + threadIdx.x]);
+        lim[threadIdx.x] = r;
+    }
+    __syncthreads();
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    uint64_t key = 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float span = lim[k + 3] - lim[k];
+        float t = span > 0.f ? (P[3 * (size_t)n + k] - lim[k]) / span : 0.f;
+        t = fminf(fmaxf(t, 0.f), 1.f);
+        key |= spread21((uint64_t)(t * 2097151.f)) << k;
+    }
+    keys[n] = key;
+    vals[n] = (uint32_t)n;
+}
+
+__global__ __launch_bounds__(256) void gather_rows(const double* __restrict__ X, const double* __restrict__ n2,
+                                                   const uint32_t* __restrict__ perm, int N, int D,
+                                                   double* __restrict__ Xp, double* __restrict__ n2p)
+{
+    const size_t total = (size_t)N * D;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const size_t i = e / D, d = e % D;
+        const uint32_t src = perm[i];
+        Xp[e] = X[(size_t)src * D + d];
+        if (d == 0) n2p[i] = n2[src];
+    }
 }
 
 }  // namespace
@@ -769,7 +1188,7 @@ void run_tiles(int dtype, const void* op_a, const void* op_b, const float* nrm, 
     tr.cnt.alloc(nslots, mb);
     tr.tau.alloc(nslots, mb);
     tr.fail.alloc(nslots, mb);
-    TileArgs ta{N, kprime, qlist, nq, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
+    TileArgs ta{N, kprime, 0, qlist, nq, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
     const dim3 grid(ceil_div(nq, kQB));
     if (dtype == TKA_DIST_BF16) {
         hipLaunchKernelGGL(knn_tiles_bf16, grid, dim3(256), 0, s, (const __bf16*)op_a, (const __bf16*)op_b, Kp, ta);
@@ -779,6 +1198,36 @@ void run_tiles(int dtype, const void* op_a, const void* op_b, const float* nrm, 
     } else {
         hipLaunchKernelGGL(knn_tiles_fp32, grid, dim3(256), 0, s, (const float*)op_a, nrm, D, ta);
     }
+    TKA_LAUNCH_CHECK();
+}
+
+// All rows against all rows with the query operand in registers
+// (NV <= kNVLimit; the register arrays are sized for the next multiple of 4).
+// A (candidates), B (queries): padded half-major rows for ceil(N / kCB) * kCB rows.
+template <class Op, int kMode>
+void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, TileRun& tr, MemBudget* mb, hipStream_t s)
+{
+    typedef typename Op::V V;
+    require(NV >= 1 && NV <= kNVLimit, TKA_E_INTERNAL, "register tiles: operand too wide");
+    const int nslots = (int)ceil_div(N, kQB) * kQB;
+    tr.buf.alloc((size_t)nslots * kCap, mb);
+    tr.cnt.alloc(nslots, mb);
+    tr.tau.alloc(nslots, mb);
+    tr.fail.alloc(nslots, mb);
+    TileArgs ta{N, kprime, 1, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
+    const size_t lds = 2 * (size_t)kCB * (2 * NV + 1) * sizeof(V);
+    const dim3 grid(ceil_div(N, kQB));
+#define TKA_REG(W) \
+    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, W>), grid, dim3(256), lds, s, (const V*)A, (const V*)B, NV, ta)
+    if (NV <= 4)
+        TKA_REG(4);
+    else if (NV <= 8)
+        TKA_REG(8);
+    else if (NV <= 12)
+        TKA_REG(12);
+    else
+        TKA_REG(16);
+#undef TKA_REG
     TKA_LAUNCH_CHECK();
 }
 
@@ -832,47 +1281,117 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         eps_b = 2.0 * (2.0 * u32);
     };
     TileRun tr;
+    const unsigned gprep = (unsigned)std::min<size_t>(8192, ceil_div((size_t)N * 64, 256));
+    const int Npad = (int)ceil_div(N, kCB) * kCB;
+
+    // Locality order for the register tiles: rows permuted by the Morton code
+    // of a fixed random 3-D projection (see proj3).  Xs/ns are the rows the
+    // tiles and the re-rank read; perm maps them back to the caller's rows.
+    DevBuf<uint32_t> perm;
+    DevBuf<double> Xp, n2p;
+    const double* Xs = Xn;
+    const double* ns = nullptr;
+    auto locality = [&]() {
+        std::vector<float> R(3 * (size_t)D);
+        uint64_t st64 = 0x9E3779B97F4A7C15ull;
+        for (auto& r : R) {  // uniform in (-1, 1): any fixed full-rank projection works
+            st64 = st64 * 6364136223846793005ull + 1442695040888963407ull;
+            r = (float)((double)(st64 >> 11) * 0x1.0p-53 * 2.0 - 1.0);
+        }
+        DevBuf<float> Rd(R.size(), mb), P((size_t)N * 3, mb), part((size_t)6 * nbr, mb);
+        TKA_HIP(hipMemcpyAsync(Rd.get(), R.data(), R.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(proj3, dim3(nbr), dim3(256), 0, s, Xn, N, D, Rd.get(), P.get(), part.get());
+        TKA_LAUNCH_CHECK();
+        DevBuf<uint64_t> k0(N, mb), k1(N, mb);
+        DevBuf<uint32_t> v0(N, mb), v1(N, mb), scr(radix_sort_scratch_words(N), mb);
+        TKA_HIP(hipMemsetAsync(scr.get(), 0, scr.size() * sizeof(uint32_t), s));
+        hipLaunchKernelGGL(morton3, dim3(nbr), dim3(256), 0, s, P.get(), N, part.get(), nbr, k0.get(), v0.get());
+        TKA_LAUNCH_CHECK();
+        const bool alt = radix_sort_pairs<uint64_t, uint32_t>(k0.get(), v0.get(), k1.get(), v1.get(), N, 63, scr.get(), s);
+        perm.alloc(N, mb);
+        TKA_HIP(hipMemcpyAsync(perm.get(), alt ? v1.get() : v0.get(), (size_t)N * sizeof(uint32_t),
+                               hipMemcpyDeviceToDevice, s));
+        Xp.alloc((size_t)N * D, mb);
+        n2p.alloc(N, mb);
+        hipLaunchKernelGGL(gather_rows, dim3(gprep), dim3(256), 0, s, Xn, n2.get(), perm.get(), N, D, Xp.get(),
+                           n2p.get());
+        TKA_LAUNCH_CHECK();
+        TKA_HIP(hipStreamSynchronize(s));  // the temporaries are released on return
+        Xs = Xp.get();
+        ns = n2p.get();
+    };
     if (dtype == TKA_DIST_BF16) {
         Kp = ((3 * D + 2) + 15) / 16 * 16;
-        A.alloc((size_t)N * Kp, mb);
-        B.alloc((size_t)N * Kp, mb);
-        hipLaunchKernelGGL(prep_bf16x3, dim3(std::min<size_t>(8192, ceil_div((size_t)N * Kp, 256))), dim3(256), 0, s,
-                           Xn, n2.get(), N, D, Kp, A.get(), B.get());
-        TKA_LAUNCH_CHECK();
-        run_tiles(dtype, A.get(), B.get(), nullptr, N, D, Kp, kprime, nullptr, N, tr, mb, s);
+        const int NV = Kp / 16;
+        if (NV <= kNVLimit) {
+            locality();
+            const size_t rows = (size_t)Npad * 8 * (2 * NV + 1);
+            A.alloc(rows, mb);
+            B.alloc(rows, mb);
+            hipLaunchKernelGGL(prep_bf16x3_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, A.get(), B.get());
+            TKA_LAUNCH_CHECK();
+            run_tiles_reg<OpBF16, 0>(A.get(), B.get(), NV, N, kprime, tr, mb, s);
+        } else {
+            A.alloc((size_t)N * Kp, mb);
+            B.alloc((size_t)N * Kp, mb);
+            hipLaunchKernelGGL(prep_bf16x3, dim3(gprep), dim3(256), 0, s, Xn, n2.get(), N, D, Kp, A.get(), B.get());
+            TKA_LAUNCH_CHECK();
+            run_tiles(dtype, A.get(), B.get(), nullptr, N, D, Kp, kprime, nullptr, N, tr, mb, s);
+        }
         const double ub2 = std::ldexp(1.0, -16);
         const double gam = Kp * u32;
         eps_a = 2.0 * (6.2 * ub2 + 2.1 * gam);
         eps_b = 2.0 * (1.1 * ub2 + 1.1 * gam);
     } else if (dtype == TKA_DIST_FP8) {
         Kp = (D + 15) / 16 * 16;
+        const int NV = Kp / 16;
+        if (NV <= kNVLimit) locality();
         A8.alloc((size_t)N * Kp, mb);
         scale8.alloc(N, mb);
         nrm8.alloc(N, mb);
-        hipLaunchKernelGGL(prep_fp8, dim3(ceil_div(N, 256)), dim3(256), 0, s, Xn, n2.get(), N, D, Kp, A8.get(),
-                           scale8.get(), nrm8.get());
+        hipLaunchKernelGGL(prep_fp8, dim3(ceil_div(N, 256)), dim3(256), 0, s, Xs, ns ? ns : n2.get(), N, D, Kp,
+                           A8.get(), scale8.get(), nrm8.get());
         TKA_LAUNCH_CHECK();
-        run_tiles(dtype, A8.get(), scale8.get(), nrm8.get(), N, D, Kp, kprime, nullptr, N, tr, mb, s);
+        if (NV <= kNVLimit) {
+            DevBuf<uint64_t> A8h((size_t)Npad * (2 * NV + 1), mb);
+            hipLaunchKernelGGL(fp8_to_hm, dim3(gprep), dim3(256), 0, s, A8.get(), nrm8.get(), scale8.get(), N, Npad,
+                               NV, A8h.get());
+            TKA_LAUNCH_CHECK();
+            run_tiles_reg<OpFP8, 2>(A8h.get(), A8h.get(), NV, N, kprime, tr, mb, s);
+            TKA_HIP(hipStreamSynchronize(s));  // A8h is released on return
+        } else {
+            run_tiles(dtype, A8.get(), scale8.get(), nrm8.get(), N, D, Kp, kprime, nullptr, N, tr, mb, s);
+        }
         // e4m3 keeps 4 significant bits (unit roundoff 2^-4): the bound is
         // loose at any useful k'; FP8 rows are not re-solved (approximate mode)
         const double u8 = std::ldexp(1.0, -4);
         eps_a = 2.0 * (2.0 * (2 * u8 + u8 * u8) + 2.0 * D * u32);
         eps_b = 2.0 * (2.0 * u32);
     } else {
-        make_fp32();
-        run_tiles(dtype, Xf.get(), nullptr, nrm.get(), N, D, 0, kprime, nullptr, N, tr, mb, s);
+        const int NV = ((D + 1) / 2 + 3) / 4;  // float4 vectors of k-steps (2 k per step) per half row
+        if (NV <= kNVLimit) {
+            locality();
+            DevBuf<float> Xh((size_t)Npad * 4 * (2 * NV + 1), mb);
+            hipLaunchKernelGGL(prep_fp32_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, Xh.get());
+            TKA_LAUNCH_CHECK();
+            run_tiles_reg<OpF32, 1>(Xh.get(), Xh.get(), NV, N, kprime, tr, mb, s);
+            TKA_HIP(hipStreamSynchronize(s));  // Xh is released on return
+        } else {
+            make_fp32();
+            run_tiles(dtype, Xf.get(), nullptr, nrm.get(), N, D, 0, kprime, nullptr, N, tr, mb, s);
+        }
         fp32_bound();
     }
     TKA_HIP(hipStreamSynchronize(s));  // rmax is on the host now
-    RerankArgs ra{Xn, n2.get(), N, D, K, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get(),
-                  eps_a, eps_b, rmax, idx, dist, tr.fail.get()};
+    RerankArgs ra{Xs, ns ? ns : n2.get(), N, D, K, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get(),
+                  eps_a, eps_b, rmax, idx, dist, tr.fail.get(), perm.get()};
     hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(N, 4)), dim3(256), 0, s, ra);
     TKA_LAUNCH_CHECK();
 
     // Rows whose bound failed: FP32 tiles with the widest candidate list.
     DevBuf<int> list(N, mb), nfail(1, mb);
     TKA_HIP(hipMemsetAsync(nfail.get(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), N, list.get(),
+    hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), N, perm.get(), list.get(),
                        nfail.get());
     TKA_LAUNCH_CHECK();
     int hf = 0;
@@ -889,11 +1408,11 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         const int kp2 = std::min(kMaxKprime, N - 1);
         run_tiles(TKA_DIST_FP32, Xf.get(), nullptr, nrm.get(), N, D, 0, kp2, list.get(), hf, tr2, mb, s);
         RerankArgs rb{Xn, n2.get(), N, D, K, list.get(), hf, tr2.buf.get(), tr2.cnt.get(), tr2.tau.get(),
-                      eps_a, eps_b, rmax, idx, dist, tr2.fail.get()};
+                      eps_a, eps_b, rmax, idx, dist, tr2.fail.get(), nullptr};
         hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(hf, 4)), dim3(256), 0, s, rb);
         TKA_LAUNCH_CHECK();
         TKA_HIP(hipMemsetAsync(nfail.get(), 0, sizeof(int), s));
-        hipLaunchKernelGGL(collect_failures, dim3(ceil_div(hf, 256)), dim3(256), 0, s, tr2.fail.get(), hf, list.get(),
+        hipLaunchKernelGGL(collect_failures, dim3(ceil_div(hf, 256)), dim3(256), 0, s, tr2.fail.get(), hf, nullptr, list.get(),
                            nfail.get());
         TKA_LAUNCH_CHECK();
         int hf2 = 0;
