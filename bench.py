@@ -7,11 +7,20 @@ mapped to 50-D by orthonormal rows plus N(0, 0.05^2) noise), perplexity 30
 
 A "step" is one iteration of the main loop (tree build, Barnes-Hut
 repulsion, CSR attraction, gains/momentum update), on P and Y resident in
-HBM; `value` is iterations per second.  The same JSON line also reports the
-kNN + P build time with BF16 and with FP32 distance tiles, the full
-1000-iteration run and its final KL against an FP64 run from the same P and
-Y0, the roofline of the dominant kernel (HIP-event timings on the engine's
-stream) and the CPU restatement of the reference timed on this host.
+HBM; `value` is iterations per second.  The same JSON line also reports:
+  knn_p_build   kNN + P build time with FP32 and with BF16 distance tiles
+                (device work; the host-to-device copy of X is reported apart)
+  accuracy      errors against FP64: kNN recall, max |dP|, gradient at a
+                fixed Y, final KL of the FP32 loop against the FP64 loop
+  pipeline      two full 1000-iteration runs (FP32 and FP64 loops) from the
+                same P and Y0, host arrays in and out
+  precision_sweep  kNN + P at N=250,000, D=128 (anisotropic Gaussian
+                mixture) with FP32, BF16 and FP8 e4m3 distance tiles
+  roofline      of the dominant kernel (HIP events on the engine's stream)
+  cpu_baseline  the FP64 CPU restatement of the reference on one host core:
+                loop iterations timed, the VP-tree kNN and the perplexity
+                calibration timed on a strided sample of the queries, the
+                symmetrisation timed whole; full-run time extrapolated.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
 """
@@ -31,12 +40,24 @@ PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 FLOPS_PER_INTERACTION = 14   # dx, dy, D, q = 1/(1+D), z += c q, m = c q^2, f += m d (2)
 
+# HBM bytes per launch of the traversal kernel, from two rocprofv3 --pmc
+# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of
+# `bench.py --steps 10 --warmup 2 --no-cpu --no-pipeline --no-sweep --copy-calibration 256`,
+# each scaled by what the counter reads for a plain 256 MiB device copy in the
+# same run (FETCH_SIZE reads 131084 KiB for 262144 KiB moved: x2.0;
+# WRITE_SIZE reads 262144 KiB: x1.0).  Summary in
+# profiles/pmc_traffic_bh_tsne_swissroll_n100k.md.
+TRAVERSE_FETCH_KIB = 4790.0
+TRAVERSE_WRITE_KIB = 2604.0
+FETCH_SCALE = 2.0
+WRITE_SCALE = 1.0
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(row, col, val, Y0, theta, budget_s=20.0, max_iters=40):
+def cpu_loop(row, col, val, Y0, theta, budget_s=10.0, max_iters=40):
     """The FP64 single-thread restatement of the reference loop (oracle),
     timed on this host: tree build + forces + update per iteration."""
     import numpy as np
@@ -57,6 +78,91 @@ def cpu_baseline(row, col, val, Y0, theta, budget_s=20.0, max_iters=40):
     return n / dt, n, dt
 
 
+def cpu_knn_p(X, perp, idx_full, stride=50):
+    """The reference's input-similarity stage restated (oracle), one core:
+    normalisation, VP-tree over all points with the searches of every
+    `stride`-th query, the perplexity bisection of those rows, and the
+    symmetrisation of the full neighbour graph (the engine's, which the
+    VP-tree's would only approximate).  Returns seconds extrapolated to all
+    N queries, and the parts."""
+    from oracle import oracle as O
+    N = X.shape[0]
+    K = idx_full.shape[1]
+    t = time.perf_counter()
+    Xn = O.normalize(X)
+    t_norm = time.perf_counter() - t
+    _, dist, t_build, t_search = O.knn_vptree_sample(Xn, K, stride)
+    nq = dist.shape[0]
+    t = time.perf_counter()
+    O.perplexity_sparse(dist, perp)
+    t_perp = time.perf_counter() - t
+    import numpy as np
+    vals = np.full(idx_full.shape, 1.0 / K)
+    t = time.perf_counter()
+    O.symmetrize(idx_full, vals)
+    t_sym = time.perf_counter() - t
+    scale = N / nq
+    total = t_norm + t_build + (t_search + t_perp) * scale + t_sym
+    parts = {"normalize_s": round(t_norm, 4), "vptree_build_s": round(t_build, 4),
+             "vptree_search_s_est": round(t_search * scale, 3), "perplexity_s_est": round(t_perp * scale, 3),
+             "symmetrize_s": round(t_sym, 4), "queries_timed": nq}
+    return total, parts
+
+
+def recall(idx, ref):
+    """Mean fraction of each row's reference neighbours found, and the
+    fraction of rows whose neighbour sets match exactly."""
+    import numpy as np
+    a = np.sort(idx, axis=1)
+    b = np.sort(ref, axis=1)
+    K = ref.shape[1]
+    hits = 0
+    for lo in range(0, a.shape[0], 8192):   # row-blocked set intersection
+        aa, bb = a[lo:lo + 8192], b[lo:lo + 8192]
+        pos = np.clip(np.array([np.searchsorted(r, s) for r, s in zip(bb, aa)]), 0, K - 1)
+        hits += int((np.take_along_axis(bb, pos, axis=1) == aa).sum())
+    exact = float((a == b).all(axis=1).mean())
+    return hits / ref.size, exact
+
+
+def max_dp(P1, P2):
+    """max |P1 - P2| over the union of two CSR patterns."""
+    import scipy.sparse as sp
+    n = len(P1[0]) - 1
+    A = sp.csr_matrix((P1[2], P1[1], P1[0]), shape=(n, n))
+    B = sp.csr_matrix((P2[2], P2[1], P2[0]), shape=(n, n))
+    d = (A - B).tocoo()
+    return float(abs(d.data).max()) if d.nnz else 0.0
+
+
+def precision_sweep(tapkee_amd, datasets, perp=30.0):
+    """kNN + P at N=250,000, D=128 with FP32, BF16 and FP8 e4m3 tiles: device
+    time of each stage (from a session's setup), recall@K and max |dP|
+    against the FP32 result, which the re-rank certificate makes exact."""
+    X4, _ = datasets.anisotropic_mixture()
+    X4 = X4.astype("float64")
+    N, D = X4.shape
+    K = int(3 * perp)
+    out = {"workload": f"anisotropic gaussian mixture N={N} D={D} perplexity={perp} K={K}"}
+    ref_idx = ref_p = None
+    for dt in ("fp32", "bf16", "fp8"):
+        s = tapkee_amd.Session(X4, perplexity=perp, theta=0.5, dist_dtype=dt, seed=1)
+        st = s.stats()
+        s.close()
+        idx, _, fails = tapkee_amd.knn(X4, K, dist_dtype=dt)
+        P = tapkee_amd.build_p(X4, perp, dist_dtype=dt)
+        if dt == "fp32":
+            ref_idx, ref_p = idx, P
+        r, exact = recall(idx, ref_idx)
+        flops = 2.0 * N * N * D
+        out[dt] = {"knn_ms": round(st["ms_knn"], 3), "p_ms": round(st["ms_perplexity"] + st["ms_symmetrize"], 3),
+                   "distance_tflops_effective": round(flops / (st["ms_knn"] * 1e-3) / 1e12, 2),
+                   "recall_at_k": r, "rows_exact": exact, "certificate_failures": fails,
+                   "max_abs_dP_vs_fp32": max_dp(P, ref_p)}
+        log(f"[bench] sweep {dt}: {out[dt]}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +171,11 @@ def main():
     ap.add_argument("--n", type=int, default=100000)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-pipeline", action="store_true", help="skip the full 1000-iteration runs")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the N=250k precision sweep")
+    ap.add_argument("--no-accuracy", action="store_true", help="skip the error-vs-FP64 measurements")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run 3 plain device-to-device copies of MIB MiB (a known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); this engine has no multi-GPU path",
@@ -76,9 +187,21 @@ def main():
     import tapkee_amd
     from tapkee_amd import datasets
 
+    if a.copy_calibration > 0:
+        import torch
+        nbytes = a.copy_calibration << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy calibration: 3 copies of {nbytes} bytes (read {nbytes}, write {nbytes} each)")
+        del src, dst
+
     N = a.n
     X, _ = datasets.swiss_roll_50d(n=N)
     perp, theta = 30.0, 0.5
+    K = int(3 * perp)
     log(f"[bench] workload: swiss roll N={N} D={X.shape[1]} perplexity={perp} theta={theta}")
 
     # ---- kNN + P build, BF16 vs FP32 distance tiles (setup of a session)
@@ -87,8 +210,9 @@ def main():
         s = tapkee_amd.Session(X, perplexity=perp, theta=theta, dist_dtype=dt, seed=1)
         st = s.stats()
         build[dt] = {"knn_ms": st["ms_knn"], "p_ms": st["ms_perplexity"] + st["ms_symmetrize"],
-                     "normalize_ms": st["ms_normalize"],
-                     "total_ms": st["ms_normalize"] + st["ms_knn"] + st["ms_perplexity"] + st["ms_symmetrize"],
+                     "normalize_ms": st["ms_normalize"] - st["ms_upload"], "upload_ms": st["ms_upload"],
+                     "total_ms": st["ms_normalize"] - st["ms_upload"] + st["ms_knn"] + st["ms_perplexity"]
+                     + st["ms_symmetrize"],
                      "certificate_failures": st["certificate_failures"], "nnz": st["nnz"]}
         log(f"[bench] build {dt}: {build[dt]}")
         if dt == "fp32":
@@ -96,20 +220,20 @@ def main():
     sess = s  # bf16 session
 
     # ---- timed steps (FP32 loop)
-    W, K = a.warmup, a.steps
+    W, K_steps = a.warmup, a.steps
     sess.step(W)
     sess.synchronize()
     t0 = time.perf_counter()
-    sess.step(K)
+    sess.step(K_steps)
     sess.synchronize()
     t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1e3 / K
-    value = K / (t1 - t0)
-    log(f"[bench] {K} steps: {ms_per_step:.4f} ms/step, {value:.1f} iters/s")
+    ms_per_step = (t1 - t0) * 1e3 / K_steps
+    value = K_steps / (t1 - t0)
+    log(f"[bench] {K_steps} steps: {ms_per_step:.4f} ms/step, {value:.1f} iters/s")
 
     # ---- per-kernel timing (HIP events on the engine stream)
     sess.profile(True)
-    sess.step(min(K, 100))
+    sess.step(min(K_steps, 100))
     kt = sess.kernel_timing()
     sess.profile(False)
     log(f"[bench] kernel timing: {kt}")
@@ -123,15 +247,42 @@ def main():
     # attraction: per edge col (4) + P (4) + y_col (8); per row y_i (8), row_ptr (4), pos (8)
     att_bytes = nnz * (4 + 4 + 8) + N * (8 + 4 + 8)
     att_gbs = att_bytes / (att_ms * 1e-3) / 1e9
+    traffic = int((TRAVERSE_FETCH_KIB * FETCH_SCALE + TRAVERSE_WRITE_KIB * WRITE_SCALE) * 1024)
     roofline = {"kernel": "bh_traverse", "bound": "mfma", "achieved": round(achieved_tflops, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 5),
-                "traffic": None, "interactions_per_launch": int(inter), "ms_per_launch": round(rep_ms, 5),
-                "note": "FP32 VALU work (14 flop per cell interaction) against the published FP32 peak"}
+                "traffic": traffic, "interactions_per_launch": int(inter), "ms_per_launch": round(rep_ms, 5),
+                "traffic_GBps": round(traffic / (rep_ms * 1e-3) / 1e9, 1),
+                "note": "FP32 VALU work (14 flop per cell interaction) against the published FP32 peak; "
+                        "the walk is bound by dependent node loads, not by HBM (traffic_GBps)"}
     kernels = {"tree_build_ms": round(build_ms, 5), "repulsion_ms": round(rep_ms, 5),
                "update_ms": round(upd_ms, 5),
                "attraction_ms": round(att_ms, 5), "attraction_concurrent_with": "tree build + traversal",
                "attraction_GBps": round(att_gbs, 1),
                "attraction_hbm_frac": round(att_gbs / PEAK_HBM_GBS, 4)}
+
+    # ---- accuracy against FP64 at this size
+    accuracy = None
+    Yfix = sess.embedding()
+    if not a.no_accuracy:
+        idx32, _, f32 = tapkee_amd.knn(X, K, dist_dtype="fp32")
+        idx16, _, f16 = tapkee_amd.knn(X, K, dist_dtype="bf16")
+        r16, e16 = recall(idx16, idx32)
+        P32 = tapkee_amd.build_p(X, perp, dist_dtype="fp32")
+        P16 = tapkee_amd.build_p(X, perp, dist_dtype="bf16")
+        row, col, val = P16
+        g32, z32 = tapkee_amd.gradient(Yfix, row, col, val, theta, precision="fp32")
+        g64, z64 = tapkee_amd.gradient(Yfix, row, col, val, theta, precision="fp64")
+        gex, zex = tapkee_amd.gradient(Yfix, row, col, val, 0.0, precision="fp64")
+        rel = lambda u, v: float(np.linalg.norm(u - v) / np.linalg.norm(v))  # noqa: E731
+        accuracy = {
+            "knn_recall_bf16_vs_exact": r16, "knn_rows_exact_bf16": e16,
+            "knn_certificate_failures": {"fp32": f32, "bf16": f16},
+            "max_abs_dP_bf16_vs_fp32": max_dp(P16, P32),
+            "gradient_rel_l2_fp32_vs_fp64": rel(g32, g64), "z_rel_fp32_vs_fp64": abs(z32 - z64) / z64,
+            "gradient_rel_l2_bh_vs_exact_fp64": rel(g64, gex),
+            "note": "FP32 distance tiles are certified exact (FP64 re-rank + error bound), so they are the "
+                    "FP64 reference for recall; gradients at the embedding after the timed steps, theta=0.5"}
+        log(f"[bench] accuracy: {accuracy}")
 
     # ---- full runs: 1000 iterations FP32 loop vs FP64 loop from the same P and Y0
     pipeline = None
@@ -147,37 +298,52 @@ def main():
             log(f"[bench] full run {prec}: {runs[prec]}")
         d_kl = (runs["fp32"]["kl_final_exact"] - runs["fp64"]["kl_final_exact"]) / runs["fp64"]["kl_final_exact"]
         pipeline = {"fp32_loop": runs["fp32"], "fp64_loop": runs["fp64"], "delta_kl_rel_vs_fp64": d_kl,
-                    "iters": 1000}
+                    "iters": 1000, "note": "wall_s: host X in, host Y out, kNN (BF16 tiles) + P + 1000 iterations"}
+        if accuracy is not None:
+            accuracy["kl_rel_fp32_loop_vs_fp64_loop"] = d_kl
 
-    # ---- CPU baseline: the reference loop restated in FP64, one core
+    sweep = None
+    if not a.no_sweep:
+        sweep = precision_sweep(tapkee_amd, datasets, perp)
+
+    # ---- CPU baseline: the reference restated in FP64, one core
     cpu = None
     if not a.no_cpu:
         row, col, val = tapkee_amd.build_p(X, perp, dist_dtype="bf16")
-        Ycpu = sess.embedding()
-        rate, n_it, dt = cpu_baseline(row, col, val, Ycpu, theta)
+        rate, n_it, dt = cpu_loop(row, col, val, Yfix, theta)
+        idx_full = tapkee_amd.knn(X, K, dist_dtype="bf16")[0]
+        knn_p_s, parts = cpu_knn_p(X, perp, idx_full)
+        full_s = knn_p_s + 1000.0 / rate
         cpu = {"value": round(rate, 4), "unit": "iters/s", "cores": 1, "kind": "port",
                "sample": f"{n_it} Barnes-Hut iterations (quadtree build + forces + update, FP64, one thread) "
-                         f"at N={N} on the engine's P, from the embedding after {W + K + min(K, 100)} "
-                         f"iterations; {dt:.1f} s"}
+                         f"at N={N} on the engine's P from the embedding after {W + K_steps + min(K_steps, 100)} "
+                         f"iterations ({dt:.1f} s); VP-tree over all {N} points with {parts['queries_timed']} "
+                         f"strided queries searched and calibrated, symmetrisation of the full graph",
+               "knn_p_s_est": round(knn_p_s, 3), "knn_p_parts": parts, "full_run_s_est": round(full_s, 2)}
         log(f"[bench] cpu baseline: {cpu}")
     sess.close()
 
     out = {
-        "metric": METRIC, "value": round(value, 3), "unit": "iters/s", "n_gpus": 1, "steps": K, "warmup": W,
+        "metric": METRIC, "value": round(value, 3), "unit": "iters/s", "n_gpus": 1, "steps": K_steps, "warmup": W,
         "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": "bh-tsne swiss-roll N=100000 D=50 perplexity=30 K=90 theta=0.5 2-D",
                    "n": N, "d": int(X.shape[1]), "perplexity": perp, "theta": theta, "knn_dtype": "bf16",
                    "loop_dtype": "fp32"},
         "knn_p_build_ms": {k: round(v["total_ms"], 3) for k, v in build.items()},
+        "knn_ms_fp32_over_bf16": round(build["fp32"]["knn_ms"] / build["bf16"]["knn_ms"], 3),
         "knn_p_build": build,
         "kernels": kernels,
         "roofline": roofline,
+        "accuracy": accuracy,
         "cpu_baseline": cpu,
         "pipeline": pipeline,
+        "precision_sweep": sweep,
     }
     if cpu:
         out["speedup_vs_cpu_iters"] = round(value / cpu["value"], 1)
+        if pipeline:
+            out["speedup_vs_cpu_full_run"] = round(cpu["full_run_s_est"] / pipeline["fp32_loop"]["wall_s"], 1)
     print(json.dumps(out), flush=True)
 
 

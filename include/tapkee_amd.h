@@ -40,7 +40,7 @@ typedef enum tka_status {
 
 /* Arithmetic of the kNN distance tiles (the re-rank is always FP64). */
 typedef enum tka_dist_dtype {
-    TKA_DIST_FP32 = 0, /* FP32 VALU tiles */
+    TKA_DIST_FP32 = 0, /* FP32 tiles (v_mfma_f32_32x32x2_f32), FP32 accumulate */
     TKA_DIST_BF16 = 1, /* BF16 MFMA tiles, FP32 accumulate */
     TKA_DIST_FP8 = 2   /* FP8 e4m3 MFMA tiles, per-row absmax scaling; approximate:
                           rows failing the error bound are counted, not re-solved */
@@ -118,6 +118,7 @@ typedef struct tka_tsne_stats {
     int32_t kl_iter[TKA_MAX_KL_TRACE];
     double kl_value[TKA_MAX_KL_TRACE];
     double kl_final_exact;       /* FP64 exact-Z KL of the final embedding */
+    double ms_upload;            /* host-to-device copy of X (included in ms_normalize) */
 } tka_tsne_stats;
 
 TKA_API void tka_default_options(tka_tsne_options* opt);

@@ -20,6 +20,7 @@ _lib = None
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = C.POINTER(C.c_double)
 
 
 def build(force: bool = False) -> str:
@@ -60,6 +61,8 @@ def lib():
     L.or_sqdist.argtypes = [_dp, C.c_int, C.c_int, _dp, C.c_int]
     L.or_knn_brute.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, _dp]
     L.or_vptree_knn.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, _dp]
+    L.or_vptree_knn_sample.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _dp, _f64p, _f64p]
+    L.or_vptree_knn_sample.restype = C.c_int
     L.or_perplexity_sparse.argtypes = [_dp, C.c_int, C.c_int, C.c_double, _dp, _dp, _ip]
     L.or_perplexity_sparse.restype = C.c_int
     L.or_perplexity_dense.argtypes = [_dp, C.c_int, C.c_double, _dp, _dp]
@@ -147,6 +150,19 @@ def knn_vptree(X, K):
     dist = np.empty((N, K))
     lib().or_vptree_knn(X, N, X.shape[1], K, idx, dist)
     return idx, dist
+
+
+def knn_vptree_sample(X, K, stride):
+    """VP-tree over all rows, searches of rows 0, stride, 2*stride, ... only.
+    Returns (idx, dist, seconds_build, seconds_search) for the timed sample."""
+    X = _f64(X)
+    N = X.shape[0]
+    nq = (N + stride - 1) // stride
+    idx = np.empty((nq, K), np.int32)
+    dist = np.empty((nq, K))
+    tb, ts = C.c_double(0), C.c_double(0)
+    lib().or_vptree_knn_sample(X, N, X.shape[1], K, stride, idx, dist, C.byref(tb), C.byref(ts))
+    return idx, dist, tb.value, ts.value
 
 
 def perplexity_sparse(dist, perplexity):

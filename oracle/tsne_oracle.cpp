@@ -29,6 +29,7 @@
 //   update                   scalar recurrence in numpy
 //   gaussian_random/rand     glibc rand() known-answer sequence (seed 1)
 #include <algorithm>
+#include <chrono>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -256,6 +257,45 @@ OR_API void or_vptree_knn(const double* X, int N, int D, int K, int* idx, double
         }
     }
     delete t.root;
+}
+
+// Timing sample of or_vptree_knn for the CPU baseline: the tree over all N
+// points, then the searches of queries 0, stride, 2*stride, ... only.  Rows of
+// idx/dist hold those queries in order.  Returns the number of queries;
+// *sec_build / *sec_search receive the wall time of each phase.
+OR_API int or_vptree_knn_sample(const double* X, int N, int D, int K, int stride, int* idx, double* dist,
+                                double* sec_build, double* sec_search)
+{
+    VpTree t;
+    t.X = X;
+    t.D = D;
+    t.items.resize(N);
+    for (int n = 0; n < N; n++) t.items[n] = n;
+    const auto t0 = std::chrono::steady_clock::now();
+    t.root = t.build(0, N);
+    const auto t1 = std::chrono::steady_clock::now();
+    std::vector<VpHeapItem> res;
+    int q = 0;
+    for (int n = 0; n < N; n += stride, q++) {
+        std::priority_queue<VpHeapItem> heap;
+        t.tau = DBL_MAX;
+        t.search(t.root, n, K + 1, heap);
+        res.clear();
+        while (!heap.empty()) {
+            res.push_back(heap.top());
+            heap.pop();
+        }
+        std::reverse(res.begin(), res.end());
+        for (int k = 0; k < K; k++) {
+            idx[(size_t)q * K + k] = t.items[res[k + 1].index];
+            dist[(size_t)q * K + k] = res[k + 1].dist;
+        }
+    }
+    const auto t2 = std::chrono::steady_clock::now();
+    *sec_build = std::chrono::duration<double>(t1 - t0).count();
+    *sec_search = std::chrono::duration<double>(t2 - t1).count();
+    delete t.root;
+    return q;
 }
 
 // ---------------------------------------------------------------------------

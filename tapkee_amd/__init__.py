@@ -250,7 +250,7 @@ def _stats_dict(st):
     n = min(st.kl_count, _lib.MAX_KL_TRACE)
     d = {k: getattr(st, k) for k in ("ms_normalize", "ms_knn", "ms_perplexity", "ms_symmetrize", "ms_loop",
                                      "ms_total", "nnz", "knn_k", "bisection_unconverged", "certificate_failures",
-                                     "kl_final_exact")}
+                                     "kl_final_exact", "ms_upload")}
     d["kl_iter"] = list(st.kl_iter[:n])
     d["kl"] = list(st.kl_value[:n])
     return d

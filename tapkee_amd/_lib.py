@@ -58,7 +58,7 @@ class Stats(C.Structure):
         ("nnz", C.c_int64), ("knn_k", C.c_int32), ("bisection_unconverged", C.c_int32),
         ("certificate_failures", C.c_int32), ("kl_count", C.c_int32),
         ("kl_iter", C.c_int32 * MAX_KL_TRACE), ("kl_value", C.c_double * MAX_KL_TRACE),
-        ("kl_final_exact", C.c_double),
+        ("kl_final_exact", C.c_double), ("ms_upload", C.c_double),
     ]
 
 

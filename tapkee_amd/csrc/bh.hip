@@ -479,8 +479,27 @@ TKA_DEV int bh_visit(const BhNode<T>& nd, int t, uint32_t p, T px, T py, T theta
     return t + 1;
 }
 
+// Node t of a block of four consecutive nodes held in registers (a register
+// array indexed by a per-lane value would go to scratch memory).
+template <class T>
+TKA_DEV BhNode<T> pick4(const BhNode<T> (&b)[4], int k)
+{
+    const BhNode<T> lo = (k & 1) ? b[1] : b[0];
+    const BhNode<T> hi = (k & 1) ? b[3] : b[2];
+    return (k & 2) ? hi : lo;
+}
+
 // One thread per point, in Morton order, so the lanes of a wave walk similar
 // paths and their node gathers mostly hit the same cache lines.
+//
+// The walk is a chain of dependent loads; two things shorten it:
+//  - nodes are fetched in aligned blocks of four (one cache line of FP32
+//    nodes): descending and leaf steps go to t+1, which the block usually
+//    holds, so only jumps past accepted cells and block ends wait on memory;
+//  - workgroups are dispatched to the 8 XCDs round-robin, so workgroup b is
+//    given the points of logical block x*q + min(x, r) + k (x = b % 8,
+//    k = b / 8, grid = 8q + r): every XCD walks one contiguous Morton range,
+//    whose cells are a small part of the tree, and its L2 holds them.
 template <class T>
 __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__ nodes,
                                                    const uint32_t* __restrict__ pre,
@@ -491,7 +510,9 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
                                                    unsigned long long* __restrict__ interactions)
 {
     __shared__ double sh[4];
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int nb = (int)gridDim.x, xcd = (int)blockIdx.x & 7, kx = (int)blockIdx.x >> 3;
+    const int lb = xcd * (nb >> 3) + min(xcd, nb & 7) + kx;
+    const int j = lb * 256 + threadIdx.x;
     const bool active = j < N;
     double zd = 0.0;
     unsigned cnt = 0;
@@ -505,8 +526,19 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
     }
     T fx = 0, fy = 0, z = 0;
     if (active) {
-        int t = 0;
-        while (t < total) t = bh_visit<T>(nodes[t], t, p, px, py, theta2, size0sq, fx, fy, z, cnt);
+        BhNode<T> blk[4];
+        int t = 0, base = -4;
+        while (t < total) {
+            if ((t & ~3) != base) {  // the node buffer is padded to whole blocks
+                base = t & ~3;
+                const BhNode<T>* b4 = nodes + base;
+                blk[0] = b4[0];
+                blk[1] = b4[1];
+                blk[2] = b4[2];
+                blk[3] = b4[3];
+            }
+            t = bh_visit<T>(pick4(blk, t & 3), t, p, px, py, theta2, size0sq, fx, fy, z, cnt);
+        }
     }
     if (active) {
         neg[2 * (size_t)p] = fx;
@@ -514,7 +546,7 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
         zd = (double)z;
     }
     zd = block_sum(zd, sh);
-    if (threadIdx.x == 0) zpart[blockIdx.x] = zd;
+    if (threadIdx.x == 0) zpart[lb] = zd;
     if (interactions != nullptr) {
         const double c = block_sum((double)cnt, sh);
         if (threadIdx.x == 0) atomicAdd(interactions, (unsigned long long)c);
@@ -546,7 +578,7 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     sx.alloc(n + 1, mb);
     sy.alloc(n + 1, mb);
     part.alloc(3 * (size_t)ceil_div(n, kTreeScanTile), mb);
-    nodes.alloc(2 * (size_t)n, mb);
+    nodes.alloc(2 * (size_t)n + 4, mb);  // whole blocks of four for the traversal's loads
     root.alloc(1, mb);
 }
 

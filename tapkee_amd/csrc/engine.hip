@@ -553,6 +553,8 @@ tka_session* create_session(const double* Xh, int N, int D, int no_dims, double 
 
     DevBuf<double> X;  // normalise (tsne.hpp:88-89)
     upload(X, Xh, (size_t)N * D, s, mb);
+    c.stream.sync();
+    st.ms_upload = now_ms() - t0;
     normalize_device(X.get(), N, D, mb, s);
     c.stream.sync();
     const double t1 = now_ms();
