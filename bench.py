@@ -16,6 +16,8 @@ HBM; `value` is iterations per second.  The same JSON line also reports:
                 same P and Y0, host arrays in and out
   precision_sweep  kNN + P at N=250,000, D=128 (anisotropic Gaussian
                 mixture) with FP32, BF16 and FP8 e4m3 distance tiles
+  parametric    N=10,000, D=784: a 784-256-256-2 MLP trained by Adam on the
+                engine's KL autograd function vs non-parametric t-SNE
   roofline      of the dominant kernel (HIP events on the engine's stream)
   cpu_baseline  the FP64 CPU restatement of the reference on one host core:
                 loop iterations timed, the VP-tree kNN and the perplexity
@@ -163,6 +165,34 @@ def precision_sweep(tapkee_amd, datasets, perp=30.0):
     return out
 
 
+def parametric_leg(tapkee_amd, datasets, perp=30.0, steps=500):
+    """Workload 5: a 784-256-256-2 FP32 MLP trained by full-batch Adam
+    (lr 1e-3, 500 steps) on KL(P||Q) through the engine's autograd function
+    (exact Z), against the non-parametric Barnes-Hut t-SNE of the same data;
+    both final embeddings scored with the exact FP64 KL."""
+    import numpy as np
+    import torch
+
+    from tapkee_amd import parametric
+    X5, _ = datasets.autograd_mixture()
+    X5 = X5.astype(np.float64)
+    net, hist = parametric.train(X5, perplexity=perp, theta=0.0, steps=steps, lr=1e-3, seed=0)
+    Xt = torch.tensor((X5 - X5.mean(0)) / np.abs(X5 - X5.mean(0)).max(), dtype=torch.float32, device="cuda")
+    with torch.no_grad():
+        Ym = net(Xt).double().cpu().numpy()
+    row, col, val = tapkee_amd.build_p(X5, perp)
+    kl_mlp = tapkee_amd.kl(Ym, row, col, val, mode="exact")
+    Yt, st = tapkee_amd.tsne(X5, perplexity=perp, theta=0.5, seed=1)
+    kl_tsne = tapkee_amd.kl(Yt, row, col, val, mode="exact")
+    out = {"workload": f"gaussian mixture N={X5.shape[0]} D={X5.shape[1]} perplexity={perp}",
+           "mlp": "784-256-256-2 ReLU, FP32, Adam lr 1e-3, full batch", "steps": steps,
+           "mlp_train_s": round(hist["seconds"], 3), "ms_per_step": round(hist["seconds"] * 1e3 / steps, 4),
+           "kl_mlp_exact": kl_mlp, "kl_tsne_exact": kl_tsne, "kl_ratio_mlp_over_tsne": kl_mlp / kl_tsne,
+           "kl_trace": [round(v, 4) for v in hist["kl"]]}
+    log(f"[bench] parametric: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +203,7 @@ def main():
     ap.add_argument("--no-pipeline", action="store_true", help="skip the full 1000-iteration runs")
     ap.add_argument("--no-sweep", action="store_true", help="skip the N=250k precision sweep")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error-vs-FP64 measurements")
+    ap.add_argument("--no-parametric", action="store_true", help="skip the parametric (MLP) t-SNE leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run 3 plain device-to-device copies of MIB MiB (a known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -183,12 +214,12 @@ def main():
         sys.exit(2)
 
     import numpy as np
+    import torch  # before the engine library: both then use one HIP runtime
 
     import tapkee_amd
     from tapkee_amd import datasets
 
     if a.copy_calibration > 0:
-        import torch
         nbytes = a.copy_calibration << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
@@ -305,6 +336,9 @@ def main():
     sweep = None
     if not a.no_sweep:
         sweep = precision_sweep(tapkee_amd, datasets, perp)
+    param = None
+    if not a.no_parametric:
+        param = parametric_leg(tapkee_amd, datasets, perp)
 
     # ---- CPU baseline: the reference restated in FP64, one core
     cpu = None
@@ -339,6 +373,7 @@ def main():
         "cpu_baseline": cpu,
         "pipeline": pipeline,
         "precision_sweep": sweep,
+        "parametric": param,
     }
     if cpu:
         out["speedup_vs_cpu_iters"] = round(value / cpu["value"], 1)

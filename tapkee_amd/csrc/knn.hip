@@ -405,18 +405,21 @@ TKA_DEV void filter_tile(const float (&v)[16], int t, int cb, int q, int slot_lo
 // threshold being T itself (a dropped entry then equals the threshold, which
 // the re-rank's strict certificate treats as possibly closer: such a row is
 // re-solved if it matters).
+// Wave w owns the ROWS row-slots ROWS*w .. ROWS*w + ROWS-1 of the block
+// (ROWS = 32 or 64: one or two 32-query tiles).
+template <int ROWS = 32>
 TKA_DEV void compact_if_needed(int w, const TileArgs& a, int slot0, int* s_cnt, float* s_tau, int* hist)
 {
     const int lane = threadIdx.x & 63;
     wave_lds_sync();
-    const int my = 32 * w + (lane & 31);
-    const bool need = lane < 32 && s_cnt[my] > kCap - kCB;
+    const int my = ROWS * w + (lane % ROWS);
+    const bool need = lane < ROWS && s_cnt[my] > kCap - kCB;
     uint64_t mask = __ballot(need);
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     while (mask) {
         const int l = __ffsll((long long)mask) - 1;
         mask &= mask - 1;
-        const int sl = 32 * w + l;
+        const int sl = ROWS * w + l;
         const int c = s_cnt[sl];
         uint64_t* qbuf = a.buf + (size_t)(slot0 + sl) * kCap;
         uint64_t v[8];
@@ -726,6 +729,28 @@ struct OpF32 {  // four k-steps of v_mfma_f32_32x32x2_f32 per vector
     }
 };
 
+// Order in which a block streams the nch candidate chunks.  With rows in
+// locality order, chunk s holds the block's own queries: the block first
+// takes the seeds s-1, s, s+1, s+2 (its nearest rows, so the first
+// compaction already sets tight thresholds), then every other chunk in
+// ascending order.  That second part is the same for all blocks, so the
+// blocks running together on an XCD read the same chunks at about the same
+// time and the XCD's L2 serves them; streams that each started at their own
+// chunk would all miss.  s < 0: plain ascending order.
+struct ChunkOrder {
+    int nch, nseed, a;  // seeds: a, a+1, ..., a+nseed-1 (mod nch)
+    TKA_DEV ChunkOrder(int s, int n) : nch(n), nseed(s < 0 ? 0 : min(4, n)), a(s < 0 ? 0 : (s - 1 + n) % n) {}
+    // c-th chunk of the stream, 0 <= c < nch
+    TKA_DEV int at(int c) const
+    {
+        if (c < nseed) return (a + c) % nch;
+        const int k = c - nseed;            // k-th chunk that is not a seed, ascending
+        const int wrap = a + nseed - nch;   // seeds that wrapped to 0, 1, ...
+        if (wrap > 0) return wrap + k;      // non-seeds are wrap .. a-1
+        return k < a ? k : k + nseed;
+    }
+};
+
 typedef __attribute__((address_space(3))) void lds_void;
 
 // Copies `bytes` (a multiple of 1024) from global src to LDS dst, 16 bytes
@@ -740,81 +765,114 @@ TKA_DEV void dma_to_lds(const unsigned char* src, unsigned char* dst, int bytes)
 // kMode 0: the value is the accumulator (BF16: ||c||^2 folded into the operands)
 //       1: ||c||^2 - 2 acc (FP32)
 //       2: ||c||^2 - 2 s_q s_c acc (FP8 with per-row scales)
-template <class Op, int kMode, int kNVMax>
+// QT: 32-query tiles per wave (block = 4 waves x 32 QT query rows).  With
+// QT = 2 every candidate fragment read from LDS feeds two MFMAs, and every
+// chunk copied into LDS serves 256 queries: half the LDS and L2 traffic per
+// distance of QT = 1.
+template <class Op, int kMode, int NV, int QT>
 __global__ __launch_bounds__(256) void knn_tiles_reg(const typename Op::V* __restrict__ A,
-                                                     const typename Op::V* __restrict__ B, int NV, TileArgs a)
+                                                     const typename Op::V* __restrict__ B, TileArgs a)
 {
     typedef typename Op::V V;
+    constexpr int QB = 128 * QT;  // query rows per block
     extern __shared__ __align__(16) unsigned char dyn_lds[];  // two chunks of kCB rows x (2*NV + 1) vectors
-    __shared__ int s_cnt[kQB];
-    __shared__ float s_tau[kQB];
+    __shared__ int s_cnt[QB];
+    __shared__ float s_tau[QB];
     __shared__ int s_hist[4][256];  // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
-    const int slot0 = blockIdx.x * kQB;
-    const int sl = 32 * w + (lane & 31);
-    const int q = query_row(a, slot0 + sl);
+    const int slot0 = blockIdx.x * QB;
     const int RS = 2 * NV + 1;
     const int chunk_bytes = kCB * RS * (int)sizeof(V);
-    if (tid < kQB) {
-        s_cnt[tid] = 0;
-        s_tau[tid] = HUGE_VALF;
+    for (int i = tid; i < QB; i += 256) {
+        s_cnt[i] = 0;
+        s_tau[i] = HUGE_VALF;
     }
-    V bq[kNVMax];
-    float sq = 0.f;
-    {
-        const V* qr = B + (size_t)(q >= 0 ? q : 0) * RS;
+    int sl[QT], q[QT];
+    V bq[QT][NV];
+    float sq[QT];
+    uint64_t* qbuf[QT];
 #pragma unroll
-        for (int g = 0; g < kNVMax; g++) bq[g] = (g < NV && q >= 0) ? qr[half * NV + g] : V{};
-        if (kMode == 2 && q >= 0) sq = reinterpret_cast<const float*>(qr + 2 * NV)[1];
+    for (int u = 0; u < QT; u++) {
+        sl[u] = 32 * QT * w + 32 * u + (lane & 31);
+        q[u] = query_row(a, slot0 + sl[u]);
+        const V* qr = B + (size_t)(q[u] >= 0 ? q[u] : 0) * RS;
+#pragma unroll
+        for (int g = 0; g < NV; g++) bq[u][g] = (q[u] >= 0) ? qr[half * NV + g] : V{};
+        sq[u] = (kMode == 2 && q[u] >= 0) ? reinterpret_cast<const float*>(qr + 2 * NV)[1] : 0.f;
+        qbuf[u] = a.buf + (size_t)(slot0 + sl[u]) * kCap;
     }
-    uint64_t* qbuf = a.buf + (size_t)(slot0 + sl) * kCap;
     const unsigned char* src = reinterpret_cast<const unsigned char*>(A);
     const int nch = (a.N + kCB - 1) / kCB;
-    const int c0 = a.rotate ? (int)blockIdx.x % nch : 0;
-    dma_to_lds(src + (size_t)c0 * kCB * RS * sizeof(V), dyn_lds, chunk_bytes);
+    // the chunk holding the block's first query rows (rows are in locality order)
+    const ChunkOrder order(a.rotate ? (int)(((size_t)blockIdx.x * QB / kCB) % nch) : -1, nch);
+    dma_to_lds(src + (size_t)order.at(0) * kCB * RS * sizeof(V), dyn_lds, chunk_bytes);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int c = 0; c < nch; c++) {
-        const int cb = ((c0 + c) % nch) * kCB;
+        const int cb = order.at(c) * kCB;
         unsigned char* cur = dyn_lds + (c & 1) * chunk_bytes;
         if (c + 1 < nch)
-            dma_to_lds(src + (size_t)(((c0 + c + 1) % nch) * kCB) * RS * sizeof(V), dyn_lds + ((c + 1) & 1) * chunk_bytes,
+            dma_to_lds(src + (size_t)order.at(c + 1) * kCB * RS * sizeof(V), dyn_lds + ((c + 1) & 1) * chunk_bytes,
                        chunk_bytes);
         const V* sA = reinterpret_cast<const V*>(cur);
-        f32x16 acc[4];
+        f32x16 acc[QT][4];
 #pragma unroll
-        for (int t = 0; t < 4; t++) acc[t] = f32x16{};
+        for (int u = 0; u < QT; u++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[u][t] = f32x16{};
         const V* rowp = sA + (lane & 31) * RS + half * NV;
+        // k-step g+1's four candidate fragments are read from LDS before
+        // step g's MFMAs issue, so the reads complete under them (the
+        // scheduling barriers keep the compiler from pairing each read with
+        // its MFMA behind a full wait)
+        V af[4], an[4];
 #pragma unroll
-        for (int g = 0; g < kNVMax; g++) {
-            if (g < NV) {
+        for (int t = 0; t < 4; t++) af[t] = rowp[32 * t * RS];
 #pragma unroll
-                for (int t = 0; t < 4; t++) acc[t] = Op::mma(rowp[32 * t * RS + g], bq[g], acc[t]);
+        for (int g = 0; g < NV; g++) {
+            if (g + 1 < NV) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) an[t] = rowp[32 * t * RS + g + 1];
             }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int u = 0; u < QT; u++)
+#pragma unroll
+                for (int t = 0; t < 4; t++) acc[u][t] = Op::mma(af[t], bq[u][g], acc[u][t]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < 4; t++) af[t] = an[t];
         }
-        const float tau = s_tau[sl];
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            float v[16];
+        for (int u = 0; u < QT; u++) {
+            const float tau = s_tau[sl[u]];
 #pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int m = 32 * t + 8 * (i >> 2) + 4 * half + (i & 3);
-                if (kMode == 0) {
-                    v[i] = acc[t][i];
-                } else {
-                    const float* pad = reinterpret_cast<const float*>(sA + m * RS + 2 * NV);
-                    v[i] = kMode == 1 ? fmaf(-2.f, acc[t][i], pad[0]) : fmaf(-2.f * sq * pad[1], acc[t][i], pad[0]);
+            for (int t = 0; t < 4; t++) {
+                float v[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const int m = 32 * t + 8 * (i >> 2) + 4 * half + (i & 3);
+                    if (kMode == 0) {
+                        v[i] = acc[u][t][i];
+                    } else {
+                        const float* pad = reinterpret_cast<const float*>(sA + m * RS + 2 * NV);
+                        v[i] = kMode == 1 ? fmaf(-2.f, acc[u][t][i], pad[0])
+                                          : fmaf(-2.f * sq[u] * pad[1], acc[u][t][i], pad[0]);
+                    }
                 }
+                filter_tile(v, t, cb, q[u], sl[u], tau, a, s_cnt, qbuf[u]);
             }
-            filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
         }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau, s_hist[w]);
+        compact_if_needed<32 * QT>(w, a, slot0, s_cnt, s_tau, s_hist[w]);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's copy of the next chunk has landed
         __syncthreads();                // every wave's has, and the current chunk is free
     }
-    if (lane < 32 && slot0 + sl < a.nq) {
-        a.cnt_out[slot0 + sl] = s_cnt[sl];
-        a.tau_out[slot0 + sl] = s_tau[sl];
+#pragma unroll
+    for (int u = 0; u < QT; u++) {
+        if (lane < 32 && slot0 + sl[u] < a.nq) {
+            a.cnt_out[slot0 + sl[u]] = s_cnt[sl[u]];
+            a.tau_out[slot0 + sl[u]] = s_tau[sl[u]];
+        }
     }
 }
 
@@ -887,7 +945,7 @@ __global__ __launch_bounds__(256) void prep_fp32_hm(const double* __restrict__ X
 // (vector (h, g) = 8 bytes holding k = 16 g + 8 h + 0..7); the pad vector
 // holds the floats {||x||^2, scale}.
 __global__ __launch_bounds__(256) void fp8_to_hm(const uint8_t* __restrict__ src, const float* __restrict__ nrm,
-                                                 const float* __restrict__ scale, int N, int Npad, int NV,
+                                                 const float* __restrict__ scale, int N, int Npad, int NVsrc, int NV,
                                                  uint64_t* __restrict__ dst)
 {
     const int RS = 2 * NV + 1;
@@ -897,8 +955,8 @@ __global__ __launch_bounds__(256) void fp8_to_hm(const uint8_t* __restrict__ src
         uint64_t v = 0;
         if (n < N) {
             if (vi < 2 * NV) {
-                const int h = vi / NV, g = vi - h * NV;
-                v = *reinterpret_cast<const uint64_t*>(src + (size_t)n * 16 * NV + 16 * g + 8 * h);
+                const int h = vi / NV, g = vi - h * NV;  // source rows: NVsrc vectors per half
+                if (g < NVsrc) v = *reinterpret_cast<const uint64_t*>(src + (size_t)n * 16 * NVsrc + 16 * g + 8 * h);
             } else {
                 v = ((uint64_t)__float_as_uint(scale[n]) << 32) | __float_as_uint(nrm[n]);
             }
@@ -1201,33 +1259,39 @@ void run_tiles(int dtype, const void* op_a, const void* op_b, const float* nrm, 
     TKA_LAUNCH_CHECK();
 }
 
-// All rows against all rows with the query operand in registers
-// (NV <= kNVLimit; the register arrays are sized for the next multiple of 4).
+// All rows against all rows with the query operand in registers.  NV
+// (vectors per half row, 1..kNVLimit) is a template constant, so the k-loop
+// is fully unrolled and the LDS reads of step g+1 are scheduled under the
+// MFMAs of step g.  QT: query tiles per wave (see knn_tiles_reg).
 // A (candidates), B (queries): padded half-major rows for ceil(N / kCB) * kCB rows.
-template <class Op, int kMode>
+template <class Op, int kMode, int NV, int QT>
+void launch_tiles_reg(const void* A, const void* B, dim3 grid, size_t lds, const TileArgs& ta, hipStream_t s)
+{
+    typedef typename Op::V V;
+    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, NV, QT>), grid, dim3(256), lds, s, (const V*)A, (const V*)B, ta);
+}
+
+template <class Op, int kMode, int QT>
 void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, TileRun& tr, MemBudget* mb, hipStream_t s)
 {
     typedef typename Op::V V;
-    require(NV >= 1 && NV <= kNVLimit, TKA_E_INTERNAL, "register tiles: operand too wide");
-    const int nslots = (int)ceil_div(N, kQB) * kQB;
+    require(NV >= 1 && NV <= kNVLimit, TKA_E_INTERNAL, "register tiles: bad operand width");
+    constexpr int QB = 128 * QT;
+    const int nslots = (int)ceil_div(N, QB) * QB;
     tr.buf.alloc((size_t)nslots * kCap, mb);
     tr.cnt.alloc(nslots, mb);
     tr.tau.alloc(nslots, mb);
     tr.fail.alloc(nslots, mb);
     TileArgs ta{N, kprime, 1, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
     const size_t lds = 2 * (size_t)kCB * (2 * NV + 1) * sizeof(V);
-    const dim3 grid(ceil_div(N, kQB));
-#define TKA_REG(W) \
-    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, W>), grid, dim3(256), lds, s, (const V*)A, (const V*)B, NV, ta)
-    if (NV <= 4)
-        TKA_REG(4);
-    else if (NV <= 8)
-        TKA_REG(8);
-    else if (NV <= 12)
-        TKA_REG(12);
-    else
-        TKA_REG(16);
-#undef TKA_REG
+    const dim3 grid(ceil_div(N, QB));
+#define TKA_W(W) \
+    case W: launch_tiles_reg<Op, kMode, W, QT>(A, B, grid, lds, ta, s); break
+    switch (NV) {
+        TKA_W(1); TKA_W(2); TKA_W(3); TKA_W(4); TKA_W(5); TKA_W(6); TKA_W(7); TKA_W(8);
+        TKA_W(9); TKA_W(10); TKA_W(11); TKA_W(12); TKA_W(13); TKA_W(14); TKA_W(15); TKA_W(16);
+    }
+#undef TKA_W
     TKA_LAUNCH_CHECK();
 }
 
@@ -1330,7 +1394,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
             B.alloc(rows, mb);
             hipLaunchKernelGGL(prep_bf16x3_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, A.get(), B.get());
             TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpBF16, 0>(A.get(), B.get(), NV, N, kprime, tr, mb, s);
+            run_tiles_reg<OpBF16, 0, 2>(A.get(), B.get(), NV, N, kprime, tr, mb, s);
         } else {
             A.alloc((size_t)N * Kp, mb);
             B.alloc((size_t)N * Kp, mb);
@@ -1355,9 +1419,9 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         if (NV <= kNVLimit) {
             DevBuf<uint64_t> A8h((size_t)Npad * (2 * NV + 1), mb);
             hipLaunchKernelGGL(fp8_to_hm, dim3(gprep), dim3(256), 0, s, A8.get(), nrm8.get(), scale8.get(), N, Npad,
-                               NV, A8h.get());
+                               Kp / 16, NV, A8h.get());
             TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpFP8, 2>(A8h.get(), A8h.get(), NV, N, kprime, tr, mb, s);
+            run_tiles_reg<OpFP8, 2, 1>(A8h.get(), A8h.get(), NV, N, kprime, tr, mb, s);
             TKA_HIP(hipStreamSynchronize(s));  // A8h is released on return
         } else {
             run_tiles(dtype, A8.get(), scale8.get(), nrm8.get(), N, D, Kp, kprime, nullptr, N, tr, mb, s);
@@ -1374,7 +1438,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
             DevBuf<float> Xh((size_t)Npad * 4 * (2 * NV + 1), mb);
             hipLaunchKernelGGL(prep_fp32_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, Xh.get());
             TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpF32, 1>(Xh.get(), Xh.get(), NV, N, kprime, tr, mb, s);
+            run_tiles_reg<OpF32, 1, 1>(Xh.get(), Xh.get(), NV, N, kprime, tr, mb, s);
             TKA_HIP(hipStreamSynchronize(s));  // Xh is released on return
         } else {
             make_fp32();

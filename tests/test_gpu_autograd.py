@@ -77,3 +77,31 @@ def test_autograd_drives_an_optimiser(engine):
         opt.step()
         first = loss.item() if first is None else first
     assert loss.item() < 0.5 * first
+
+
+def test_gradient_at_workload5_size(engine, oracle):
+    """N=10,000, D=784 mixture (workload 5): the engine's FP64 exact
+    gradient against the oracle's, against central differences of the exact
+    KL (h=1e-6), and the Barnes-Hut gradient against the oracle's pointer
+    quadtree at theta=0.5."""
+    from tapkee_amd import datasets
+    X, _ = datasets.autograd_mixture()
+    X = X.astype(np.float64)
+    N = X.shape[0]
+    row, col, val = engine.build_p(X, 30.0)
+    Y = rng(3).normal(size=(N, 2)) * 3.0
+    Y -= Y.mean(0)
+    dY, z = engine.gradient(Y, row, col, val, 0.0, precision="fp64")
+    ref, zr = oracle.gradient_exact_sparse(row, col, val, Y)
+    assert np.linalg.norm(dY - ref) / np.linalg.norm(ref) < 1e-10
+    assert abs(z - zr) / zr < 1e-12
+    h = 1e-6
+    for (i, d) in [(0, 0), (4321, 1), (9999, 0)]:
+        Yp, Ym = Y.copy(), Y.copy()
+        Yp[i, d] += h
+        Ym[i, d] -= h
+        fd = (engine.kl(Yp, row, col, val, mode="exact") - engine.kl(Ym, row, col, val, mode="exact")) / (2 * h)
+        assert abs(fd - 4 * dY[i, d]) < 1e-6 * max(1.0, abs(fd)), (i, d, fd, 4 * dY[i, d])
+    gb, _ = engine.gradient(Y, row, col, val, 0.5, precision="fp64")
+    rb, _ = oracle.gradient_bh(row, col, val, Y, 0.5)
+    assert np.linalg.norm(gb - rb) / np.linalg.norm(rb) < 1e-9
