@@ -287,7 +287,7 @@ def main():
                         "the walk is bound by dependent node loads, not by HBM (traffic_GBps)"}
     kernels = {"tree_build_ms": round(build_ms, 5), "repulsion_ms": round(rep_ms, 5),
                "update_ms": round(upd_ms, 5),
-               "attraction_ms": round(att_ms, 5), "attraction_concurrent_with": "tree build + traversal",
+               "attraction_ms": round(att_ms, 5), "attraction_concurrent_with": "tree build (sort and node kernels)",
                "attraction_GBps": round(att_gbs, 1),
                "attraction_hbm_frac": round(att_gbs / PEAK_HBM_GBS, 4)}
 

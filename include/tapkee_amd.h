@@ -156,7 +156,7 @@ typedef struct tka_kernel_timing {
     int32_t n;
     int32_t dims;
     int64_t launches_attraction; /* Barnes-Hut loop: attraction passes timed (second stream,
-                                    concurrent with the tree build and traversal) */
+                                    concurrent with the tree's sort and node kernels) */
     double ms_attraction;
 } tka_kernel_timing;
 

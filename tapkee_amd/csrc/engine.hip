@@ -277,8 +277,8 @@ struct Session : tka_session {
     DevBuf<uint32_t> rn_scratch;
 
     // Barnes-Hut loop: the attraction pass runs on a second stream beside
-    // the tree build and traversal.  ev_y marks "Y centred and final for this
-    // iteration"; the update waits for ev_attr.
+    // the tree's sort and node kernels.  ev_y marks "Y centred and final for
+    // this iteration"; the update waits for ev_attr.
     hipStream_t s2 = nullptr;
     hipEvent_t ev_y = nullptr, ev_attr = nullptr;
 
@@ -383,12 +383,13 @@ struct Session : tka_session {
                     renumber(tree.sorted_perm);  // order of the previous iteration's tree
                 if (profile) timers[kTimBuild].begin(s);
                 bh_build_keys<T>(Ycur, N, L.box.get(), nbox, tree, s);
+                // Y is final after the key kernel: the (HBM-bound) attraction
+                // runs beside the sort and node kernels, which occupy few
+                // CUs; beside the traversal it would slow the traversal's
+                // node loads by about its own duration
+                launch_attraction();
                 bh_build_tree<T>(Ycur, N, tree, s);
                 if (profile) timers[kTimBuild].end(s);
-                // Y is final after the key kernel; starting the attraction
-                // here overlaps it with the (latency-bound) traversal rather
-                // than with the sort
-                launch_attraction();
                 if (profile) timers[kTimRepulsion].begin(s);
                 bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(),
                                 profile ? interactions.get() : nullptr, s);

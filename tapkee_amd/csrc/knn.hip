@@ -370,14 +370,24 @@ TKA_DEV float pick16(const float (&v)[16], int i)
 // passes; otherwise one LDS atomic reserves the lane's slots and a loop over
 // the lane's passing values stores them (a uniform loop over all 16 values
 // with predicated stores measured slower).
+TKA_DEV void filter_append(const float (&v)[16], int t, int cb, int q, int slot_local, float tau, const TileArgs& a,
+                           int* s_cnt, uint64_t* qbuf);
+
 TKA_DEV void filter_tile(const float (&v)[16], int t, int cb, int q, int slot_local, float tau, const TileArgs& a,
                          int* s_cnt, uint64_t* qbuf)
 {
-    const int half = (threadIdx.x & 63) >> 5;
     float mn = v[0];
 #pragma unroll
     for (int i = 1; i < 16; i++) mn = fminf(mn, v[i]);
     if (!(mn < tau) || q < 0) return;
+    filter_append(v, t, cb, q, slot_local, tau, a, s_cnt, qbuf);
+}
+
+// The appends of filter_tile, for a tile known to hold a value below tau.
+TKA_DEV void filter_append(const float (&v)[16], int t, int cb, int q, int slot_local, float tau, const TileArgs& a,
+                           int* s_cnt, uint64_t* qbuf)
+{
+    const int half = (threadIdx.x & 63) >> 5;
     const int base = cb + 32 * t + 4 * half;  // candidate of value i: base + 8*(i/4) + i%4
     const bool edge = base + 27 >= a.N || (q >= base && q <= base + 27);
     uint32_t m = 0;
@@ -754,36 +764,38 @@ struct ChunkOrder {
 typedef __attribute__((address_space(3))) void lds_void;
 
 // Copies `bytes` (a multiple of 1024) from global src to LDS dst, 16 bytes
-// per lane, each wave instruction filling one contiguous KiB.
-TKA_DEV void dma_to_lds(const unsigned char* src, unsigned char* dst, int bytes)
+// per lane, each wave instruction filling one contiguous KiB; the block's
+// `waves` waves share the copy.
+TKA_DEV void dma_to_lds(const unsigned char* src, unsigned char* dst, int bytes, int waves)
 {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int e = w; e < bytes / 1024; e += 4)
+    for (int e = w; e < bytes / 1024; e += waves)
         __builtin_amdgcn_global_load_lds(src + (size_t)e * 1024 + 16 * lane, (lds_void*)(dst + e * 1024), 16, 0, 0);
 }
 
 // kMode 0: the value is the accumulator (BF16: ||c||^2 folded into the operands)
 //       1: ||c||^2 - 2 acc (FP32)
 //       2: ||c||^2 - 2 s_q s_c acc (FP8 with per-row scales)
-// QT: 32-query tiles per wave (block = 4 waves x 32 QT query rows).  With
-// QT = 2 every candidate fragment read from LDS feeds two MFMAs, and every
-// chunk copied into LDS serves 256 queries: half the LDS and L2 traffic per
-// distance of QT = 1.
-template <class Op, int kMode, int NV, int QT>
-__global__ __launch_bounds__(256) void knn_tiles_reg(const typename Op::V* __restrict__ A,
-                                                     const typename Op::V* __restrict__ B, TileArgs a)
+// QT: 32-query tiles per wave, WAVES: waves per block (block = 32 WAVES QT
+// query rows).  With QT = 2 every candidate fragment read from LDS feeds two
+// MFMAs; every chunk copied into LDS serves all the block's queries.  Two
+// waves per SIMD (WAVES = 8, or two blocks per CU) let one wave's filter run
+// under another's MFMAs.
+template <class Op, int kMode, int NV, int QT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V* __restrict__ A,
+                                                            const typename Op::V* __restrict__ B, TileArgs a)
 {
     typedef typename Op::V V;
-    constexpr int QB = 128 * QT;  // query rows per block
+    constexpr int QB = 32 * WAVES * QT;  // query rows per block
     extern __shared__ __align__(16) unsigned char dyn_lds[];  // two chunks of kCB rows x (2*NV + 1) vectors
     __shared__ int s_cnt[QB];
     __shared__ float s_tau[QB];
-    __shared__ int s_hist[4][256];  // per-wave selection histogram
+    __shared__ int s_hist[WAVES][256];  // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * QB;
     const int RS = 2 * NV + 1;
     const int chunk_bytes = kCB * RS * (int)sizeof(V);
-    for (int i = tid; i < QB; i += 256) {
+    for (int i = tid; i < QB; i += 64 * WAVES) {
         s_cnt[i] = 0;
         s_tau[i] = HUGE_VALF;
     }
@@ -805,7 +817,7 @@ __global__ __launch_bounds__(256) void knn_tiles_reg(const typename Op::V* __res
     const int nch = (a.N + kCB - 1) / kCB;
     // the chunk holding the block's first query rows (rows are in locality order)
     const ChunkOrder order(a.rotate ? (int)(((size_t)blockIdx.x * QB / kCB) % nch) : -1, nch);
-    dma_to_lds(src + (size_t)order.at(0) * kCB * RS * sizeof(V), dyn_lds, chunk_bytes);
+    dma_to_lds(src + (size_t)order.at(0) * kCB * RS * sizeof(V), dyn_lds, chunk_bytes, WAVES);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int c = 0; c < nch; c++) {
@@ -813,7 +825,7 @@ __global__ __launch_bounds__(256) void knn_tiles_reg(const typename Op::V* __res
         unsigned char* cur = dyn_lds + (c & 1) * chunk_bytes;
         if (c + 1 < nch)
             dma_to_lds(src + (size_t)order.at(c + 1) * kCB * RS * sizeof(V), dyn_lds + ((c + 1) & 1) * chunk_bytes,
-                       chunk_bytes);
+                       chunk_bytes, WAVES);
         const V* sA = reinterpret_cast<const V*>(cur);
         f32x16 acc[QT][4];
 #pragma unroll
@@ -919,25 +931,33 @@ __global__ __launch_bounds__(256) void prep_bf16x3_hm(const double* __restrict__
     }
 }
 
-// Padded half-major FP32 operand: S k-steps of 2 per row (S = 4 NV);
-// position h*S + s holds k = 2 s + h; the pad vector is {||x||^2, 0, 0, 0}.
-__global__ __launch_bounds__(256) void prep_fp32_hm(const double* __restrict__ X, const double* __restrict__ n2, int N,
-                                                    int Npad, int D, int NV, float* __restrict__ Xh)
+// Padded half-major FP32 operands with the norm folded in, so the MFMA
+// chain yields ||c||^2 - 2 q.c: candidates A = [c | ||c||^2], queries
+// B = [-2q | 1] (x -2 is exact), D+1 k-values.  S = 4 NV k-steps of 2 per
+// half row; position h*S + s holds k = 2 s + h; the pad vector and the rows
+// past N are zero.
+__global__ __launch_bounds__(256) void prep_fp32n_hm(const double* __restrict__ X, const double* __restrict__ n2,
+                                                     int N, int Npad, int D, int NV, float* __restrict__ A,
+                                                     float* __restrict__ B)
 {
     const int S = 4 * NV, RSe = 4 * (2 * NV + 1);
     const size_t total = (size_t)Npad * RSe;
     for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
         const int n = (int)(e / RSe), p = (int)(e % RSe);
-        float v = 0.f;
-        if (n < N) {
-            if (p < 2 * S) {
-                const int h = p / S, k = 2 * (p - h * S) + h;
-                v = k < D ? (float)X[(size_t)n * D + k] : 0.f;
-            } else if (p == 2 * S) {
-                v = (float)n2[n];
+        float av = 0.f, bv = 0.f;
+        if (n < N && p < 2 * S) {
+            const int h = p / S, k = 2 * (p - h * S) + h;
+            if (k < D) {
+                const float x = (float)X[(size_t)n * D + k];
+                av = x;
+                bv = -2.f * x;
+            } else if (k == D) {
+                av = (float)n2[n];
+                bv = 1.f;
             }
         }
-        Xh[e] = v;
+        A[e] = av;
+        B[e] = bv;
     }
 }
 
@@ -1264,19 +1284,20 @@ void run_tiles(int dtype, const void* op_a, const void* op_b, const float* nrm, 
 // is fully unrolled and the LDS reads of step g+1 are scheduled under the
 // MFMAs of step g.  QT: query tiles per wave (see knn_tiles_reg).
 // A (candidates), B (queries): padded half-major rows for ceil(N / kCB) * kCB rows.
-template <class Op, int kMode, int NV, int QT>
+template <class Op, int kMode, int NV, int QT, int WAVES>
 void launch_tiles_reg(const void* A, const void* B, dim3 grid, size_t lds, const TileArgs& ta, hipStream_t s)
 {
     typedef typename Op::V V;
-    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, NV, QT>), grid, dim3(256), lds, s, (const V*)A, (const V*)B, ta);
+    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, NV, QT, WAVES>), grid, dim3(64 * WAVES), lds, s, (const V*)A,
+                       (const V*)B, ta);
 }
 
-template <class Op, int kMode, int QT>
+template <class Op, int kMode, int QT, int WAVES>
 void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, TileRun& tr, MemBudget* mb, hipStream_t s)
 {
     typedef typename Op::V V;
     require(NV >= 1 && NV <= kNVLimit, TKA_E_INTERNAL, "register tiles: bad operand width");
-    constexpr int QB = 128 * QT;
+    constexpr int QB = 32 * WAVES * QT;
     const int nslots = (int)ceil_div(N, QB) * QB;
     tr.buf.alloc((size_t)nslots * kCap, mb);
     tr.cnt.alloc(nslots, mb);
@@ -1286,7 +1307,7 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, Tile
     const size_t lds = 2 * (size_t)kCB * (2 * NV + 1) * sizeof(V);
     const dim3 grid(ceil_div(N, QB));
 #define TKA_W(W) \
-    case W: launch_tiles_reg<Op, kMode, W, QT>(A, B, grid, lds, ta, s); break
+    case W: launch_tiles_reg<Op, kMode, W, QT, WAVES>(A, B, grid, lds, ta, s); break
     switch (NV) {
         TKA_W(1); TKA_W(2); TKA_W(3); TKA_W(4); TKA_W(5); TKA_W(6); TKA_W(7); TKA_W(8);
         TKA_W(9); TKA_W(10); TKA_W(11); TKA_W(12); TKA_W(13); TKA_W(14); TKA_W(15); TKA_W(16);
@@ -1394,7 +1415,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
             B.alloc(rows, mb);
             hipLaunchKernelGGL(prep_bf16x3_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, A.get(), B.get());
             TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpBF16, 0, 2>(A.get(), B.get(), NV, N, kprime, tr, mb, s);
+            run_tiles_reg<OpBF16, 0, 1, 8>(A.get(), B.get(), NV, N, kprime, tr, mb, s);
         } else {
             A.alloc((size_t)N * Kp, mb);
             B.alloc((size_t)N * Kp, mb);
@@ -1421,7 +1442,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
             hipLaunchKernelGGL(fp8_to_hm, dim3(gprep), dim3(256), 0, s, A8.get(), nrm8.get(), scale8.get(), N, Npad,
                                Kp / 16, NV, A8h.get());
             TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpFP8, 2, 1>(A8h.get(), A8h.get(), NV, N, kprime, tr, mb, s);
+            run_tiles_reg<OpFP8, 2, 1, 4>(A8h.get(), A8h.get(), NV, N, kprime, tr, mb, s);
             TKA_HIP(hipStreamSynchronize(s));  // A8h is released on return
         } else {
             run_tiles(dtype, A8.get(), scale8.get(), nrm8.get(), N, D, Kp, kprime, nullptr, N, tr, mb, s);
@@ -1432,19 +1453,25 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         eps_a = 2.0 * (2.0 * (2 * u8 + u8 * u8) + 2.0 * D * u32);
         eps_b = 2.0 * (2.0 * u32);
     } else {
-        const int NV = ((D + 1) / 2 + 3) / 4;  // float4 vectors of k-steps (2 k per step) per half row
+        // float4 vectors of k-steps (2 k per step) per half row, D+1 k-values
+        const int NV = ((D + 2) / 2 + 3) / 4;
         if (NV <= kNVLimit) {
             locality();
-            DevBuf<float> Xh((size_t)Npad * 4 * (2 * NV + 1), mb);
-            hipLaunchKernelGGL(prep_fp32_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, Xh.get());
+            DevBuf<float> Ah((size_t)Npad * 4 * (2 * NV + 1), mb), Bh((size_t)Npad * 4 * (2 * NV + 1), mb);
+            hipLaunchKernelGGL(prep_fp32n_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, Ah.get(),
+                               Bh.get());
             TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpF32, 1, 1>(Xh.get(), Xh.get(), NV, N, kprime, tr, mb, s);
-            TKA_HIP(hipStreamSynchronize(s));  // Xh is released on return
+            run_tiles_reg<OpF32, 0, 1, 8>(Ah.get(), Bh.get(), NV, N, kprime, tr, mb, s);
+            TKA_HIP(hipStreamSynchronize(s));  // Ah, Bh are released on return
+            // the norm is one more term of the FP32 chain: its rounding is
+            // bounded like the products' (D+1 terms plus the conversions)
+            eps_a = 2.0 * (2.0 * (D + 3) * u32);
+            eps_b = eps_a;
         } else {
             make_fp32();
             run_tiles(dtype, Xf.get(), nullptr, nrm.get(), N, D, 0, kprime, nullptr, N, tr, mb, s);
+            fp32_bound();
         }
-        fp32_bound();
     }
     TKA_HIP(hipStreamSynchronize(s));  // rmax is on the host now
     RerankArgs ra{Xs, ns ? ns : n2.get(), N, D, K, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get(),

@@ -104,10 +104,23 @@ TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems],
     }
     if (tile == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
     __syncthreads();
+    // Keys that arrive nearly sorted (CSR edges, Morton keys of points kept
+    // in Morton order) give whole waves one digit: those add once, from one
+    // lane, instead of 64 atomics serialised on a single bin.
+    const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++)
-        if (ok[k])
-            for (int p = 0; p < passes; p++) atomicAdd(&h[p][(uint32_t)(key[k] >> (8 * p)) & 255u], 1u);
+    for (int k = 0; k < kRsItems; k++) {
+        const uint64_t act = __ballot(ok[k]);
+        for (int p = 0; p < passes; p++) {
+            const uint32_t d = (uint32_t)(key[k] >> (8 * p)) & 255u;
+            const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+            if (__ballot(ok[k] && d == d0) == act) {
+                if (act != 0 && lane == __ffsll((long long)act) - 1) atomicAdd(&h[p][d0], (uint32_t)__popcll(act));
+            } else if (ok[k]) {
+                atomicAdd(&h[p][d], 1u);
+            }
+        }
+    }
     __syncthreads();
     for (int p = 0; p < passes; p++) {
         const uint32_t c = h[p][t];
