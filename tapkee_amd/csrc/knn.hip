@@ -718,7 +718,16 @@ __global__ __launch_bounds__(256) void prep_fp8(const double* __restrict__ X, co
 // copy of chunk c+1 running while chunk c is multiplied and filtered.  The
 // odd row stride (2*NV + 1 vectors) spreads consecutive rows over the LDS
 // banks.
-constexpr int kNVLimit = 16;  // widest operand row (vectors per half) kept in registers
+constexpr int kNVLimit = 20;  // widest operand row (vectors per half) kept in registers
+
+// The register tiles' double-buffered chunks plus the kernel's own LDS
+// (row counters, thresholds and 8 selection histograms) must fit the 160 KiB
+// of a CU.
+inline bool reg_tiles_fit(int NV, size_t vec_bytes)
+{
+    const size_t dyn = 2 * (size_t)kCB * (2 * NV + 1) * vec_bytes;
+    return NV <= kNVLimit && dyn + 256 * 8 + 8 * 256 * 4 <= 160 * 1024;
+}
 
 struct OpBF16 {  // v_mfma_f32_32x32x16_bf16: one k-step of 16 per vector pair
     typedef bf16x8 V;
@@ -1309,8 +1318,9 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, Tile
 #define TKA_W(W) \
     case W: launch_tiles_reg<Op, kMode, W, QT, WAVES>(A, B, grid, lds, ta, s); break
     switch (NV) {
-        TKA_W(1); TKA_W(2); TKA_W(3); TKA_W(4); TKA_W(5); TKA_W(6); TKA_W(7); TKA_W(8);
-        TKA_W(9); TKA_W(10); TKA_W(11); TKA_W(12); TKA_W(13); TKA_W(14); TKA_W(15); TKA_W(16);
+        TKA_W(1); TKA_W(2); TKA_W(3); TKA_W(4); TKA_W(5); TKA_W(6); TKA_W(7); TKA_W(8); TKA_W(9); TKA_W(10);
+        TKA_W(11); TKA_W(12); TKA_W(13); TKA_W(14); TKA_W(15); TKA_W(16); TKA_W(17); TKA_W(18); TKA_W(19);
+        TKA_W(20);
     }
 #undef TKA_W
     TKA_LAUNCH_CHECK();
@@ -1408,7 +1418,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     if (dtype == TKA_DIST_BF16) {
         Kp = ((3 * D + 2) + 15) / 16 * 16;
         const int NV = Kp / 16;
-        if (NV <= kNVLimit) {
+        if (reg_tiles_fit(NV, sizeof(bf16x8))) {
             locality();
             const size_t rows = (size_t)Npad * 8 * (2 * NV + 1);
             A.alloc(rows, mb);
@@ -1430,14 +1440,15 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     } else if (dtype == TKA_DIST_FP8) {
         Kp = (D + 15) / 16 * 16;
         const int NV = Kp / 16;
-        if (NV <= kNVLimit) locality();
+        const bool reg = reg_tiles_fit(NV, sizeof(uint64_t));
+        if (reg) locality();
         A8.alloc((size_t)N * Kp, mb);
         scale8.alloc(N, mb);
         nrm8.alloc(N, mb);
         hipLaunchKernelGGL(prep_fp8, dim3(ceil_div(N, 256)), dim3(256), 0, s, Xs, ns ? ns : n2.get(), N, D, Kp,
                            A8.get(), scale8.get(), nrm8.get());
         TKA_LAUNCH_CHECK();
-        if (NV <= kNVLimit) {
+        if (reg) {
             DevBuf<uint64_t> A8h((size_t)Npad * (2 * NV + 1), mb);
             hipLaunchKernelGGL(fp8_to_hm, dim3(gprep), dim3(256), 0, s, A8.get(), nrm8.get(), scale8.get(), N, Npad,
                                Kp / 16, NV, A8h.get());
@@ -1455,7 +1466,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     } else {
         // float4 vectors of k-steps (2 k per step) per half row, D+1 k-values
         const int NV = ((D + 2) / 2 + 3) / 4;
-        if (NV <= kNVLimit) {
+        if (reg_tiles_fit(NV, sizeof(float4))) {
             locality();
             DevBuf<float> Ah((size_t)Npad * 4 * (2 * NV + 1), mb), Bh((size_t)Npad * 4 * (2 * NV + 1), mb);
             hipLaunchKernelGGL(prep_fp32n_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, Ah.get(),
