@@ -28,6 +28,8 @@ namespace {
 
 constexpr int kCap = 512;      // candidate slots per row
 constexpr int kCB = 128;       // candidates per LDS chunk
+constexpr int kSeedChunks = 4; // chunks a block streams first, stored whole (seed_store)
+static_assert(kSeedChunks * kCB <= kCap, "the seed chunks must fit the row buffer");
 constexpr int kQB = 128;       // query rows per block (4 waves x 32)
 constexpr int kMaxKprime = kCap - kCB;
 
@@ -290,57 +292,32 @@ TKA_DEV int wave_incl_scan_i(int v)
 // entries u[r] at position r*64 + lane (empty entries are ~0): the smallest
 // T, to 8*rounds leading bits (the remaining low bits of T set), with at
 // least kth entries whose key is <= T; all ones if there are fewer than kth
-// entries.  With rounds = 4, T is exactly the kth smallest key.  Radix
-// select on 8-bit digits with a 256-bin histogram in the wave's LDS (hist).
+// entries.  With rounds = 4, T is exactly the kth smallest key.  Bisection
+// from the top bit: each step counts the keys at or below a probe with one
+// compare and ballot per register.  (Candidate distances share their
+// leading bits, so a digit histogram in LDS would serialise its atomics on
+// one or two bins.)
 template <int R>
-TKA_DEV uint32_t wave_select(const uint64_t (&u)[R], int kth, int rounds, int* hist)
+TKA_DEV uint32_t wave_select(const uint64_t (&u)[R], int kth, int rounds)
 {
-    const int lane = threadIdx.x & 63;
-    uint32_t prefix = 0;
-    int below = 0;  // entries in bins below the selected ones so far
-    for (int rd = 0; rd < rounds; rd++) {
-        const int sh = 24 - 8 * rd;
-        const uint32_t hmask = rd == 0 ? 0u : (0xffffffffu << (sh + 8));
+    uint64_t valid[R];
+    int total = 0;
 #pragma unroll
-        for (int b = 0; b < 4; b++) hist[4 * lane + b] = 0;
-        wave_lds_sync();
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const uint32_t key = (uint32_t)(u[r] >> 32);
-            if (u[r] != ~0ull && (key & hmask) == prefix) atomicAdd(&hist[(key >> sh) & 255u], 1);
-        }
-        wave_lds_sync();
-        int h[4];
-#pragma unroll
-        for (int b = 0; b < 4; b++) h[b] = hist[4 * lane + b];
-        const int s = h[0] + h[1] + h[2] + h[3];
-        const int incl = wave_incl_scan_i(s), excl = incl - s;
-        const int need = kth - below;
-        const uint64_t owner = __ballot(excl < need && incl >= need);
-        if (owner == 0) return 0xffffffffu;  // fewer than kth entries
-        const int ol = __ffsll((long long)owner) - 1;
-        int bin = 0, before = 0;
-        if (lane == ol) {
-            // first of the lane's bins at which the running count reaches need
-            int c = excl, b = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (b == k && c + h[k] < need) {
-                    c += h[k];
-                    b = k + 1;
-                }
-            }
-            bin = 4 * lane + b;
-            before = c;
-        }
-        bin = __shfl(bin, ol, 64);
-        before = __shfl(before, ol, 64);
-        below += before;
-        prefix |= (uint32_t)bin << sh;
-        wave_lds_sync();
+    for (int r = 0; r < R; r++) {
+        valid[r] = __ballot(u[r] != ~0ull);
+        total += __popcll(valid[r]);
     }
+    if (total < kth) return 0xffffffffu;
     const int low = 32 - 8 * rounds;
-    return low >= 32 ? 0xffffffffu : (prefix | (low ? ((1u << low) - 1u) : 0u));
+    uint32_t res = 0;
+    for (int b = 31; b >= low; b--) {
+        const uint32_t probe = res | ((1u << b) - 1u);  // bit b clear, every lower bit set
+        int c = 0;
+#pragma unroll
+        for (int r = 0; r < R; r++) c += __popcll(__ballot((uint32_t)(u[r] >> 32) <= probe) & valid[r]);
+        if (c < kth) res |= 1u << b;
+    }
+    return low == 0 ? res : (res | ((1u << low) - 1u));
 }
 
 // v[i] for a per-lane index i, by a tree of bit-mask merges (register arrays
@@ -407,6 +384,30 @@ TKA_DEV void filter_append(const float (&v)[16], int t, int cb, int q, int slot_
     }
 }
 
+// Seed chunks (the first ones a block streams, while the row threshold is
+// still unset): every value is kept, so the lane's 16 values of tile t are
+// stored at their fixed places in the row buffer -- chunk c of the seed
+// phase fills slots c*kCB .. c*kCB + kCB-1 in candidate order -- with 32-byte
+// vector stores instead of appends.  The query itself and rows past N are
+// stored as empty entries (~0), which the compaction and the re-rank skip.
+TKA_DEV void seed_store(const float (&v)[16], int t, int cb, int q, int c, const TileArgs& a, uint64_t* qbuf)
+{
+    const int half = (threadIdx.x & 63) >> 5;
+    const int off = 32 * t + 4 * half;  // candidate of value i: cb + off + 8*(i/4) + i%4
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        uint64_t e[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int j = cb + off + 8 * g + k;
+            e[k] = (j < a.N && j != q) ? (((uint64_t)f2ord(v[4 * g + k]) << 32) | (uint32_t)j) : ~0ull;
+        }
+        ulonglong2* dst = reinterpret_cast<ulonglong2*>(qbuf + c * kCB + off + 8 * g);
+        dst[0] = make_ulonglong2(e[0], e[1]);
+        dst[1] = make_ulonglong2(e[2], e[3]);
+    }
+}
+
 // After a chunk: every row of this wave whose buffer could overflow on the
 // next chunk keeps only its entries up to a selected threshold (>= kprime of
 // them, 16-bit resolution), which becomes the row's new threshold.  If that
@@ -418,7 +419,7 @@ TKA_DEV void filter_append(const float (&v)[16], int t, int cb, int q, int slot_
 // Wave w owns the ROWS row-slots ROWS*w .. ROWS*w + ROWS-1 of the block
 // (ROWS = 32 or 64: one or two 32-query tiles).
 template <int ROWS = 32>
-TKA_DEV void compact_if_needed(int w, const TileArgs& a, int slot0, int* s_cnt, float* s_tau, int* hist)
+TKA_DEV void compact_if_needed(int w, const TileArgs& a, int slot0, int* s_cnt, float* s_tau)
 {
     const int lane = threadIdx.x & 63;
     wave_lds_sync();
@@ -439,13 +440,13 @@ TKA_DEV void compact_if_needed(int w, const TileArgs& a, int slot0, int* s_cnt, 
             v[r] = p < c ? qbuf[p] : ~0ull;
         }
         const int kp = a.kprime;
-        uint32_t T = wave_select<8>(v, kp, 2, hist);
+        uint32_t T = wave_select<8>(v, kp, 2);
         int kept = 0;
 #pragma unroll
         for (int r = 0; r < 8; r++) kept += __popcll(__ballot(v[r] != ~0ull && (uint32_t)(v[r] >> 32) <= T));
         bool exact = false;
         if (kept > kCap - kCB) {
-            T = wave_select<8>(v, kp, 4, hist);
+            T = wave_select<8>(v, kp, 4);
             exact = true;
         }
         // keep key < T, and key == T (all of them, or with an exact T up to
@@ -487,7 +488,6 @@ __global__ __launch_bounds__(256) void knn_tiles_fp32(const float* __restrict__ 
     __shared__ float sN[kCB];
     __shared__ int s_cnt[kQB];
     __shared__ float s_tau[kQB];
-    __shared__ int s_hist[4][256];      // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * kQB;
     const int sl = 32 * w + (lane & 31);
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(256) void knn_tiles_fp32(const float* __restrict__ 
             }
             filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
         }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau, s_hist[w]);
+        compact_if_needed(w, a, slot0, s_cnt, s_tau);
     }
     if (lane < 32 && slot0 + sl < a.nq) {
         a.cnt_out[slot0 + sl] = s_cnt[sl];
@@ -559,7 +559,6 @@ __global__ __launch_bounds__(256) void knn_tiles_bf16(const __bf16* __restrict__
     __shared__ bf16x8 sA[kCB][KV + 1];      // +1 vector of padding per row
     __shared__ int s_cnt[kQB];
     __shared__ float s_tau[kQB];
-    __shared__ int s_hist[4][256];      // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * kQB;
     const int sl = 32 * w + (lane & 31);
@@ -604,7 +603,7 @@ __global__ __launch_bounds__(256) void knn_tiles_bf16(const __bf16* __restrict__
             for (int i = 0; i < 16; i++) v[i] = acc[t][i];
             filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
         }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau, s_hist[w]);
+        compact_if_needed(w, a, slot0, s_cnt, s_tau);
     }
     if (lane < 32 && slot0 + sl < a.nq) {
         a.cnt_out[slot0 + sl] = s_cnt[sl];
@@ -624,7 +623,6 @@ __global__ __launch_bounds__(256) void knn_tiles_fp8(const uint8_t* __restrict__
     __shared__ float sN[kCB], sS[kCB];
     __shared__ int s_cnt[kQB];
     __shared__ float s_tau[kQB];
-    __shared__ int s_hist[4][256];      // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * kQB;
     const int sl = 32 * w + (lane & 31);
@@ -676,7 +674,7 @@ __global__ __launch_bounds__(256) void knn_tiles_fp8(const uint8_t* __restrict__
             }
             filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
         }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau, s_hist[w]);
+        compact_if_needed(w, a, slot0, s_cnt, s_tau);
     }
     if (lane < 32 && slot0 + sl < a.nq) {
         a.cnt_out[slot0 + sl] = s_cnt[sl];
@@ -721,12 +719,11 @@ __global__ __launch_bounds__(256) void prep_fp8(const double* __restrict__ X, co
 constexpr int kNVLimit = 20;  // widest operand row (vectors per half) kept in registers
 
 // The register tiles' double-buffered chunks plus the kernel's own LDS
-// (row counters, thresholds and 8 selection histograms) must fit the 160 KiB
-// of a CU.
+// (row counters and thresholds) must fit the 160 KiB of a CU.
 inline bool reg_tiles_fit(int NV, size_t vec_bytes)
 {
     const size_t dyn = 2 * (size_t)kCB * (2 * NV + 1) * vec_bytes;
-    return NV <= kNVLimit && dyn + 256 * 8 + 8 * 256 * 4 <= 160 * 1024;
+    return NV <= kNVLimit && dyn + 256 * 8 <= 160 * 1024;
 }
 
 struct OpBF16 {  // v_mfma_f32_32x32x16_bf16: one k-step of 16 per vector pair
@@ -758,7 +755,7 @@ struct OpF32 {  // four k-steps of v_mfma_f32_32x32x2_f32 per vector
 // chunk would all miss.  s < 0: plain ascending order.
 struct ChunkOrder {
     int nch, nseed, a;  // seeds: a, a+1, ..., a+nseed-1 (mod nch)
-    TKA_DEV ChunkOrder(int s, int n) : nch(n), nseed(s < 0 ? 0 : min(4, n)), a(s < 0 ? 0 : (s - 1 + n) % n) {}
+    TKA_DEV ChunkOrder(int s, int n) : nch(n), nseed(s < 0 ? 0 : min(kSeedChunks, n)), a(s < 0 ? 0 : (s - 1 + n) % n) {}
     // c-th chunk of the stream, 0 <= c < nch
     TKA_DEV int at(int c) const
     {
@@ -799,7 +796,6 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
     extern __shared__ __align__(16) unsigned char dyn_lds[];  // two chunks of kCB rows x (2*NV + 1) vectors
     __shared__ int s_cnt[QB];
     __shared__ float s_tau[QB];
-    __shared__ int s_hist[WAVES][256];  // per-wave selection histogram
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * QB;
     const int RS = 2 * NV + 1;
@@ -864,6 +860,7 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
 #pragma unroll
             for (int t = 0; t < 4; t++) af[t] = an[t];
         }
+        const bool seeding = c < order.nseed;
 #pragma unroll
         for (int u = 0; u < QT; u++) {
             const float tau = s_tau[sl[u]];
@@ -881,10 +878,19 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
                                           : fmaf(-2.f * sq[u] * pad[1], acc[u][t][i], pad[0]);
                     }
                 }
-                filter_tile(v, t, cb, q[u], sl[u], tau, a, s_cnt, qbuf[u]);
+                if (seeding) {
+                    if (q[u] >= 0) seed_store(v, t, cb, q[u], c, a, qbuf[u]);
+                } else {
+                    filter_tile(v, t, cb, q[u], sl[u], tau, a, s_cnt, qbuf[u]);
+                }
             }
         }
-        compact_if_needed<32 * QT>(w, a, slot0, s_cnt, s_tau, s_hist[w]);
+        if (c == order.nseed - 1 && lane < 32) {
+#pragma unroll
+            for (int u = 0; u < QT; u++)  // the seed chunks' slots (rows past N stored nothing)
+                if (q[u] >= 0) s_cnt[sl[u]] = order.nseed * kCB;
+        }
+        if (c >= order.nseed - 1) compact_if_needed<32 * QT>(w, a, slot0, s_cnt, s_tau);
         __builtin_amdgcn_s_waitcnt(0);  // this wave's copy of the next chunk has landed
         __syncthreads();                // every wave's has, and the current chunk is free
     }
@@ -1073,7 +1079,6 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     const double eps = a.eps_a * sqrt(qn2) * a.rmax + a.eps_b * a.rmax * a.rmax + 1e-12 * (qn2 + a.rmax * a.rmax);
     const int K = a.K;
 
-    __shared__ int s_hist[4][256];
     __shared__ uint64_t s_list[4][kCap];
     const int w = threadIdx.x >> 6;
     uint64_t u[8];
@@ -1083,7 +1088,7 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
         u[r] = p < c ? qbuf[p] : ~0ull;
     }
     // v_(K) exactly (radix select), then the candidates within thr, packed
-    const uint32_t vk_bits = wave_select<8>(u, K, 4, s_hist[w]);
+    const uint32_t vk_bits = wave_select<8>(u, K, 4);
     const double thr = c >= K ? (double)ord2f(vk_bits) + 2.0 * eps : HUGE_VAL;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int m = 0;
