@@ -568,7 +568,8 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     k64.alloc(n, mb);
     p2.alloc(n, mb);
     sort_scratch.alloc(radix_sort_scratch_words(n), mb);
-    TKA_HIP(hipMemset(sort_scratch.get(), 0, sort_scratch.size() * sizeof(uint32_t)));
+    // on the stream the buffer was allocated on (ordered after the allocation)
+    TKA_HIP(hipMemsetAsync(sort_scratch.get(), 0, sort_scratch.size() * sizeof(uint32_t), alloc_stream()));
     mask.alloc(n, mb);
     node_a.alloc(n, mb);
     node_b.alloc(n, mb);

@@ -43,6 +43,34 @@ inline void require(bool ok, tka_status s, const std::string& m)
     if (!ok) fail(s, m);
 }
 
+// Device memory comes from a process-wide cache of blocks (engine.hip):
+// freeing a block records an event on its allocation stream and keeps the
+// block; an allocation of the same size class on the same device reuses a
+// cached block whose event has completed, else calls hipMalloc.  A hipFree of
+// a large block costs hundreds of microseconds (and synchronises the device);
+// the setup of a t-SNE run allocates and frees sort buffers of hundreds of
+// MB, so repeated runs in one process would pay that on every phase.
+void* dev_alloc(size_t bytes, size_t* block_bytes, int* device);
+void dev_free(void* p, size_t block_bytes, int device, hipStream_t stream) noexcept;
+
+// Stream whose work a DevBuf's block must outlive (its free records the
+// reuse event there): the null stream -- ordered after all work on the
+// engine's blocking streams -- unless an entry point that works on a
+// caller's stream, which may be non-blocking, names that stream for its
+// scope.
+inline hipStream_t& alloc_stream()
+{
+    static thread_local hipStream_t s = nullptr;
+    return s;
+}
+struct AllocStream {
+    hipStream_t prev;
+    explicit AllocStream(hipStream_t s) : prev(alloc_stream()) { alloc_stream() = s; }
+    ~AllocStream() { alloc_stream() = prev; }
+    AllocStream(const AllocStream&) = delete;
+    AllocStream& operator=(const AllocStream&) = delete;
+};
+
 // Owning device allocation.  Allocation size is tracked against an optional
 // cap so tests can force the out-of-memory path from the host.
 struct MemBudget {
@@ -74,7 +102,8 @@ class DevBuf {
         if (b && b->cap > 0 && b->used + bytes > b->cap)
             fail(TKA_E_NOT_ENOUGH_MEMORY, "device allocation of " + std::to_string(bytes) +
                                               " bytes exceeds the configured cap");
-        TKA_HIP(hipMalloc((void**)&p_, (size_t)bytes));
+        stream_ = alloc_stream();
+        p_ = static_cast<T*>(dev_alloc((size_t)bytes, &block_, &dev_));
         n_ = n;
         budget_ = b;
         bytes_ = bytes;
@@ -83,7 +112,7 @@ class DevBuf {
     void release()
     {
         if (p_) {
-            (void)hipFree(p_);
+            dev_free(p_, block_, dev_, stream_);
             if (budget_) budget_->used -= bytes_;
         }
         p_ = nullptr;
@@ -102,8 +131,14 @@ class DevBuf {
         std::swap(n_, o.n_);
         std::swap(budget_, o.budget_);
         std::swap(bytes_, o.bytes_);
+        std::swap(stream_, o.stream_);
+        std::swap(block_, o.block_);
+        std::swap(dev_, o.dev_);
     }
     T* p_ = nullptr;
+    hipStream_t stream_ = nullptr;
+    size_t block_ = 0;
+    int dev_ = 0;
     size_t n_ = 0;
     MemBudget* budget_ = nullptr;
     int64_t bytes_ = 0;

@@ -11,7 +11,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +25,102 @@
 #include "radix_sort.hpp"
 
 namespace tka {
+
+namespace {
+struct CachedBlock {
+    void* p;
+    hipEvent_t ready;  // recorded at the free: the last work that may use p
+};
+std::mutex g_cache_mu;
+std::map<std::pair<int, size_t>, std::vector<CachedBlock>> g_cache;  // (device, block bytes) -> free blocks
+size_t g_cache_bytes = 0;
+constexpr size_t kCacheLimit = (size_t)32 << 30;  // cached bytes beyond this are returned with hipFree
+
+// Size classes: powers of two up to 1 MiB, whole 2 MiB pages above.
+size_t size_class(size_t b)
+{
+    if (b <= ((size_t)1 << 20)) {
+        size_t r = 256;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    const size_t page = (size_t)2 << 20;
+    return (b + page - 1) / page * page;
+}
+
+// Returns every cached block of the device (after its last use) to the runtime.
+void trim_cache(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    for (auto it = g_cache.begin(); it != g_cache.end();) {
+        if (it->first.first != dev) {
+            ++it;
+            continue;
+        }
+        for (auto& b : it->second) {
+            (void)hipEventSynchronize(b.ready);
+            (void)hipEventDestroy(b.ready);
+            (void)hipFree(b.p);
+            g_cache_bytes -= it->first.second;
+        }
+        it = g_cache.erase(it);
+    }
+}
+}  // namespace
+
+void* dev_alloc(size_t bytes, size_t* block_bytes, int* device)
+{
+    const size_t bb = size_class(bytes);
+    int dev = 0;
+    TKA_HIP(hipGetDevice(&dev));
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_cache.find({dev, bb});
+        if (it != g_cache.end()) {
+            auto& v = it->second;
+            for (size_t i = 0; i < v.size(); i++) {
+                if (hipEventQuery(v[i].ready) != hipSuccess) continue;  // still in use by queued work
+                void* p = v[i].p;
+                (void)hipEventDestroy(v[i].ready);
+                v.erase(v.begin() + (long)i);
+                g_cache_bytes -= bb;
+                *block_bytes = bb;
+                *device = dev;
+                return p;
+            }
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bb);
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) {
+        (void)hipGetLastError();
+        trim_cache(dev);
+        e = hipMalloc(&p, bb);
+    }
+    TKA_HIP(e);
+    *block_bytes = bb;
+    *device = dev;
+    return p;
+}
+
+void dev_free(void* p, size_t bb, int dev, hipStream_t stream) noexcept
+{
+    hipEvent_t ev = nullptr;
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess && hipEventRecord(ev, stream) == hipSuccess) {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        if (g_cache_bytes + bb <= kCacheLimit) {
+            g_cache[{dev, bb}].push_back(CachedBlock{p, ev});
+            g_cache_bytes += bb;
+            return;
+        }
+    }
+    if (ev) {
+        (void)hipEventSynchronize(ev);
+        (void)hipEventDestroy(ev);
+    }
+    (void)hipFree(p);
+}
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -1194,6 +1292,7 @@ TKA_API tka_status tka_gradient_dev(const float* Y, int32_t N, int32_t no_dims, 
         MemBudget mb;
         Stream st(stream);
         hipStream_t s = st.s;
+        AllocStream as(s);  // temporaries ordered on the caller's stream, which may be non-blocking
         const size_t nd = (size_t)N * no_dims;
         DevBuf<float> neg(nd);
         int nz;
@@ -1236,6 +1335,7 @@ TKA_API tka_status tka_kl_dev(const float* Y, int32_t N, int32_t no_dims, const 
         MemBudget mb;
         Stream st(stream);
         hipStream_t s = st.s;
+        AllocStream as(s);  // temporaries ordered on the caller's stream, which may be non-blocking
         double Z;
         if (theta > 0.0) {
             require(no_dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut KL needs 2 dimensions");

@@ -101,43 +101,48 @@ TKA_DEV void wave_sort_pairs(uint64_t (&d)[8], uint32_t (&ix)[8])
 
 // ---------------------------------------------------------------- normalise
 // Column means with the reference's summation order: row 0, 1, 2, ... into a
-// zero-initialised FP64 accumulator (tsne.hpp:496-506).  One block per 64
-// columns; all waves prefetch the next 64-row chunk into registers while wave
-// 0 adds the current chunk from LDS.
-__global__ __launch_bounds__(256) void col_mean_seq(const double* __restrict__ X, int N, int D,
-                                                    double* __restrict__ mean)
+// zero-initialised FP64 accumulator, no contraction (tsne.hpp:496-506).  Each
+// column's sum is one chain of N dependent adds, so one wave takes one column:
+// its lanes keep kColGroups groups of 64 consecutive rows in flight (strided
+// loads of that column; the columns' waves share the cache lines), and the
+// chain takes the values of a group lane by lane through readlane.
+constexpr int kColGroups = 8;
+
+TKA_DEV double readlane_d(double v, int j)
 {
-    constexpr int R = 64;
-    __shared__ double tile[R][65];
-    const int c0 = blockIdx.x * 64;
-    const int tid = threadIdx.x;
-    double regs[R * 64 / 256];
-    auto load = [&](int r0) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, j);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), j);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__global__ __launch_bounds__(64) void col_mean_seq(const double* __restrict__ X, int N, int D,
+                                                   double* __restrict__ mean)
+{
+    const int d = blockIdx.x, lane = threadIdx.x;
+    double buf[kColGroups];
 #pragma unroll
-        for (int e = 0; e < R * 64 / 256; e++) {
-            const int idx = e * 256 + tid;
-            const int r = idx >> 6, c = idx & 63;
-            const int row = r0 + r, col = c0 + c;
-            regs[e] = (row < N && col < D) ? X[(size_t)row * D + col] : 0.0;
-        }
-    };
+    for (int g = 0; g < kColGroups; g++) {
+        const int row = g * 64 + lane;
+        buf[g] = row < N ? X[(size_t)row * D + d] : 0.0;
+    }
     double acc = 0.0;
-    load(0);
-    for (int r0 = 0; r0 < N; r0 += R) {
-        __syncthreads();
+    for (int r0 = 0; r0 < N; r0 += 64 * kColGroups) {
 #pragma unroll
-        for (int e = 0; e < R * 64 / 256; e++) {
-            const int idx = e * 256 + tid;
-            tile[idx >> 6][idx & 63] = regs[e];
-        }
-        __syncthreads();
-        if (r0 + R < N) load(r0 + R);
-        if (tid < 64) {
-            const int rows = (N - r0) < R ? (N - r0) : R;
-            for (int r = 0; r < rows; r++) acc = dadd_sep(acc, tile[r][tid]);
+        for (int g = 0; g < kColGroups; g++) {
+            const int base = r0 + 64 * g;
+            const double v = buf[g];
+            const int next = base + 64 * kColGroups + lane;  // refill with the group kColGroups ahead
+            buf[g] = next < N ? X[(size_t)next * D + d] : 0.0;
+            if (base + 64 <= N) {
+#pragma unroll
+                for (int j = 0; j < 64; j++) acc = dadd_sep(acc, readlane_d(v, j));
+            } else if (base < N) {
+                for (int j = 0; j < N - base; j++) acc = dadd_sep(acc, __shfl(v, j, 64));
+            }
         }
     }
-    if (tid < 64 && c0 + tid < D) mean[c0 + tid] = acc / (double)N;
+    if (lane == 0) mean[d] = acc / (double)N;
 }
 
 __global__ __launch_bounds__(256) void center_max(double* __restrict__ X, size_t total, int D,
@@ -275,17 +280,6 @@ TKA_DEV void wave_lds_sync()
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-TKA_DEV int wave_incl_scan_i(int v)
-{
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
-    }
-    return v;
 }
 
 // Threshold selection over the ordered keys (high words) of up to 64*R
@@ -1245,7 +1239,7 @@ __global__ __launch_bounds__(256) void gather_rows(const double* __restrict__ X,
 void normalize_device(double* X, int N, int D, MemBudget* mb, hipStream_t s)
 {
     DevBuf<double> mean(D, mb), bmax(1024, mb), mx(1, mb);
-    hipLaunchKernelGGL(col_mean_seq, dim3(ceil_div(D, 64)), dim3(256), 0, s, X, N, D, mean.get());
+    hipLaunchKernelGGL(col_mean_seq, dim3(D), dim3(64), 0, s, X, N, D, mean.get());
     TKA_LAUNCH_CHECK();
     const size_t total = (size_t)N * D;
     const int nb = (int)std::min<size_t>(1024, ceil_div(total, 256));

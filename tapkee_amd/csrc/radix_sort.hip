@@ -25,20 +25,35 @@ constexpr uint32_t kValMask = (1u << 30) - 1;
 constexpr int kSpinLimit = 1 << 22;
 constexpr int kLookback = 16;             // predecessor tiles read per look-back round
 
+// Standalone counting launch: every block histograms a strided set of tiles
+// in LDS and adds to the global counts once (one atomic per pass and digit
+// per tile would pile thousands of tiles onto the same 256*passes counters).
+constexpr int kRsCountBlocks = 256;
+
 template <class K>
 __global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys, size_t n, int passes, RsLayout L,
                                                      int tiles)
 {
-    K key[kRsItems];
-    bool ok[kRsItems];
-    const size_t base = (size_t)blockIdx.x * kRsBlock * kRsItems;
+    __shared__ uint32_t h[kRsMaxPasses][256];
+    const int t = threadIdx.x;
+    for (int p = 0; p < passes; p++) h[p][t] = 0;
+    if (blockIdx.x == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
+    __syncthreads();
+    for (int tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        for (int p = 0; p < passes; p++) L.status[((size_t)p * tiles + tile) * 256 + t] = 0;
+        K key[kRsItems];
+        bool ok[kRsItems];
+        const size_t base = (size_t)tile * kRsBlock * kRsItems;
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
-        const size_t i = base + (size_t)k * kRsBlock + threadIdx.x;
-        ok[k] = i < n;
-        key[k] = ok[k] ? keys[i] : K(0);
+        for (int k = 0; k < kRsItems; k++) {
+            const size_t i = base + (size_t)k * kRsBlock + t;
+            ok[k] = i < n;
+            key[k] = ok[k] ? keys[i] : K(0);
+        }
+        rs_tile_histogram<K>(key, ok, passes, h);
     }
-    rs_count_tile<K>(key, ok, passes, L, (int)blockIdx.x, tiles);
+    __syncthreads();
+    rs_count_flush(passes, L, h, (int)gridDim.x);
 }
 
 // Stable scatter of one tile.  Item k of lane l in wave w sits at tile
@@ -234,7 +249,8 @@ bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit
     const int passes = (end_bit + 7) / 8;
     const RsLayout L = rs_layout(scratch);
     if (!counted) {
-        hipLaunchKernelGGL((rs_count<K>), dim3(tiles), dim3(kRsBlock), 0, s, keys, n, passes, L, tiles);
+        hipLaunchKernelGGL((rs_count<K>), dim3(std::min(tiles, kRsCountBlocks)), dim3(kRsBlock), 0, s, keys, n, passes, L,
+                           tiles);
         TKA_LAUNCH_CHECK();
     }
     K* ka = keys;

@@ -85,28 +85,14 @@ TKA_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total)
     return before + inc - v;
 }
 
-// Counting step of the sort for one tile, called by all kRsBlock threads of
-// the workgroup that holds tile `tile` (keys tile*kRsBlock*kRsItems + k*kRsBlock
-// + threadIdx.x, k < kRsItems; ok[k] false past n).  Adds the tile's digit
-// histograms of all passes to the global counts, clears the tile's status
-// words and, in the last tile to arrive, turns the counts into digit bases.
+// Adds the digit histograms (all passes) of one tile's keys to a block's
+// LDS histograms h.  Keys that arrive nearly sorted (CSR edges, Morton keys
+// of points kept in Morton order) give whole waves one digit: those add once,
+// from one lane, instead of 64 atomics serialised on a single bin.
 template <class K>
-TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems], int passes, RsLayout L, int tile,
-                           int tiles)
+TKA_DEV void rs_tile_histogram(const K (&key)[kRsItems], const bool (&ok)[kRsItems], int passes,
+                               uint32_t (*h)[256])
 {
-    __shared__ uint32_t h[kRsMaxPasses][256];
-    __shared__ uint32_t sh[kRsBlock / 64];
-    __shared__ int is_last;
-    const int t = threadIdx.x;
-    for (int p = 0; p < passes; p++) {
-        h[p][t] = 0;
-        L.status[((size_t)p * tiles + tile) * 256 + t] = 0;
-    }
-    if (tile == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
-    __syncthreads();
-    // Keys that arrive nearly sorted (CSR edges, Morton keys of points kept
-    // in Morton order) give whole waves one digit: those add once, from one
-    // lane, instead of 64 atomics serialised on a single bin.
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < kRsItems; k++) {
@@ -121,23 +107,56 @@ TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems],
             }
         }
     }
-    __syncthreads();
+}
+
+// Adds a block's histograms to the global per-pass counts; the last of the
+// `arrivals` blocks to get here turns the counts into digit bases (and
+// clears them for the next sort).  Called by all kRsBlock threads after a
+// barrier that completes h.
+TKA_DEV void rs_count_flush(int passes, RsLayout L, uint32_t (*h)[256], int arrivals)
+{
+    __shared__ uint32_t sh[kRsBlock / 64];
+    __shared__ int is_last;
+    const int t = threadIdx.x;
     for (int p = 0; p < passes; p++) {
         const uint32_t c = h[p][t];
         if (c) atomicAdd(&L.gcnt[p * 256 + t], c);
     }
     __threadfence();
     __syncthreads();
-    if (t == 0) is_last = (atomicAdd(L.ticket, 1u) == (uint32_t)(tiles - 1));
+    if (t == 0) is_last = (atomicAdd(L.ticket, 1u) == (uint32_t)(arrivals - 1));
     __syncthreads();
     if (!is_last) return;
-    __threadfence();  // acquire: every tile's counts are in
+    __threadfence();  // acquire: every block's counts are in
     for (int p = 0; p < passes; p++) {
         const uint32_t c = rs_ld(&L.gcnt[p * 256 + t]);
         L.gbase[p * 256 + t] = block_exclusive_scan(c, sh, nullptr);
         L.gcnt[p * 256 + t] = 0;
     }
     if (t == 0) atomicExch(L.ticket, 0u);
+}
+
+// Counting step of the sort for one tile, called by all kRsBlock threads of
+// the workgroup that holds tile `tile` (keys tile*kRsBlock*kRsItems + k*kRsBlock
+// + threadIdx.x, k < kRsItems; ok[k] false past n), one workgroup per tile.
+// Adds the tile's digit histograms of all passes to the global counts, clears
+// the tile's status words and, in the last tile to arrive, turns the counts
+// into digit bases.
+template <class K>
+TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems], int passes, RsLayout L, int tile,
+                           int tiles)
+{
+    __shared__ uint32_t h[kRsMaxPasses][256];
+    const int t = threadIdx.x;
+    for (int p = 0; p < passes; p++) {
+        h[p][t] = 0;
+        L.status[((size_t)p * tiles + tile) * 256 + t] = 0;
+    }
+    if (tile == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
+    __syncthreads();
+    rs_tile_histogram<K>(key, ok, passes, h);
+    __syncthreads();
+    rs_count_flush(passes, L, h, tiles);
 }
 
 // Nonzero if a sort using this scratch gave up waiting on a predecessor tile
