@@ -43,14 +43,14 @@ PEAK_HBM_GBS = 8000.0
 FLOPS_PER_INTERACTION = 14   # dx, dy, D, q = 1/(1+D), z += c q, m = c q^2, f += m d (2)
 
 # HBM bytes per launch of the traversal kernel, from two rocprofv3 --pmc
-# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch) of
-# `bench.py --steps 10 --warmup 2 --no-cpu --no-pipeline --no-sweep --copy-calibration 256`,
-# each scaled by what the counter reads for a plain 256 MiB device copy in the
-# same run (FETCH_SIZE reads 131084 KiB for 262144 KiB moved: x2.0;
-# WRITE_SIZE reads 262144 KiB: x1.0).  Summary in
-# profiles/pmc_traffic_bh_tsne_swissroll_n100k.md.
-TRAVERSE_FETCH_KIB = 4790.0
-TRAVERSE_WRITE_KIB = 2604.0
+# passes (FETCH_SIZE, then WRITE_SIZE, KiB per dispatch, mean of 22
+# dispatches) of `bench.py --steps 10 --warmup 2 --no-cpu --no-pipeline
+# --no-sweep --no-accuracy --no-parametric --copy-calibration 256`, each scaled
+# by what the counter reads for a plain 256 MiB device copy in the same run
+# (FETCH_SIZE reads 131093 KiB for 262144 KiB moved: x2.0; WRITE_SIZE reads
+# 262144 KiB: x1.0).  Summary in profiles/pmc_traffic_bh_traverse_swissroll_n100k.md.
+TRAVERSE_FETCH_KIB = 2211.2
+TRAVERSE_WRITE_KIB = 2593.0
 FETCH_SCALE = 2.0
 WRITE_SCALE = 1.0
 
