@@ -524,6 +524,25 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
         px = ys[j].x;
         py = ys[j].y;
     }
+    // The tree was written by the previous kernel through other XCDs' L2s,
+    // so this XCD's L2 starts cold and every first touch of a node line waits
+    // on the memory behind it.  The XCD's own points have their leaves and
+    // finest cells in one pre-order range, [pre[j0], pre[j1]); its workgroups
+    // first fetch one word of every 128-byte line of that range, each a slice,
+    // with independent loads that overlap, so the walks' dependent node loads
+    // mostly hit L2.  The fetched words are only consumed at the end.
+    T warm = (T)0;
+    {
+        const int q = nb >> 3, r = nb & 7, count = q + (xcd < r ? 1 : 0);
+        const int first = xcd * q + min(xcd, r);
+        const int j0 = min(N, first * 256), j1 = min(N, (first + count) * 256);
+        const int a = (int)pre[j0], b = (int)pre[j1];
+        const int s0 = a + (int)((long long)(b - a) * kx / count);
+        const int s1 = a + (int)((long long)(b - a) * (kx + 1) / count);
+        constexpr int kPerLine = 128 / (int)sizeof(BhNode<T>);
+        const int t = s0 + kPerLine * (int)threadIdx.x;
+        if (t < s1) warm = nodes[t].x;
+    }
     T fx = 0, fy = 0, z = 0;
     if (active) {
         BhNode<T> blk[4];
@@ -545,6 +564,7 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
         neg[2 * (size_t)p + 1] = fy;
         zd = (double)z;
     }
+    asm volatile("" ::"v"(warm));  // keeps the warming loads
     zd = block_sum(zd, sh);
     if (threadIdx.x == 0) zpart[lb] = zd;
     if (interactions != nullptr) {

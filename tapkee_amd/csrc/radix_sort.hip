@@ -2,18 +2,22 @@
 //
 // Sort = one counting launch + one launch per 8-bit digit ("one sweep"):
 //   rs_count     every tile histograms all digits of its keys at once and adds
-//                them to global per-pass counts; the last tile to arrive turns
-//                the counts into per-pass digit bases (and clears them for the
-//                next sort).  It also clears the tile status words.  A kernel
-//                that produces the keys can do this itself (rs_count_tile).
-//   rs_onesweep  takes a tile number in arrival order, ranks its keys by the
-//                pass's digit (wave ballots + per-wave LDS counters), publishes
-//                its per-digit counts, sums its predecessors' counts by
-//                decoupled look-back (Merrill & Garland 2016; Adinets & Merrill
-//                2022) and scatters.  Tiles are numbered in the order their
+//                them to global per-pass counts; it also clears the tile
+//                status words.  A kernel that produces the keys can do this
+//                itself (rs_count_tile).
+//   rs_onesweep  turns the pass's global counts into digit bases (a block
+//                scan, in every tile: no launch or arrival round trip between
+//                counting and scattering), ranks its keys by the pass's digit
+//                (wave ballots + per-wave LDS counters), publishes its
+//                per-digit counts, sums its predecessors' counts by decoupled
+//                look-back (Merrill & Garland 2016; Adinets & Merrill 2022)
+//                and scatters.  The last tile to read the counts clears them.
+//                Up to kRsMaxIndexedTiles tiles, tile = workgroup index: the
+//                whole grid fits on the device, so a tile only waits on tiles
+//                that run.  Beyond, tiles are numbered in the order their
 //                workgroups start, so a tile only waits on tiles that are
-//                already running: the wait always ends.  It is bounded anyway;
-//                a tile that gives up sets a fault word the host can read.
+//                already running.  The wait is bounded anyway; a tile that
+//                gives up sets a fault word the host can read.
 #include "radix_sort.hpp"
 
 namespace tka {
@@ -23,7 +27,7 @@ constexpr uint32_t kFlagAgg = 1u << 30;   // tile's own count published
 constexpr uint32_t kFlagInc = 2u << 30;   // inclusive prefix through this tile published
 constexpr uint32_t kValMask = (1u << 30) - 1;
 constexpr int kSpinLimit = 1 << 22;
-constexpr int kLookback = 16;             // predecessor tiles read per look-back round
+constexpr int kLookback = 4;              // predecessor tiles read per look-back round
 
 // Standalone counting launch: every block histograms a strided set of tiles
 // in LDS and adds to the global counts once (one atomic per pass and digit
@@ -53,7 +57,7 @@ __global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys,
         rs_tile_histogram<K>(key, ok, passes, h);
     }
     __syncthreads();
-    rs_count_flush(passes, L, h, (int)gridDim.x);
+    rs_count_flush(passes, L, h);
 }
 
 // Stable scatter of one tile.  Item k of lane l in wave w sits at tile
@@ -62,17 +66,23 @@ __global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys,
 template <class K, class V, int ITEMS>
 __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ kin, const V* __restrict__ vin,
                                                         K* __restrict__ kout, V* __restrict__ vout, size_t n,
-                                                        int pass, RsLayout L, int tiles)
+                                                        int pass, RsLayout L, int tiles, int ticketed)
 {
     constexpr int kWaves = kRsBlock / 64;
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t base[256];
+    __shared__ uint32_t scan_sh[kWaves];
     __shared__ uint32_t tile_sh;
+    __shared__ int last_sh;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // the global count of digit threadIdx.x, in flight with the keys
+    const uint32_t gc = L.gcnt[pass * 256 + threadIdx.x];
     for (int t = threadIdx.x; t < kWaves * 256; t += kRsBlock) (&wcnt[0][0])[t] = 0;
-    if (threadIdx.x == 0) tile_sh = atomicAdd(&L.tile_ctr[pass], 1u);
-    __syncthreads();
-    const uint32_t tile = tile_sh;
+    if (ticketed) {
+        if (threadIdx.x == 0) tile_sh = atomicAdd(&L.tile_ctr[pass], 1u);
+        __syncthreads();
+    }
+    const uint32_t tile = ticketed ? tile_sh : blockIdx.x;
     const int shift = 8 * pass;
 
     K key[ITEMS];
@@ -86,8 +96,15 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
         const bool ok = i < n;
         key[k] = ok ? kin[i] : K(0);
         val[k] = ok ? vin[i] : V(0);
-        dig[k] = ok ? (int)((uint32_t)(key[k] >> shift) & 255u) : -1;
     }
+    // digit bases; the barrier inside also completes the clearing of wcnt
+    const uint32_t gbase = block_exclusive_scan(gc, scan_sh, nullptr);
+    // every thread has its count now: register this tile as a reader (the
+    // result is needed only at the end)
+    uint32_t readers = 0;
+    if (threadIdx.x == 0) readers = atomicAdd(&L.rd_ctr[pass], 1u);
+#pragma unroll
+    for (int k = 0; k < ITEMS; k++) dig[k] = (tile0 + (size_t)k * 64 < n) ? (int)((uint32_t)(key[k] >> shift) & 255u) : -1;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) {
@@ -158,7 +175,8 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
             }
             rs_st(st + (size_t)tile * 256, kFlagInc | (excl + cnt));
         }
-        base[d] = L.gbase[pass * 256 + d] + excl;
+        base[d] = gbase + excl;
+        if (threadIdx.x == 0) last_sh = readers == (uint32_t)(tiles - 1);
     }
     __syncthreads();
 #pragma unroll
@@ -169,6 +187,11 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
             kout[pos] = key[k];
             vout[pos] = val[k];
         }
+    }
+    // every tile has read the counts: clear them for the next sort
+    if (last_sh) {
+        L.gcnt[pass * 256 + threadIdx.x] = 0;
+        if (threadIdx.x == 0) L.rd_ctr[pass] = 0;
     }
 }
 
@@ -259,7 +282,7 @@ bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit
     V* vb = vals_alt;
     for (int p = 0; p < passes; p++) {
         hipLaunchKernelGGL((rs_onesweep<K, V, ITEMS>), dim3(tiles), dim3(kRsBlock), 0, s, ka, va, kb, vb, n, p, L,
-                           tiles);
+                           tiles, tiles > kRsMaxIndexedTiles ? 1 : 0);
         TKA_LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);

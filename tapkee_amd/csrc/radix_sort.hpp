@@ -33,23 +33,27 @@ bool radix_sort_pairs_counted(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t
                               hipStream_t stream);
 
 // ---------------------------------------------------------------- device side
-// Scratch layout (words): [0] arrival ticket, [1] fault, [2, 2+8) tile counters
-// per pass, then global digit counts and bases (8 x 256 each), then tile
-// status words (passes x tiles x 256).
-constexpr size_t kRsHeader = 16 + 2 * (size_t)kRsMaxPasses * 256;
+// Scratch layout (words): [1] fault, [2, 2+8) tile tickets per pass, [16,
+// 16+8) per-pass count readers, then the global digit counts (8 x 256), then
+// the tile status words (passes x tiles x 256).
+constexpr size_t kRsHeader = 32 + (size_t)kRsMaxPasses * 256;
+
+// Tiles numbered by workgroup index (no arrival tickets) up to this many: all
+// of a sort's workgroups then fit on the device at once, so every tile a
+// workgroup waits on gets to run.  Larger sorts number tiles in arrival order.
+constexpr int kRsMaxIndexedTiles = 512;
 
 struct RsLayout {
-    uint32_t* ticket;
     uint32_t* fault;
     uint32_t* tile_ctr;
+    uint32_t* rd_ctr;
     uint32_t* gcnt;
-    uint32_t* gbase;
     uint32_t* status;
 };
 
 inline RsLayout rs_layout(uint32_t* s)
 {
-    return {s, s + 1, s + 2, s + 16, s + 16 + kRsMaxPasses * 256, s + kRsHeader};
+    return {s + 1, s + 2, s + 16, s + 32, s + kRsHeader};
 }
 
 TKA_DEV uint32_t rs_ld(const uint32_t* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
@@ -109,39 +113,25 @@ TKA_DEV void rs_tile_histogram(const K (&key)[kRsItems], const bool (&ok)[kRsIte
     }
 }
 
-// Adds a block's histograms to the global per-pass counts; the last of the
-// `arrivals` blocks to get here turns the counts into digit bases (and
-// clears them for the next sort).  Called by all kRsBlock threads after a
-// barrier that completes h.
-TKA_DEV void rs_count_flush(int passes, RsLayout L, uint32_t (*h)[256], int arrivals)
+// Adds a block's histograms to the global per-pass counts.  Called by all
+// kRsBlock threads after a barrier that completes h.  The counts are turned
+// into digit bases by each tile of the pass that uses them, and cleared by
+// the last tile to read them, ready for the next sort.
+TKA_DEV void rs_count_flush(int passes, RsLayout L, uint32_t (*h)[256])
 {
-    __shared__ uint32_t sh[kRsBlock / 64];
-    __shared__ int is_last;
     const int t = threadIdx.x;
     for (int p = 0; p < passes; p++) {
         const uint32_t c = h[p][t];
         if (c) atomicAdd(&L.gcnt[p * 256 + t], c);
     }
-    __threadfence();
-    __syncthreads();
-    if (t == 0) is_last = (atomicAdd(L.ticket, 1u) == (uint32_t)(arrivals - 1));
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();  // acquire: every block's counts are in
-    for (int p = 0; p < passes; p++) {
-        const uint32_t c = rs_ld(&L.gcnt[p * 256 + t]);
-        L.gbase[p * 256 + t] = block_exclusive_scan(c, sh, nullptr);
-        L.gcnt[p * 256 + t] = 0;
-    }
-    if (t == 0) atomicExch(L.ticket, 0u);
 }
 
 // Counting step of the sort for one tile, called by all kRsBlock threads of
 // the workgroup that holds tile `tile` (keys tile*kRsBlock*kRsItems + k*kRsBlock
 // + threadIdx.x, k < kRsItems; ok[k] false past n), one workgroup per tile.
-// Adds the tile's digit histograms of all passes to the global counts, clears
-// the tile's status words and, in the last tile to arrive, turns the counts
-// into digit bases.
+// Adds the tile's digit histograms of all passes to the global counts and
+// clears the tile's status words.  The sort's passes must follow: they clear
+// the counts again.
 template <class K>
 TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems], int passes, RsLayout L, int tile,
                            int tiles)
@@ -156,7 +146,7 @@ TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems],
     __syncthreads();
     rs_tile_histogram<K>(key, ok, passes, h);
     __syncthreads();
-    rs_count_flush(passes, L, h, tiles);
+    rs_count_flush(passes, L, h);
 }
 
 // Nonzero if a sort using this scratch gave up waiting on a predecessor tile
