@@ -196,62 +196,69 @@ __global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ khi, 
 }
 
 // ---------------------------------------------------------------- pairs
-// Keys [w0, w0 + kPairWin) of the sorted array are staged in LDS; the short
-// searches of deep cells stay inside it.
+// Keys [w0, w0 + kPairWin) of the sorted array are staged in LDS: the
+// searches of deep cells end inside it (binary search in LDS).  A search whose
+// answer lies outside the window (shallow cells, or a cell cut by the window's
+// edge) is done by the whole wave, one lane's search at a time, as a 64-way
+// search over global memory: each round 64 probes narrow the range 64-fold,
+// so the root's search over N keys takes 3 dependent loads at N = 2^18
+// instead of 2 log2 N for a per-thread gallop.
 constexpr int kPairWin = 3 * 256;
 
-TKA_DEV uint64_t key_at(const uint64_t* win, int w0, const uint64_t* __restrict__ k, int g)
+// Smallest g in [lo, hi) with win[g - w0] >= t (strict: > t), or hi; the
+// range lies inside the window.
+TKA_DEV int lds_lower_bound(const uint64_t* win, int w0, int lo, int hi, uint64_t t, bool strict)
 {
-    const int o = g - w0;
-    return (o >= 0 && o < kPairWin) ? win[o] : k[g];
-}
-
-TKA_DEV int gallop_left(const uint64_t* win, int w0, const uint64_t* __restrict__ k, int i, uint64_t lo)
-{
-    // smallest a <= i with k[a] >= lo (k[i] >= lo)
-    int hi = i, lw = -1, step = 1;
-    while (true) {
-        const int probe = i - step;
-        if (probe < 0) break;
-        if (key_at(win, w0, k, probe) < lo) {
-            lw = probe;
-            break;
-        }
-        hi = probe;
-        step <<= 1;
-    }
-    while (hi - lw > 1) {
-        const int mid = (hi + lw) >> 1;
-        if (key_at(win, w0, k, mid) >= lo)
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const uint64_t v = win[mid - w0];
+        if (strict ? v > t : v >= t)
             hi = mid;
         else
-            lw = mid;
+            lo = mid + 1;
+    }
+    return lo;
+}
+
+// The same over global keys k[lo, hi), by all 64 lanes of the wave for one
+// (wave-uniform) query.
+TKA_DEV int wave_lower_bound(const uint64_t* __restrict__ k, int lo, int hi, uint64_t t, bool strict)
+{
+    const int lane = threadIdx.x & 63;
+    while (lo < hi) {
+        const int step = (hi - lo + 63) >> 6;
+        const int q = lo + lane * step;
+        const bool in = q < hi;
+        const uint64_t v = in ? k[q] : 0ull;
+        const uint64_t bal = __ballot(in && (strict ? v > t : v >= t));
+        if (bal == 0ull) {
+            lo += ((hi - 1 - lo) / step) * step + 1;  // past the last probe
+        } else {
+            const int f = __ffsll((long long)bal) - 1;
+            if (f == 0) return lo;
+            hi = lo + f * step;
+            lo += (f - 1) * step + 1;
+        }
     }
     return hi;
 }
 
-TKA_DEV int gallop_right(const uint64_t* win, int w0, const uint64_t* __restrict__ k, int N, int i, uint64_t hi_key)
+// Runs the searches flagged in `need` (one bit per lane; query t of that lane
+// over [lo, hi), lo and hi per lane) with wave_lower_bound; each flagged lane
+// receives its result in *res.
+TKA_DEV void wave_searches(uint64_t need, const uint64_t* __restrict__ k, int lo, int hi, uint64_t t, bool strict,
+                           int* res)
 {
-    // smallest b > i with k[b] > hi_key, or N (k[i] <= hi_key)
-    int lw = i, hi = N, step = 1;
-    while (true) {
-        const int probe = i + step;
-        if (probe >= N) break;
-        if (key_at(win, w0, k, probe) > hi_key) {
-            hi = probe;
-            break;
-        }
-        lw = probe;
-        step <<= 1;
+    const int lane = threadIdx.x & 63;
+    while (need != 0ull) {
+        const int l = __ffsll((long long)need) - 1;
+        need &= need - 1ull;
+        const int ql = __shfl(lo, l, 64), qh = __shfl(hi, l, 64);
+        const uint64_t qt = ((uint64_t)(uint32_t)__shfl((int)(t >> 32), l, 64) << 32) |
+                            (uint32_t)__shfl((int)(uint32_t)t, l, 64);
+        const int r = wave_lower_bound(k, ql, qh, qt, strict);
+        if (lane == l) *res = r;
     }
-    while (hi - lw > 1) {
-        const int mid = (hi + lw) >> 1;
-        if (key_at(win, w0, k, mid) <= hi_key)
-            lw = mid;
-        else
-            hi = mid;
-    }
-    return hi;
 }
 
 __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ keys, int N,
@@ -265,33 +272,53 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
         win[t] = (g >= 0 && g < N) ? keys[g] : 0ull;
     }
     __syncthreads();
+    // the window's valid keys: [wl, wh)
+    const int wl = max(w0, 0), wh = min(w0 + kPairWin, N);
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= N - 1) return;
-    const uint64_t ki = win[i - w0], kj = win[i + 1 - w0];
+    const bool act = i < N - 1;  // pair (i, i+1); every lane stays for the wave's searches
+    const uint64_t ki = act ? win[i - w0] : 0ull, kj = act ? win[i + 1 - w0] : 0ull;
     const uint64_t x = ki ^ kj;
-    int lvl, a, b;
-    bool owner;
-    if (x == 0ull) {
-        lvl = kB;
-        owner = (i == 0) || (key_at(win, w0, keys, i - 1) != ki);
-        a = i;
-        b = owner ? gallop_right(win, w0, keys, N, i + 1, ki) : 0;
-    } else {
-        lvl = __clzll((long long)x) >> 1;
-        const int lowbits = 2 * (kB - lvl);  // 2..64
-        const uint64_t lowmask = (lowbits >= 64) ? ~0ull : ((1ull << lowbits) - 1ull);
-        const uint64_t lo = ki & ~lowmask, hi = ki | lowmask;
-        a = gallop_left(win, w0, keys, i, lo);
-        const int sh = lowbits - 2;
-        owner = (ki >> sh) == (key_at(win, w0, keys, a) >> sh);
-        b = owner ? gallop_right(win, w0, keys, N, i + 1, hi) : 0;
+    const int lvl = (x == 0ull) ? kB : (__clzll((long long)x) >> 1);
+    const int lowbits = 2 * (kB - lvl);  // 0..64
+    const uint64_t lowmask = (lowbits >= 64) ? ~0ull : ((1ull << lowbits) - 1ull);
+    const uint64_t lo_key = ki & ~lowmask, hi_key = ki | lowmask;
+
+    // a = first key of the level-lvl cell holding ki (k[a] >= lo_key; k[i] >= lo_key)
+    int a = i;
+    bool far = false;
+    if (act && x != 0ull) {
+        far = w0 > 0 && win[0] >= lo_key;  // every window key is in the cell: the start is before it
+        if (!far) a = lds_lower_bound(win, w0, wl, i, lo_key, false);
     }
+    wave_searches(__ballot(far), keys, 0, w0 + 1, lo_key, false, &a);
+
+    // owner: equal keys -> the first pair of the run; else the pair straddling
+    // the end of the cell's first child (ki in the same child as k[a])
+    bool owner = false;
+    if (act) {
+        if (x == 0ull) {
+            owner = (i == 0) || (win[i - 1 - w0] != ki);
+        } else {
+            const int sh = lowbits - 2;
+            const uint64_t ka = (a >= wl && a < wh) ? win[a - w0] : keys[a];
+            owner = (ki >> sh) == (ka >> sh);
+        }
+    }
+    // b = first index past the cell (k[b] > hi_key), or N
+    int b = 0;
+    far = false;
+    if (owner) {
+        far = wh < N && win[wh - 1 - w0] <= hi_key;  // every window key up to its end is in the cell
+        if (!far) b = lds_lower_bound(win, w0, i + 1, wh, hi_key, true);
+    }
+    wave_searches(__ballot(far), keys, wh, N, hi_key, true, &b);
+
     if (owner) {
         atomicOr(&mask[a], 1ull << lvl);
         node_a[i] = a;
         node_b[i] = b;
         node_lvl[i] = lvl;
-    } else {
+    } else if (act) {
         node_lvl[i] = -1;
     }
 }
