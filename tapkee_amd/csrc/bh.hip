@@ -16,16 +16,20 @@
 //          ceil(t * 2^32) - 1, so coordinates on a split line go west/north;
 //          its top 16 bits equal ceil(t * 2^16) - 1, so the key's high word
 //          is the 16+16 bit key of the level-16 cell.
-//   sort   stable LSD radix sort of (high word, point).
-//   fix    runs of equal high words (points within 2^-16 of the box of each
-//          other: rare) are put in full-key order by a stable insertion sort,
-//          so the result equals a stable sort of the 64-bit keys.
+//   sort   stable LSD radix sort of (top 8*passes key bits, point), 3
+//          passes (levels 1-12) up to 2^22 points, else 4.
+//   fix    runs of equal top bits (points within 2^-12 of the box of each
+//          other, a few at most in t-SNE embeddings) are put in full-key
+//          order: short runs by a stable insertion sort, long ones by a
+//          wave-wide bitonic sort of (key, point); the result equals a stable
+//          sort of the 64-bit keys.
 //   pairs  adjacent sorted keys i, i+1 first differ at quadtree level
 //          l = clz64(k_i ^ k_i+1)/2; the level-l cell holding both has >= 2
 //          non-empty children, i.e. it is a node of the compressed tree.
 //          The pair whose keys straddle the end of that cell's first child
-//          owns the node; galloping searches (over an LDS window of keys,
-//          global memory beyond it) give the cell's key range [a,b).
+//          owns the node; searches give the cell's key range [a,b) (binary
+//          searches in an LDS window of keys, a wave-wide 64-way search in
+//          global memory beyond it).
 //          Chains of one-child cells are skipped: such a cell has the same
 //          count and centre of mass as its child and a larger size, so the
 //          reference accepts it only if it would accept the child.
@@ -51,7 +55,6 @@ namespace {
 constexpr int kB = kMortonBits;
 constexpr int kTreeScanItems = 4;
 constexpr int kTreeScanTile = 256 * kTreeScanItems;
-constexpr int kHiPasses = 4;  // radix passes over the 32-bit high key words
 
 // ---------------------------------------------------------------- bbox
 template <class T>
@@ -120,12 +123,12 @@ TKA_DEV uint32_t quantise(double t)
 
 // One radix-sort tile of points per workgroup: root cell (every workgroup
 // reduces the box partials itself; workgroup 0 stores it), centre, key, then
-// the sort's counting step on the high words (so the sort needs no counting
+// the sort's counting step on the top key bits (so the sort needs no counting
 // launch).
 template <class T>
 __global__ __launch_bounds__(kRsBlock) void bh_keys(T* __restrict__ Y, int N, const double* __restrict__ box_part,
-                                                    int nbox, RootBox* __restrict__ root,
-                                                    uint32_t* __restrict__ khi, uint32_t* __restrict__ klo,
+                                                    int nbox, RootBox* __restrict__ root, int passes,
+                                                    uint32_t* __restrict__ ktop, uint64_t* __restrict__ kfull,
                                                     uint32_t* __restrict__ perm,
                                                     unsigned long long* __restrict__ mask, RsLayout L, int tiles)
 {
@@ -147,51 +150,182 @@ __global__ __launch_bounds__(kRsBlock) void bh_keys(T* __restrict__ Y, int N, co
             Y[2 * (size_t)i + 1] = y;
             const uint32_t qx = quantise(((double)x + hw) / (2.0 * hw));
             const uint32_t qy = quantise(((double)y + hh) / (2.0 * hh));
-            hk[k] = (spread16(qy >> 16) << 1) | spread16(qx >> 16);
-            khi[i] = hk[k];
-            klo[i] = (spread16(qy) << 1) | spread16(qx);
+            const uint64_t key = ((uint64_t)((spread16(qy >> 16) << 1) | spread16(qx >> 16)) << 32) |
+                                 ((spread16(qy) << 1) | spread16(qx));
+            hk[k] = (uint32_t)(key >> (64 - 8 * passes));
+            ktop[i] = hk[k];
+            kfull[i] = key;
             perm[i] = (uint32_t)i;
             mask[i] = 0ull;
         }
     }
-    rs_count_tile<uint32_t>(hk, ok, kHiPasses, L, (int)blockIdx.x, tiles);
+    rs_count_tile<uint32_t>(hk, ok, passes, L, (int)blockIdx.x, tiles);
 }
 
 // Full 64-bit keys in sorted order, the final point order and the sorted
-// coordinates.  A point whose high word differs from both neighbours is
-// already in place; the first point of a run of equal high words sorts the
-// run by low word (stable insertion sort, in place in the outputs).
+// coordinates.  The radix sort ordered the points by the top bits of their
+// keys, keeping equal top bits in point order; a run of equal top bits is
+// then sorted by (full key, point), which equals a stable sort by full key:
+//  - a run of at most kShortRun points by its first thread (insertion sort);
+//  - a longer run by the whole wave holding its first point, as a bitonic
+//    sort in the wave's LDS (at most kWaveRun points) or in place in the
+//    outputs in global memory.
+// The bitonic network compares every pair (a, b), a < b, into ascending order
+// (the first step of each merge compares mirror positions), so a run of any
+// length sorts as if padded with +infinity: comparisons with b past the run
+// are skipped.
+constexpr int kShortRun = 16;
+constexpr int kWaveRun = 1024;
+
+TKA_DEV void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+TKA_DEV void wave_sync_global()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+}
+
+template <bool kGlobal>
+TKA_DEV void wave_bitonic(uint64_t* K, uint32_t* P, int len)
+{
+    const int lane = threadIdx.x & 63;
+    int n = 1;
+    while (n < len) n <<= 1;
+    auto cmpx = [&](int a, int b) {
+        if (b >= len) return;
+        const uint64_t ka = K[a], kb = K[b];
+        const uint32_t pa = P[a], pb = P[b];
+        if (kb < ka || (kb == ka && pb < pa)) {
+            K[a] = kb;
+            K[b] = ka;
+            P[a] = pb;
+            P[b] = pa;
+        }
+    };
+    for (int size = 2; size <= n; size <<= 1) {
+        const int half = size >> 1;
+        for (int t = lane; t < (n >> 1); t += 64) {
+            const int blk = (t / half) * size, x = t % half;
+            cmpx(blk + x, blk + size - 1 - x);
+        }
+        if (kGlobal) wave_sync_global(); else wave_sync_lds();
+        for (int d = size >> 2; d >= 1; d >>= 1) {
+            for (int t = lane; t < (n >> 1); t += 64) {
+                const int a = (t / d) * 2 * d + (t % d);
+                cmpx(a, a + d);
+            }
+            if (kGlobal) wave_sync_global(); else wave_sync_lds();
+        }
+    }
+}
+
 template <class T>
-__global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ khi, const uint32_t* __restrict__ perm,
-                                              const uint32_t* __restrict__ klo, const T* __restrict__ Y, int N,
+__global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ ktop, const uint32_t* __restrict__ perm,
+                                              const uint64_t* __restrict__ kfull, const T* __restrict__ Y, int N,
                                               uint64_t* __restrict__ k64, uint32_t* __restrict__ p2,
                                               typename Vec2<T>::type* __restrict__ ys)
 {
+    __shared__ uint64_t sk[4][kWaveRun];
+    __shared__ uint32_t sp[4][kWaveRun];
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= N) return;
-    const uint32_t h = khi[j];
-    if (j > 0 && khi[j - 1] == h) return;  // inside a run: its first point handles it
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // does a run start at j, and where does it end (gallop, then bisect)
+    bool start = false;
     int e = j + 1;
-    while (e < N && khi[e] == h) e++;
-    const uint64_t hk = (uint64_t)h << 32;
-    for (int i = j; i < e; i++) {
-        const uint32_t p = perm[i];
-        const uint32_t l = klo[p];
-        int k = i;
-        while (k > j && (uint32_t)k64[k - 1] > l) {
-            k64[k] = k64[k - 1];
-            p2[k] = p2[k - 1];
-            k--;
+    if (j < N) {
+        const uint32_t h = ktop[j];
+        start = (j == 0) || (ktop[j - 1] != h);
+        if (start && e < N && ktop[e] == h) {
+            int lo = e, hi = N, step = 2;  // ktop[lo] == h; hi: first index known past the run
+            while (j + step < N) {
+                if (ktop[j + step] != h) {
+                    hi = j + step;
+                    break;
+                }
+                lo = j + step;
+                step <<= 1;
+            }
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (ktop[mid] == h)
+                    lo = mid;
+                else
+                    hi = mid;
+            }
+            e = hi;
         }
-        k64[k] = hk | l;
-        p2[k] = p;
     }
-    for (int i = j; i < e; i++) {
-        const uint32_t p = p2[i];
-        typename Vec2<T>::type v;
-        v.x = Y[2 * (size_t)p];
-        v.y = Y[2 * (size_t)p + 1];
-        ys[i] = v;
+    if (start && e - j <= kShortRun) {
+        for (int i = j; i < e; i++) {
+            const uint32_t p = perm[i];
+            const uint64_t key = kfull[p];
+            int k = i;
+            while (k > j && k64[k - 1] > key) {
+                k64[k] = k64[k - 1];
+                p2[k] = p2[k - 1];
+                k--;
+            }
+            k64[k] = key;
+            p2[k] = p;
+        }
+        for (int i = j; i < e; i++) {
+            const uint32_t p = p2[i];
+            typename Vec2<T>::type v;
+            v.x = Y[2 * (size_t)p];
+            v.y = Y[2 * (size_t)p + 1];
+            ys[i] = v;
+        }
+    }
+    // long runs, one at a time, by the whole wave
+    uint64_t longs = __ballot(start && e - j > kShortRun);
+    while (longs != 0ull) {
+        const int l = __ffsll((long long)longs) - 1;
+        longs &= longs - 1ull;
+        const int rj = __shfl(j, l, 64), len = __shfl(e, l, 64) - rj;
+        if (len <= kWaveRun) {
+            uint64_t* K = sk[w];
+            uint32_t* P = sp[w];
+            for (int i = lane; i < len; i += 64) {
+                const uint32_t p = perm[rj + i];
+                P[i] = p;
+                K[i] = kfull[p];
+            }
+            wave_sync_lds();
+            wave_bitonic<false>(K, P, len);
+            for (int i = lane; i < len; i += 64) {
+                const uint32_t p = P[i];
+                k64[rj + i] = K[i];
+                p2[rj + i] = p;
+                typename Vec2<T>::type v;
+                v.x = Y[2 * (size_t)p];
+                v.y = Y[2 * (size_t)p + 1];
+                ys[rj + i] = v;
+            }
+            wave_sync_lds();  // the next run reuses the LDS
+        } else {
+            uint64_t* K = k64 + rj;
+            uint32_t* P = p2 + rj;
+            for (int i = lane; i < len; i += 64) {
+                const uint32_t p = perm[rj + i];
+                P[i] = p;
+                K[i] = kfull[p];
+            }
+            wave_sync_global();
+            wave_bitonic<true>(K, P, len);  // ends with a fence: P is complete
+            for (int i = lane; i < len; i += 64) {
+                const uint32_t p = P[i];
+                typename Vec2<T>::type v;
+                v.x = Y[2 * (size_t)p];
+                v.y = Y[2 * (size_t)p + 1];
+                ys[rj + i] = v;
+            }
+        }
     }
 }
 
@@ -611,7 +745,8 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     keys_alt.alloc(n, mb);
     perm.alloc(n, mb);
     perm_alt.alloc(n, mb);
-    klo.alloc(n, mb);
+    kfull.alloc(n, mb);
+    passes = n <= (1 << 22) ? 3 : 4;
     k64.alloc(n, mb);
     p2.alloc(n, mb);
     sort_scratch.alloc(radix_sort_scratch_words(n), mb);
@@ -643,8 +778,8 @@ void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr,
     require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "tree storage too small");
     const int tiles = (int)radix_tiles((size_t)N);
     hipLaunchKernelGGL(bh_keys<T>, dim3(tiles), dim3(kRsBlock), 0, s, Y, N, box_part, nbox, tr.root.get(),
-                       tr.keys.get(),
-                       tr.klo.get(), tr.perm.get(), tr.mask.get(), rs_layout(tr.sort_scratch.get()), tiles);
+                       tr.passes, tr.keys.get(), tr.kfull.get(), tr.perm.get(), tr.mask.get(),
+                       rs_layout(tr.sort_scratch.get()), tiles);
     TKA_LAUNCH_CHECK();
 }
 
@@ -660,10 +795,10 @@ void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s)
 {
     const unsigned g = ceil_div(N, 256);
     const bool alt = radix_sort_pairs_counted<uint32_t, uint32_t>(
-        tr.keys.get(), tr.perm.get(), tr.keys_alt.get(), tr.perm_alt.get(), N, 8 * kHiPasses, tr.sort_scratch.get(), s);
+        tr.keys.get(), tr.perm.get(), tr.keys_alt.get(), tr.perm_alt.get(), N, 8 * tr.passes, tr.sort_scratch.get(), s);
     const uint32_t* hk = alt ? tr.keys_alt.get() : tr.keys.get();
     const uint32_t* hp = alt ? tr.perm_alt.get() : tr.perm.get();
-    hipLaunchKernelGGL(bh_fix<T>, dim3(g), dim3(256), 0, s, hk, hp, tr.klo.get(), Y, N, tr.k64.get(), tr.p2.get(),
+    hipLaunchKernelGGL(bh_fix<T>, dim3(g), dim3(256), 0, s, hk, hp, tr.kfull.get(), Y, N, tr.k64.get(), tr.p2.get(),
                        tr.ys.get());
     TKA_LAUNCH_CHECK();
     const uint32_t* p = tr.p2.get();

@@ -114,8 +114,9 @@ struct alignas(4 * sizeof(T)) BhNode {
 template <class T>
 struct BhTree {
     int N = 0;
-    DevBuf<uint32_t> keys, keys_alt, perm, perm_alt, sort_scratch;  // high key words, radix-sorted
-    DevBuf<uint32_t> klo;        // low key word per point
+    int passes = 4;              // radix passes: the sort orders the top 8*passes key bits
+    DevBuf<uint32_t> keys, keys_alt, perm, perm_alt, sort_scratch;  // top key bits, radix-sorted
+    DevBuf<uint64_t> kfull;      // full key per point
     DevBuf<uint64_t> k64;        // full keys, sorted
     DevBuf<uint32_t> p2;         // point at each sorted position
     DevBuf<unsigned long long> mask;  // N: levels of internal nodes whose range starts at key j

@@ -236,6 +236,31 @@ def test_bh_tree_matches_reference_tree(engine, oracle, n):
     np.testing.assert_allclose(t["root_box"][2:], r["root_box"][2:], rtol=1e-15)
 
 
+@pytest.mark.parametrize("m", [300, 3000])
+def test_bh_tree_tight_clusters(engine, oracle, m):
+    # m points (and 40 more elsewhere) within ~1e-5 of each other in a box of
+    # width ~9: their keys share the sorted top bits (level-12 cells), so the
+    # full-key order comes from the run sort: a wave's LDS sort for runs of
+    # 17..1024 points, an in-place sort in global memory for the 3000-point
+    # run.  The 64-bit keys are all distinct (checked when the case was made).
+    r = rng(17)
+    Y = r.normal(size=(20000, 2))
+    Y[:m] = r.normal(size=(m, 2)) * 1e-5 + [0.3, -0.2]
+    Y[m:m + 40] = r.normal(size=(40, 2)) * 1e-5 + [-1.1, 0.7]
+    Y -= Y.mean(0)
+    t = engine.bh_tree(Y)
+    ref = oracle.quadtree(Y)
+    keep = (ref["cum"] > 0) & ((ref["leaf"] == 1) | (ref["nchild"] >= 2))
+    rleaf = ref["leaf"][keep] == 1
+    assert len(ref["level"][keep]) == len(t["count"])
+    np.testing.assert_array_equal(ref["cum"][keep], t["count"])
+    np.testing.assert_array_equal(rleaf, t["leaf_point"] >= 0)
+    np.testing.assert_array_equal(ref["level"][keep][~rleaf], t["level"][~rleaf])
+    np.testing.assert_array_equal(ref["index"][keep][rleaf], t["leaf_point"][rleaf])
+    np.testing.assert_allclose(t["comx"], ref["comx"][keep], rtol=0, atol=1e-9)
+    np.testing.assert_allclose(t["comy"], ref["comy"][keep], rtol=0, atol=1e-9)
+
+
 # ---------------------------------------------------------------- KL
 def test_kl_variants(engine, oracle, small_problem):
     row, col, val, Y = small_problem
