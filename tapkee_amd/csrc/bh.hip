@@ -603,43 +603,6 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
 }
 
 // ---------------------------------------------------------------- traverse
-// One node of point p's traversal (quadtree.hpp:317-354): a leaf other than
-// p itself interacts and the walk goes to t+1; an accepted cell
-// (size^2 < theta^2 D) interacts with count*q and the walk jumps past its
-// subtree; otherwise the walk descends to t+1.  Returns the next node.
-template <class T>
-TKA_DEV int bh_visit(const BhNode<T>& nd, int t, uint32_t p, T px, T py, T theta2, T size0sq, T& fx, T& fy, T& z,
-                     unsigned& cnt)
-{
-    const T dx = px - nd.x, dy = py - nd.y;
-    const T D = dx * dx + dy * dy;
-    if (nd.link & kLeafBit) {
-        if ((nd.link & ~kLeafBit) != p) {
-            const T q = (T)1 / ((T)1 + D);
-            z += q;
-            const T q2 = q * q;
-            fx += q2 * dx;
-            fy += q2 * dy;
-            cnt++;
-        }
-        return t + 1;
-    }
-    const int lvl = (int)(nd.cl >> kCountBits);
-    const T size2 = ldexp(size0sq, -2 * lvl);  // (size0 2^-lvl)^2, exact scaling
-    if (size2 < theta2 * D) {
-        const T q = (T)1 / ((T)1 + D);
-        const T cq = (T)(nd.cl & ((1u << kCountBits) - 1u)) * q;
-        z += cq;
-        const T m = cq * q;
-        fx += m * dx;
-        fy += m * dy;
-        cnt++;
-        const int sk = (int)nd.link;
-        return sk > t ? sk : t + 1;  // skip pointers point forward; never loop
-    }
-    return t + 1;
-}
-
 // Node t of a block of four consecutive nodes held in registers (a register
 // array indexed by a per-lane value would go to scratch memory).
 template <class T>
@@ -706,18 +669,44 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
     }
     T fx = 0, fy = 0, z = 0;
     if (active) {
-        BhNode<T> blk[4];
-        int t = 0, base = -4;
+        // the node buffer is padded to whole blocks of four
+        BhNode<T> blk[4] = {nodes[0], nodes[1], nodes[2], nodes[3]};
+        int t = 0, base = 0;
         while (t < total) {
-            if ((t & ~3) != base) {  // the node buffer is padded to whole blocks
-                base = t & ~3;
+            const BhNode<T> nd = pick4(blk, t & 3);
+            const T dx = px - nd.x, dy = py - nd.y;
+            const T D = dx * dx + dy * dy;
+            bool acc;
+            int next;
+            if (nd.link & kLeafBit) {
+                acc = (nd.link & ~kLeafBit) != p;
+                next = t + 1;
+            } else {
+                const int lvl = (int)(nd.cl >> kCountBits);
+                const T size2 = ldexp(size0sq, -2 * lvl);  // (size0 2^-lvl)^2, exact scaling
+                acc = size2 < theta2 * D;
+                const int sk = (int)nd.link;
+                next = (acc && sk > t) ? sk : t + 1;  // skip pointers point forward; never loop
+            }
+            // the next node's block is requested before this node's arithmetic
+            if ((next & ~3) != base && next < total) {
+                base = next & ~3;
                 const BhNode<T>* b4 = nodes + base;
                 blk[0] = b4[0];
                 blk[1] = b4[1];
                 blk[2] = b4[2];
                 blk[3] = b4[3];
             }
-            t = bh_visit<T>(pick4(blk, t & 3), t, p, px, py, theta2, size0sq, fx, fy, z, cnt);
+            if (acc) {  // a leaf's count is 1: count*q and count*q*q are exactly q and q*q
+                const T q = (T)1 / ((T)1 + D);
+                const T cq = (T)(nd.cl & ((1u << kCountBits) - 1u)) * q;
+                z += cq;
+                const T m = cq * q;
+                fx += m * dx;
+                fy += m * dy;
+                cnt++;
+            }
+            t = next;
         }
     }
     if (active) {
