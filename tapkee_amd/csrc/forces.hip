@@ -191,7 +191,9 @@ __global__ __launch_bounds__(256) void attract(const T* __restrict__ Y, const in
 
 // The optimiser step from precomputed attraction and repulsion, one thread
 // per row (tsne.hpp:152-167 and :344-347); also the box partials of Ynext.
-constexpr int kUpdBlock = 1024;
+// Workgroups of 256 spread the rows over 4x more CUs than 1024 would (the
+// launch is latency-bound); the key kernel reduces the N/256 box partials.
+constexpr int kUpdBlock = 256;
 
 template <class T, int DIMS>
 __global__ __launch_bounds__(kUpdBlock) void tsne_update(const T* __restrict__ Y, T* __restrict__ Ynext,
