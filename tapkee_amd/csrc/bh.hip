@@ -53,8 +53,7 @@ namespace tka {
 namespace {
 
 constexpr int kB = kMortonBits;
-constexpr int kTreeScanItems = 4;
-constexpr int kTreeScanTile = 256 * kTreeScanItems;
+constexpr int kTreeScanTile = 256;  // points per scan tile = per pairs workgroup
 
 // ---------------------------------------------------------------- bbox
 template <class T>
@@ -130,8 +129,13 @@ __global__ __launch_bounds__(kRsBlock) void bh_keys(T* __restrict__ Y, int N, co
                                                     int nbox, RootBox* __restrict__ root, int passes,
                                                     uint32_t* __restrict__ ktop, uint64_t* __restrict__ kfull,
                                                     uint32_t* __restrict__ perm,
-                                                    unsigned long long* __restrict__ mask, RsLayout L, int tiles)
+                                                    unsigned long long* __restrict__ mask,
+                                                    uint32_t* __restrict__ tile_cnt, RsLayout L, int tiles)
 {
+    {
+        const int g = blockIdx.x * kRsBlock + threadIdx.x;
+        if (g < (N + kTreeScanTile - 1) / kTreeScanTile) tile_cnt[g] = 0u;
+    }
     const RootBox rb = reduce_root_box<kRsBlock>(box_part, nbox, N);
     if (blockIdx.x == 0 && threadIdx.x == 0) *root = rb;
     const double mx = rb.mean_x, my = rb.mean_y, hw = rb.hw, hh = rb.hh;
@@ -395,16 +399,27 @@ TKA_DEV void wave_searches(uint64_t need, const uint64_t* __restrict__ k, int lo
     }
 }
 
-__global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ keys, int N,
+//
+// The workgroup also leaves the scan's tile partials for its 256 points (one
+// scan tile): the FP64 sums of their coordinates, and, in tile_cnt, the
+// number of internal nodes whose range starts in each tile (the owners'
+// counts, gathered per workgroup for the two tiles the window covers).
+template <class T>
+__global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ keys,
+                                                const typename Vec2<T>::type* __restrict__ ys, int N,
                                                 unsigned long long* __restrict__ mask, int* __restrict__ node_a,
-                                                int* __restrict__ node_b, int* __restrict__ node_lvl)
+                                                int* __restrict__ node_b, int* __restrict__ node_lvl,
+                                                uint32_t* __restrict__ tile_cnt, double* __restrict__ part)
 {
     __shared__ uint64_t win[kPairWin];
+    __shared__ uint32_t own_cnt[2];  // owners with a in tile blockIdx.x - 1, blockIdx.x
+    __shared__ double sh[4];
     const int w0 = (int)blockIdx.x * 256 - 256;
     for (int t = threadIdx.x; t < kPairWin; t += 256) {
         const int g = w0 + t;
         win[t] = (g >= 0 && g < N) ? keys[g] : 0ull;
     }
+    if (threadIdx.x < 2) own_cnt[threadIdx.x] = 0u;
     __syncthreads();
     // the window's valid keys: [wl, wh)
     const int wl = max(w0, 0), wh = min(w0 + kPairWin, N);
@@ -452,39 +467,31 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
         node_a[i] = a;
         node_b[i] = b;
         node_lvl[i] = lvl;
+        const int ta = a / kTreeScanTile;
+        if (ta >= (int)blockIdx.x - 1)
+            atomicAdd(&own_cnt[ta - ((int)blockIdx.x - 1)], 1u);
+        else
+            atomicAdd(&tile_cnt[ta], 1u);
     } else if (act) {
         node_lvl[i] = -1;
+    }
+    // coordinate sums of this tile's points (fixed order: deterministic)
+    const bool pt = i < N;
+    const double sx = block_sum(pt ? (double)ys[i].x : 0.0, sh);
+    const double sy = block_sum(pt ? (double)ys[i].y : 0.0, sh);  // its barriers also complete own_cnt
+    if (threadIdx.x == 0) {
+        part[2 * (size_t)blockIdx.x] = sx;
+        part[2 * (size_t)blockIdx.x + 1] = sy;
+        if (own_cnt[1]) atomicAdd(&tile_cnt[blockIdx.x], own_cnt[1]);
+        if (own_cnt[0]) atomicAdd(&tile_cnt[blockIdx.x - 1], own_cnt[0]);
     }
 }
 
 // ---------------------------------------------------------------- scan
-template <class T>
-__global__ __launch_bounds__(256) void bh_scan_part(const unsigned long long* __restrict__ mask,
-                                                    const typename Vec2<T>::type* __restrict__ ys, int N,
-                                                    double* __restrict__ part)
-{
-    __shared__ double sh[4];
-    const int base = blockIdx.x * kTreeScanTile + threadIdx.x * kTreeScanItems;
-    double c = 0.0, sx = 0.0, sy = 0.0;
-#pragma unroll
-    for (int k = 0; k < kTreeScanItems; k++) {
-        const int j = base + k;
-        if (j < N) {
-            c += (double)(__popcll(mask[j]) + 1);
-            sx += (double)ys[j].x;
-            sy += (double)ys[j].y;
-        }
-    }
-    c = block_sum(c, sh);
-    sx = block_sum(sx, sh);
-    sy = block_sum(sy, sh);
-    if (threadIdx.x == 0) {
-        part[3 * (size_t)blockIdx.x] = c;
-        part[3 * (size_t)blockIdx.x + 1] = sx;
-        part[3 * (size_t)blockIdx.x + 2] = sy;
-    }
-}
-
+// Pre-order positions and coordinate prefix sums, one 256-point tile per
+// workgroup: the prefix of the earlier tiles from their partials (node
+// counts exact in integers, coordinate sums in a fixed order), then the
+// tile's own scan.
 TKA_DEV double wave_incl_scan_d(double v)
 {
     const int lane = threadIdx.x & 63;
@@ -499,37 +506,29 @@ TKA_DEV double wave_incl_scan_d(double v)
 template <class T>
 __global__ __launch_bounds__(256) void bh_scan_write(const unsigned long long* __restrict__ mask,
                                                      const typename Vec2<T>::type* __restrict__ ys, int N,
+                                                     const uint32_t* __restrict__ tile_cnt,
                                                      const double* __restrict__ part, uint32_t* __restrict__ pre,
                                                      double* __restrict__ psx, double* __restrict__ psy)
 {
     __shared__ double sh[4];
     __shared__ double wsum[3][4];
-    // prefix of earlier tiles
-    double b0 = 0.0, b1 = 0.0, b2 = 0.0;
+    // prefix of earlier tiles: nodes = points + internal nodes starting there
+    uint32_t bc = 0;
+    double b1 = 0.0, b2 = 0.0;
     for (int b = threadIdx.x; b < (int)blockIdx.x; b += 256) {
-        b0 += part[3 * (size_t)b];
-        b1 += part[3 * (size_t)b + 1];
-        b2 += part[3 * (size_t)b + 2];
+        bc += (uint32_t)kTreeScanTile + tile_cnt[b];
+        b1 += part[2 * (size_t)b];
+        b2 += part[2 * (size_t)b + 1];
     }
-    b0 = block_sum(b0, sh);
+    const double b0 = block_sum((double)bc, sh);  // integers below 2^53: exact
     b1 = block_sum(b1, sh);
     b2 = block_sum(b2, sh);
-    const int base = blockIdx.x * kTreeScanTile + threadIdx.x * kTreeScanItems;
-    uint32_t c[kTreeScanItems];
-    double x[kTreeScanItems], y[kTreeScanItems];
-    double tc = 0.0, tx = 0.0, ty = 0.0;
-#pragma unroll
-    for (int k = 0; k < kTreeScanItems; k++) {
-        const int j = base + k;
-        const bool ok = j < N;
-        c[k] = ok ? (uint32_t)(__popcll(mask[j]) + 1) : 0u;
-        x[k] = ok ? (double)ys[j].x : 0.0;
-        y[k] = ok ? (double)ys[j].y : 0.0;
-        tc += (double)c[k];
-        tx += x[k];
-        ty += y[k];
-    }
-    const double ic = wave_incl_scan_d(tc), ix = wave_incl_scan_d(tx), iy = wave_incl_scan_d(ty);
+    const int j = blockIdx.x * kTreeScanTile + threadIdx.x;
+    const bool ok = j < N;
+    const double c = ok ? (double)(__popcll(mask[j]) + 1) : 0.0;
+    const double x = ok ? (double)ys[j].x : 0.0;
+    const double y = ok ? (double)ys[j].y : 0.0;
+    const double ic = wave_incl_scan_d(c), ix = wave_incl_scan_d(x), iy = wave_incl_scan_d(y);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     if (lane == 63) {
         wsum[0][w] = ic;
@@ -543,22 +542,14 @@ __global__ __launch_bounds__(256) void bh_scan_write(const unsigned long long* _
         ox += wsum[1][ww];
         oy += wsum[2][ww];
     }
-    double rc = oc + ic - tc, rx = ox + ix - tx, ry = oy + iy - ty;
-#pragma unroll
-    for (int k = 0; k < kTreeScanItems; k++) {
-        const int j = base + k;
-        if (j < N) {
-            pre[j] = (uint32_t)rc;
-            psx[j] = rx;
-            psy[j] = ry;
-        }
-        rc += (double)c[k];
-        rx += x[k];
-        ry += y[k];
+    if (ok) {
+        pre[j] = (uint32_t)(oc + ic - c);
+        psx[j] = ox + ix - x;
+        psy[j] = oy + iy - y;
         if (j == N - 1) {
-            pre[N] = (uint32_t)rc;
-            psx[N] = rx;
-            psy[N] = ry;
+            pre[N] = (uint32_t)(oc + ic);
+            psx[N] = ox + ix;
+            psy[N] = oy + iy;
         }
     }
 }
@@ -749,7 +740,8 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     pre.alloc(n + 1, mb);
     sx.alloc(n + 1, mb);
     sy.alloc(n + 1, mb);
-    part.alloc(3 * (size_t)ceil_div(n, kTreeScanTile), mb);
+    part.alloc(2 * (size_t)ceil_div(n, kTreeScanTile), mb);
+    tile_cnt.alloc(ceil_div(n, kTreeScanTile), mb);
     nodes.alloc(2 * (size_t)n + 4, mb);  // whole blocks of four for the traversal's loads
     root.alloc(1, mb);
 }
@@ -767,7 +759,7 @@ void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr,
     require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "tree storage too small");
     const int tiles = (int)radix_tiles((size_t)N);
     hipLaunchKernelGGL(bh_keys<T>, dim3(tiles), dim3(kRsBlock), 0, s, Y, N, box_part, nbox, tr.root.get(),
-                       tr.passes, tr.keys.get(), tr.kfull.get(), tr.perm.get(), tr.mask.get(),
+                       tr.passes, tr.keys.get(), tr.kfull.get(), tr.perm.get(), tr.mask.get(), tr.tile_cnt.get(),
                        rs_layout(tr.sort_scratch.get()), tiles);
     TKA_LAUNCH_CHECK();
 }
@@ -792,14 +784,12 @@ void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s)
     TKA_LAUNCH_CHECK();
     const uint32_t* p = tr.p2.get();
     tr.sorted_perm = p;
-    hipLaunchKernelGGL(bh_pairs, dim3(g), dim3(256), 0, s, tr.k64.get(), N, tr.mask.get(), tr.node_a.get(),
-                       tr.node_b.get(), tr.node_lvl.get());
+    static_assert(kTreeScanTile == 256, "one scan tile per pairs workgroup");
+    hipLaunchKernelGGL(bh_pairs<T>, dim3(g), dim3(256), 0, s, tr.k64.get(), tr.ys.get(), N, tr.mask.get(),
+                       tr.node_a.get(), tr.node_b.get(), tr.node_lvl.get(), tr.tile_cnt.get(), tr.part.get());
     TKA_LAUNCH_CHECK();
-    const unsigned gs = ceil_div(N, kTreeScanTile);
-    hipLaunchKernelGGL(bh_scan_part<T>, dim3(gs), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N, tr.part.get());
-    TKA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bh_scan_write<T>, dim3(gs), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N, tr.part.get(),
-                       tr.pre.get(), tr.sx.get(), tr.sy.get());
+    hipLaunchKernelGGL(bh_scan_write<T>, dim3(g), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N,
+                       tr.tile_cnt.get(), tr.part.get(), tr.pre.get(), tr.sx.get(), tr.sy.get());
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(bh_emit<T>, dim3(g), dim3(256), 0, s, tr.mask.get(), tr.node_a.get(), tr.node_b.get(),
                        tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(), tr.nodes.get());

@@ -125,7 +125,8 @@ struct BhTree {
     DevBuf<typename Vec2<T>::type> ys;  // sorted coordinates
     DevBuf<uint32_t> pre;        // N+1 exclusive prefix of (popcount(mask)+1)
     DevBuf<double> sx, sy;       // N+1 exclusive prefix sums of sorted coordinates
-    DevBuf<double> part;         // scan partials
+    DevBuf<double> part;         // scan partials: coordinate sums per 256-point tile
+    DevBuf<uint32_t> tile_cnt;   // internal nodes whose range starts in each 256-point tile
     DevBuf<BhNode<T>> nodes;     // pre-order, 2N - 1 at most
     DevBuf<RootBox> root;
     const uint32_t* sorted_perm = nullptr;  // valid after build
