@@ -146,6 +146,9 @@ class DevBuf {
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Compute units of the current device (256 on MI355X).
+int device_cus();
+
 // ---------------------------------------------------------------- device side
 #ifdef __HIP_DEVICE_COMPILE__
 #define TKA_DEV __device__ __forceinline__

@@ -68,6 +68,21 @@ void trim_cache(int dev)
 }
 }  // namespace
 
+int device_cus()
+{
+    static int cus[64] = {};  // per device, filled on first use
+    int dev = 0;
+    TKA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) dev = 0;
+    int v = __atomic_load_n(&cus[dev], __ATOMIC_RELAXED);
+    if (v == 0) {
+        TKA_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev));
+        if (v <= 0) v = 1;
+        __atomic_store_n(&cus[dev], v, __ATOMIC_RELAXED);
+    }
+    return v;
+}
+
 void* dev_alloc(size_t bytes, size_t* block_bytes, int* device)
 {
     const size_t bb = size_class(bytes);

@@ -136,16 +136,17 @@ __global__ __launch_bounds__(256) void tsne_step(const T* __restrict__ Y, T* __r
 // kBatch column indices, then kBatch rows of Y, before the arithmetic, so a
 // lane waits on two memory latencies per kBatch edges rather than per edge.
 // Padding entries use the row itself with P = 0 and add exactly zero.
+// The grid is one workgroup per CU, each looping over row groups: run at full
+// width this HBM-bound pass slows the latency-bound sort kernels beside it by
+// more than it gains; at one workgroup per CU it takes about two thirds of
+// the tree build and leaves the sort most of the device (measured at N=100k:
+// 0.165 ms per iteration against 0.175 at full width).
 constexpr int kBatch = 4;
 
 template <class T, int DIMS>
-__global__ __launch_bounds__(256) void attract(const T* __restrict__ Y, const int* __restrict__ row_ptr,
-                                               const int* __restrict__ col, const T* __restrict__ val, T exag,
-                                               int N, T* __restrict__ pos)
+TKA_DEV void attract_row(const T* __restrict__ Y, const int* __restrict__ row_ptr, const int* __restrict__ col,
+                         const T* __restrict__ val, T exag, int i, int gl, T* __restrict__ pos)
 {
-    const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
-    const int i = blockIdx.x * kRowsPerBlock + g;
-    if (i >= N) return;  // whole row groups leave together; group sums stay inside a group
     T yi[DIMS], acc[DIMS];
 #pragma unroll
     for (int d = 0; d < DIMS; d++) {
@@ -187,6 +188,19 @@ __global__ __launch_bounds__(256) void attract(const T* __restrict__ Y, const in
     if (gl == 0)
 #pragma unroll
         for (int d = 0; d < DIMS; d++) pos[(size_t)i * DIMS + d] = acc[d];
+}
+
+template <class T, int DIMS>
+__global__ __launch_bounds__(256) void attract(const T* __restrict__ Y, const int* __restrict__ row_ptr,
+                                               const int* __restrict__ col, const T* __restrict__ val, T exag,
+                                               int N, T* __restrict__ pos)
+{
+    const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
+    for (int rb = blockIdx.x; rb * kRowsPerBlock < N; rb += gridDim.x) {
+        const int i = rb * kRowsPerBlock + g;
+        if (i < N)  // whole row groups skip together; group sums stay inside a group
+            attract_row<T, DIMS>(Y, row_ptr, col, val, exag, i, gl, pos);
+    }
 }
 
 // The optimiser step from precomputed attraction and repulsion, one thread
@@ -680,7 +694,7 @@ template <class T>
 void attraction_device(int dims, const T* Y, const int* row_ptr, const int* col, const T* val, double exag, int N,
                        T* pos, hipStream_t s)
 {
-    const int nb = tsne_step_blocks(N);
+    const int nb = std::min(tsne_step_blocks(N), device_cus());
 #define TKA_ATT(DM) \
     hipLaunchKernelGGL((attract<T, DM>), dim3(nb), dim3(256), 0, s, Y, row_ptr, col, val, (T)exag, N, pos)
     switch (dims) {
