@@ -21,6 +21,7 @@
 #include "knn.hpp"
 #include "radix_sort.hpp"
 
+#include <type_traits>
 #include <vector>
 
 namespace tka {
@@ -710,14 +711,21 @@ __global__ __launch_bounds__(256) void prep_fp8(const double* __restrict__ X, co
 // copy of chunk c+1 running while chunk c is multiplied and filtered.  The
 // odd row stride (2*NV + 1 vectors) spreads consecutive rows over the LDS
 // banks.
-constexpr int kNVLimit = 20;  // widest operand row (vectors per half) kept in registers
+constexpr int kNVLimit = 20;       // widest row (vectors per half) streamed as whole rows
+constexpr int kNVSplitLimit = 28;  // widest row kept in registers, streamed in two k-stages
 
-// The register tiles' double-buffered chunks plus the kernel's own LDS
-// (row counters and thresholds) must fit the 160 KiB of a CU.
-inline bool reg_tiles_fit(int NV, size_t vec_bytes)
+// Rows wider than kNVLimit are streamed in two k-stages: each chunk's first
+// split_nv1(NV) vectors per half, then the rest, through the same two LDS
+// buffers, so the buffers hold about half a row each.
+inline int split_nv1(int NV) { return NV <= kNVLimit ? NV : (NV + 1) / 2; }
+
+// The register tiles' double-buffered chunks (or chunk stages) plus the
+// kernel's own LDS (row counters and thresholds) must fit the 160 KiB of a CU.
+inline bool reg_tiles_fit(int NV, size_t vec_bytes, bool allow_split = false)
 {
-    const size_t dyn = 2 * (size_t)kCB * (2 * NV + 1) * vec_bytes;
-    return NV <= kNVLimit && dyn + 256 * 8 <= 160 * 1024;
+    if (NV > (allow_split ? kNVSplitLimit : kNVLimit)) return false;
+    const size_t dyn = 2 * (size_t)kCB * (2 * split_nv1(NV) + 1) * vec_bytes;
+    return dyn + 256 * 8 <= 160 * 1024;
 }
 
 struct OpBF16 {  // v_mfma_f32_32x32x16_bf16: one k-step of 16 per vector pair
@@ -773,6 +781,54 @@ TKA_DEV void dma_to_lds(const unsigned char* src, unsigned char* dst, int bytes,
         __builtin_amdgcn_global_load_lds(src + (size_t)e * 1024 + 16 * lane, (lds_void*)(dst + e * 1024), 16, 0, 0);
 }
 
+// One k-stage of a chunk into LDS: kCB rows of RSS = 2*NVS + 1 vectors, the
+// vectors [G0, G0 + NVS) of each half of the global rows (stride RS = 2*NV +
+// 1) and their pad vector.  Each lane of a direct LDS load names its own
+// source, so the stage image is gathered in one pass.
+template <class V, int NV, int NVS, int G0, int WAVES>
+TKA_DEV void dma_stage_to_lds(const V* rows, unsigned char* dst)
+{
+    constexpr int RS = 2 * NV + 1, RSS = 2 * NVS + 1;
+    static_assert(sizeof(V) == 16, "one vector per lane per direct load");
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int e = w; e < kCB * RSS / 64; e += WAVES) {
+        const int v = e * 64 + lane, r = v / RSS, k = v % RSS;
+        const int gk = k < NVS ? G0 + k : (k < 2 * NVS ? NV + G0 + (k - NVS) : 2 * NV);
+        __builtin_amdgcn_global_load_lds(rows + (size_t)r * RS + gk, (lds_void*)(dst + e * 1024), 16, 0, 0);
+    }
+}
+
+// acc[u][t] += the products of k-steps [GQ, GQ + NVS) of the chunk image sA
+// (row stride RSS, NVS vectors per half) with the wave's query operand.  Step
+// g+1's four candidate fragments are read from LDS before step g's MFMAs
+// issue, so the reads complete under them (the scheduling barriers keep the
+// compiler from pairing each read with its MFMA behind a full wait).
+template <class Op, int NVS, int RSS, int GQ, int QT, int NV>
+TKA_DEV void tile_mma(const typename Op::V* sA, const typename Op::V (&bq)[QT][NV], f32x16 (&acc)[QT][4])
+{
+    typedef typename Op::V V;
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    const V* rowp = sA + (lane & 31) * RSS + half * NVS;
+    V af[4], an[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) af[t] = rowp[32 * t * RSS];
+#pragma unroll
+    for (int g = 0; g < NVS; g++) {
+        if (g + 1 < NVS) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) an[t] = rowp[32 * t * RSS + g + 1];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int u = 0; u < QT; u++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) acc[u][t] = Op::mma(af[t], bq[u][GQ + g], acc[u][t]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 4; t++) af[t] = an[t];
+    }
+}
+
 // kMode 0: the value is the accumulator (BF16: ||c||^2 folded into the operands)
 //       1: ||c||^2 - 2 acc (FP32)
 //       2: ||c||^2 - 2 s_q s_c acc (FP8 with per-row scales)
@@ -781,19 +837,26 @@ TKA_DEV void dma_to_lds(const unsigned char* src, unsigned char* dst, int bytes,
 // MFMAs; every chunk copied into LDS serves all the block's queries.  Two
 // waves per SIMD (WAVES = 8, or two blocks per CU) let one wave's filter run
 // under another's MFMAs.
-template <class Op, int kMode, int NV, int QT, int WAVES>
+// NV1 < NV: rows streamed in two k-stages of NV1 and NV - NV1 vectors per half.
+template <class Op, int kMode, int NV, int QT, int WAVES, int NV1 = NV>
 __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V* __restrict__ A,
                                                             const typename Op::V* __restrict__ B, TileArgs a)
 {
     typedef typename Op::V V;
+    constexpr bool kSplit = NV1 < NV;
+    constexpr int NV2 = NV - NV1;
+    static_assert(!kSplit || NV2 <= NV1, "the first stage is the wider one");
     constexpr int QB = 32 * WAVES * QT;  // query rows per block
-    extern __shared__ __align__(16) unsigned char dyn_lds[];  // two chunks of kCB rows x (2*NV + 1) vectors
+    // two chunks of kCB rows x (2*NV + 1) vectors, or two stage images of
+    // kCB rows x (2*NV1 + 1) vectors
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
     __shared__ int s_cnt[QB];
     __shared__ float s_tau[QB];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * QB;
-    const int RS = 2 * NV + 1;
-    const int chunk_bytes = kCB * RS * (int)sizeof(V);
+    constexpr int RS = 2 * NV + 1;
+    constexpr int RS1 = 2 * NV1 + 1, RS2 = 2 * NV2 + 1;
+    const int chunk_bytes = kCB * RS1 * (int)sizeof(V);  // one LDS buffer
     for (int i = tid; i < QB; i += 64 * WAVES) {
         s_cnt[i] = 0;
         s_tau[i] = HUGE_VALF;
@@ -816,43 +879,40 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
     const int nch = (a.N + kCB - 1) / kCB;
     // the chunk holding the block's first query rows (rows are in locality order)
     const ChunkOrder order(a.rotate ? (int)(((size_t)blockIdx.x * QB / kCB) % nch) : -1, nch);
-    dma_to_lds(src + (size_t)order.at(0) * kCB * RS * sizeof(V), dyn_lds, chunk_bytes, WAVES);
+    if constexpr (kSplit)
+        dma_stage_to_lds<V, NV, NV1, 0, WAVES>(A + (size_t)order.at(0) * kCB * RS, dyn_lds);
+    else
+        dma_to_lds(src + (size_t)order.at(0) * kCB * RS * sizeof(V), dyn_lds, chunk_bytes, WAVES);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int c = 0; c < nch; c++) {
         const int cb = order.at(c) * kCB;
-        unsigned char* cur = dyn_lds + (c & 1) * chunk_bytes;
-        if (c + 1 < nch)
-            dma_to_lds(src + (size_t)order.at(c + 1) * kCB * RS * sizeof(V), dyn_lds + ((c + 1) & 1) * chunk_bytes,
-                       chunk_bytes, WAVES);
-        const V* sA = reinterpret_cast<const V*>(cur);
         f32x16 acc[QT][4];
 #pragma unroll
         for (int u = 0; u < QT; u++)
 #pragma unroll
             for (int t = 0; t < 4; t++) acc[u][t] = f32x16{};
-        const V* rowp = sA + (lane & 31) * RS + half * NV;
-        // k-step g+1's four candidate fragments are read from LDS before
-        // step g's MFMAs issue, so the reads complete under them (the
-        // scheduling barriers keep the compiler from pairing each read with
-        // its MFMA behind a full wait)
-        V af[4], an[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) af[t] = rowp[32 * t * RS];
-#pragma unroll
-        for (int g = 0; g < NV; g++) {
-            if (g + 1 < NV) {
-#pragma unroll
-                for (int t = 0; t < 4; t++) an[t] = rowp[32 * t * RS + g + 1];
-            }
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int u = 0; u < QT; u++)
-#pragma unroll
-                for (int t = 0; t < 4; t++) acc[u][t] = Op::mma(af[t], bq[u][g], acc[u][t]);
-            __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int t = 0; t < 4; t++) af[t] = an[t];
+        const V* sA;  // image holding the pad vectors, for the filter
+        int sRS;
+        if constexpr (kSplit) {
+            // stage 1 is in buffer 0; stage 2 goes to buffer 1 meanwhile
+            unsigned char* b1 = dyn_lds + chunk_bytes;
+            dma_stage_to_lds<V, NV, NV2, NV1, WAVES>(A + (size_t)cb * RS, b1);
+            tile_mma<Op, NV1, RS1, 0, QT, NV>(reinterpret_cast<const V*>(dyn_lds), bq, acc);
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();  // stage 2 landed; buffer 0 is free
+            if (c + 1 < nch) dma_stage_to_lds<V, NV, NV1, 0, WAVES>(A + (size_t)order.at(c + 1) * kCB * RS, dyn_lds);
+            tile_mma<Op, NV2, RS2, NV1, QT, NV>(reinterpret_cast<const V*>(b1), bq, acc);
+            sA = reinterpret_cast<const V*>(b1);
+            sRS = RS2;
+        } else {
+            unsigned char* cur = dyn_lds + (c & 1) * chunk_bytes;
+            if (c + 1 < nch)
+                dma_to_lds(src + (size_t)order.at(c + 1) * kCB * RS * sizeof(V),
+                           dyn_lds + ((c + 1) & 1) * chunk_bytes, chunk_bytes, WAVES);
+            tile_mma<Op, NV, RS, 0, QT, NV>(reinterpret_cast<const V*>(cur), bq, acc);
+            sA = reinterpret_cast<const V*>(cur);
+            sRS = RS;
         }
         const bool seeding = c < order.nseed;
 #pragma unroll
@@ -867,7 +927,7 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
                     if (kMode == 0) {
                         v[i] = acc[u][t][i];
                     } else {
-                        const float* pad = reinterpret_cast<const float*>(sA + m * RS + 2 * NV);
+                        const float* pad = reinterpret_cast<const float*>(sA + m * sRS + sRS - 1);
                         v[i] = kMode == 1 ? fmaf(-2.f, acc[u][t][i], pad[0])
                                           : fmaf(-2.f * sq[u] * pad[1], acc[u][t][i], pad[0]);
                     }
@@ -885,8 +945,8 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
                 if (q[u] >= 0) s_cnt[sl[u]] = order.nseed * kCB;
         }
         if (c >= order.nseed - 1) compact_if_needed<32 * QT>(w, a, slot0, s_cnt, s_tau);
-        __builtin_amdgcn_s_waitcnt(0);  // this wave's copy of the next chunk has landed
-        __syncthreads();                // every wave's has, and the current chunk is free
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's copy of the next chunk (stage) has landed
+        __syncthreads();                // every wave's has, and the buffer just used is free
     }
 #pragma unroll
     for (int u = 0; u < QT; u++) {
@@ -1296,7 +1356,8 @@ template <class Op, int kMode, int NV, int QT, int WAVES>
 void launch_tiles_reg(const void* A, const void* B, dim3 grid, size_t lds, const TileArgs& ta, hipStream_t s)
 {
     typedef typename Op::V V;
-    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, NV, QT, WAVES>), grid, dim3(64 * WAVES), lds, s, (const V*)A,
+    constexpr int NV1 = NV <= kNVLimit ? NV : (NV + 1) / 2;  // split_nv1
+    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, NV, QT, WAVES, NV1>), grid, dim3(64 * WAVES), lds, s, (const V*)A,
                        (const V*)B, ta);
 }
 
@@ -1304,7 +1365,9 @@ template <class Op, int kMode, int QT, int WAVES>
 void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, TileRun& tr, MemBudget* mb, hipStream_t s)
 {
     typedef typename Op::V V;
-    require(NV >= 1 && NV <= kNVLimit, TKA_E_INTERNAL, "register tiles: bad operand width");
+    constexpr bool kSplits = std::is_same<Op, OpBF16>::value;  // two-stage rows: BF16 only
+    require(NV >= 1 && NV <= (kSplits ? kNVSplitLimit : kNVLimit), TKA_E_INTERNAL,
+            "register tiles: bad operand width");
     constexpr int QB = 32 * WAVES * QT;
     const int nslots = (int)ceil_div(N, QB) * QB;
     tr.buf.alloc((size_t)nslots * kCap, mb);
@@ -1312,7 +1375,7 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, Tile
     tr.tau.alloc(nslots, mb);
     tr.fail.alloc(nslots, mb);
     TileArgs ta{N, kprime, 1, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
-    const size_t lds = 2 * (size_t)kCB * (2 * NV + 1) * sizeof(V);
+    const size_t lds = 2 * (size_t)kCB * (2 * split_nv1(NV) + 1) * sizeof(V);
     const dim3 grid(ceil_div(N, QB));
 #define TKA_W(W) \
     case W: launch_tiles_reg<Op, kMode, W, QT, WAVES>(A, B, grid, lds, ta, s); break
@@ -1320,6 +1383,11 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, Tile
         TKA_W(1); TKA_W(2); TKA_W(3); TKA_W(4); TKA_W(5); TKA_W(6); TKA_W(7); TKA_W(8); TKA_W(9); TKA_W(10);
         TKA_W(11); TKA_W(12); TKA_W(13); TKA_W(14); TKA_W(15); TKA_W(16); TKA_W(17); TKA_W(18); TKA_W(19);
         TKA_W(20);
+    }
+    if constexpr (kSplits) {
+        switch (NV) {
+            TKA_W(21); TKA_W(22); TKA_W(23); TKA_W(24); TKA_W(25); TKA_W(26); TKA_W(27); TKA_W(28);
+        }
     }
 #undef TKA_W
     TKA_LAUNCH_CHECK();
@@ -1417,7 +1485,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     if (dtype == TKA_DIST_BF16) {
         Kp = ((3 * D + 2) + 15) / 16 * 16;
         const int NV = Kp / 16;
-        if (reg_tiles_fit(NV, sizeof(bf16x8))) {
+        if (reg_tiles_fit(NV, sizeof(bf16x8), true)) {
             locality();
             const size_t rows = (size_t)Npad * 8 * (2 * NV + 1);
             A.alloc(rows, mb);
