@@ -49,7 +49,7 @@ FLOPS_PER_INTERACTION = 14   # dx, dy, D, q = 1/(1+D), z += c q, m = c q^2, f +=
 # by what the counter reads for a plain 256 MiB device copy in the same run
 # (FETCH_SIZE reads 131093 KiB for 262144 KiB moved: x2.0; WRITE_SIZE reads
 # 262144 KiB: x1.0).  Summary in profiles/pmc_traffic_bh_traverse_swissroll_n100k.md.
-TRAVERSE_FETCH_KIB = 2211.2
+TRAVERSE_FETCH_KIB = 2226.2
 TRAVERSE_WRITE_KIB = 2593.0
 FETCH_SCALE = 2.0
 WRITE_SCALE = 1.0
@@ -293,7 +293,9 @@ def main():
                "update_ms": round(upd_ms, 5),
                "attraction_ms": round(att_ms, 5), "attraction_concurrent_with": "tree build (sort and node kernels)",
                "attraction_GBps": round(att_gbs, 1),
-               "attraction_hbm_frac": round(att_gbs / PEAK_HBM_GBS, 4)}
+               "attraction_hbm_frac": round(att_gbs / PEAK_HBM_GBS, 4),
+               "attraction_note": "one workgroup per CU by design, finishing inside the tree build; "
+                                  "at full width it streams faster but slows the sort kernels beside it"}
 
     # ---- accuracy against FP64 at this size
     accuracy = None
