@@ -36,10 +36,12 @@ def test_normalize_bit_exact(engine, oracle, shape):
 
 # ---------------------------------------------------------------- kNN
 @pytest.mark.parametrize("dtype", ["fp32", "bf16"])
-# d = 128: FP32 operand rows of 17 vectors (D+1 k-values); d = 93: BF16 rows
-# of 18 vectors (3D+2 k-values) -- the widest register-tile shapes
+# d = 128: FP32 operand rows of 17 vectors (D+1 k-values), BF16 rows of 25
+# vectors (3D+2 k-values) streamed in two k-stages; d = 93: BF16 rows of 18
+# vectors; d = 107 and 148: the narrowest (21) and widest (28) two-stage BF16
+# rows; d = 784: the LDS-staged kernels
 @pytest.mark.parametrize("n,d,K", [(700, 20, 30), (1500, 50, 90), (300, 784, 20), (129, 3, 128), (600, 128, 40),
-                                   (500, 93, 30)])
+                                   (500, 93, 30), (400, 107, 25), (350, 148, 20)])
 def test_knn_bit_exact(engine, oracle, dtype, n, d, K):
     X = oracle.normalize(mixture(n, d))
     idx, dist, fails = engine.knn(X, K, dist_dtype=dtype)
