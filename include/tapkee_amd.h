@@ -241,7 +241,9 @@ TKA_API tka_status tka_update(const double* dY, double* uY, double* gains, doubl
 /* Barnes-Hut tree export (pre-order, compressed): node count in *n_nodes;
  * per node level, count, leaf point (-1 for internal), skip index and centre
  * of mass.  Arrays need capacity 2*N.  root_box: centre x, y, half-widths
- * hw, hh (quadtree.hpp:99-134). */
+ * hw, hh (quadtree.hpp:99-134).  The tree is built twice, the second time
+ * with the radix-sort width the first build's key statistics select, and
+ * TKA_E_INTERNAL is returned unless both agree node for node. */
 TKA_API tka_status tka_bh_tree(const double* Y, int32_t N, int32_t precision, int32_t* n_nodes, int32_t* level,
                                int32_t* count, int32_t* leaf_point, int32_t* skip, double* com_x, double* com_y,
                                double* root_box);

@@ -16,8 +16,9 @@
 //          ceil(t * 2^32) - 1, so coordinates on a split line go west/north;
 //          its top 16 bits equal ceil(t * 2^16) - 1, so the key's high word
 //          is the 16+16 bit key of the level-16 cell.
-//   sort   stable LSD radix sort of (top 8*passes key bits, point), 3
-//          passes (levels 1-12) up to 2^22 points, else 4.
+//   sort   stable LSD radix sort of (top 8*passes key bits, point): 3
+//          passes (levels 1-12), or 4 when the last build saw many sorted
+//          neighbours with equal top 24 bits, and always beyond 2^22 points.
 //   fix    runs of equal top bits (points within 2^-12 of the box of each
 //          other, a few at most in t-SNE embeddings) are put in full-key
 //          order: short runs by a stable insertion sort, long ones by a
@@ -409,10 +410,12 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
                                                 const typename Vec2<T>::type* __restrict__ ys, int N,
                                                 unsigned long long* __restrict__ mask, int* __restrict__ node_a,
                                                 int* __restrict__ node_b, int* __restrict__ node_lvl,
-                                                uint32_t* __restrict__ tile_cnt, double* __restrict__ part)
+                                                uint32_t* __restrict__ tile_cnt, double* __restrict__ part,
+                                                uint32_t* __restrict__ eq24)
 {
     __shared__ uint64_t win[kPairWin];
     __shared__ uint32_t own_cnt[2];  // owners with a in tile blockIdx.x - 1, blockIdx.x
+    __shared__ uint32_t eq_cnt;      // pairs whose keys share their top 24 bits
     __shared__ double sh[4];
     const int w0 = (int)blockIdx.x * 256 - 256;
     for (int t = threadIdx.x; t < kPairWin; t += 256) {
@@ -420,6 +423,7 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
         win[t] = (g >= 0 && g < N) ? keys[g] : 0ull;
     }
     if (threadIdx.x < 2) own_cnt[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) eq_cnt = 0u;
     __syncthreads();
     // the window's valid keys: [wl, wh)
     const int wl = max(w0, 0), wh = min(w0 + kPairWin, N);
@@ -427,6 +431,10 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
     const bool act = i < N - 1;  // pair (i, i+1); every lane stays for the wave's searches
     const uint64_t ki = act ? win[i - w0] : 0ull, kj = act ? win[i + 1 - w0] : 0ull;
     const uint64_t x = ki ^ kj;
+    {
+        const uint64_t e = __ballot(act && (x >> 40) == 0ull);
+        if ((threadIdx.x & 63) == 0 && e != 0ull) atomicAdd(&eq_cnt, (uint32_t)__popcll(e));
+    }
     const int lvl = (x == 0ull) ? kB : (__clzll((long long)x) >> 1);
     const int lowbits = 2 * (kB - lvl);  // 0..64
     const uint64_t lowmask = (lowbits >= 64) ? ~0ull : ((1ull << lowbits) - 1ull);
@@ -484,6 +492,7 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
         part[2 * (size_t)blockIdx.x + 1] = sy;
         if (own_cnt[1]) atomicAdd(&tile_cnt[blockIdx.x], own_cnt[1]);
         if (own_cnt[0]) atomicAdd(&tile_cnt[blockIdx.x - 1], own_cnt[0]);
+        eq24[blockIdx.x] = eq_cnt;
     }
 }
 
@@ -562,8 +571,16 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
                                                const typename Vec2<T>::type* __restrict__ ys,
                                                const uint32_t* __restrict__ perm, int N,
                                                const uint32_t* __restrict__ pre, const double* __restrict__ psx,
-                                               const double* __restrict__ psy, BhNode<T>* __restrict__ nodes)
+                                               const double* __restrict__ psy, BhNode<T>* __restrict__ nodes,
+                                               const uint32_t* __restrict__ eq24, uint32_t* __restrict__ eq24_host)
 {
+    if (blockIdx.x == 0) {  // the pairs kernel's count, posted to the host for the next build's pass count
+        __shared__ double sh[4];
+        double c = 0.0;
+        for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) c += (double)eq24[b];
+        c = block_sum(c, sh);
+        if (threadIdx.x == 0) __hip_atomic_store(eq24_host, (uint32_t)c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= N) return;
     {  // leaf j
@@ -727,6 +744,13 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     perm_alt.alloc(n, mb);
     kfull.alloc(n, mb);
     passes = n <= (1 << 22) ? 3 : 4;
+    eq24.alloc(ceil_div(n, 256), mb);
+    if (!eq24_host) {
+        void* h = nullptr;
+        TKA_HIP(hipHostMalloc(&h, sizeof(uint32_t), hipHostMallocDefault));
+        eq24_host = static_cast<uint32_t*>(h);
+    }
+    __atomic_store_n(eq24_host, 0u, __ATOMIC_RELAXED);
     k64.alloc(n, mb);
     p2.alloc(n, mb);
     sort_scratch.alloc(radix_sort_scratch_words(n), mb);
@@ -757,6 +781,11 @@ template <class T>
 void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s)
 {
     require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "tree storage too small");
+    if (N <= (1 << 22)) {  // hysteresis between 1/16 and 1/8 of the points
+        const uint32_t eq = __atomic_load_n(tr.eq24_host, __ATOMIC_RELAXED);
+        if (tr.passes == 3 && eq > (uint32_t)N / 8) tr.passes = 4;
+        else if (tr.passes == 4 && eq < (uint32_t)N / 16) tr.passes = 3;
+    }
     const int tiles = (int)radix_tiles((size_t)N);
     hipLaunchKernelGGL(bh_keys<T>, dim3(tiles), dim3(kRsBlock), 0, s, Y, N, box_part, nbox, tr.root.get(),
                        tr.passes, tr.keys.get(), tr.kfull.get(), tr.perm.get(), tr.mask.get(), tr.tile_cnt.get(),
@@ -786,13 +815,15 @@ void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s)
     tr.sorted_perm = p;
     static_assert(kTreeScanTile == 256, "one scan tile per pairs workgroup");
     hipLaunchKernelGGL(bh_pairs<T>, dim3(g), dim3(256), 0, s, tr.k64.get(), tr.ys.get(), N, tr.mask.get(),
-                       tr.node_a.get(), tr.node_b.get(), tr.node_lvl.get(), tr.tile_cnt.get(), tr.part.get());
+                       tr.node_a.get(), tr.node_b.get(), tr.node_lvl.get(), tr.tile_cnt.get(), tr.part.get(),
+                       tr.eq24.get());
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(bh_scan_write<T>, dim3(g), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N,
                        tr.tile_cnt.get(), tr.part.get(), tr.pre.get(), tr.sx.get(), tr.sy.get());
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(bh_emit<T>, dim3(g), dim3(256), 0, s, tr.mask.get(), tr.node_a.get(), tr.node_b.get(),
-                       tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(), tr.nodes.get());
+                       tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(), tr.nodes.get(),
+                       tr.eq24.get(), tr.eq24_host);
     TKA_LAUNCH_CHECK();
 }
 

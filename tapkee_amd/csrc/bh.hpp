@@ -114,7 +114,15 @@ struct alignas(4 * sizeof(T)) BhNode {
 template <class T>
 struct BhTree {
     int N = 0;
-    int passes = 4;              // radix passes: the sort orders the top 8*passes key bits
+    // Radix passes: the sort orders the top 8*passes key bits, and bh_fix
+    // sorts each run of equal top bits by full key.  3 passes (levels 1-12)
+    // while few sorted neighbours share their top 24 bits; 4 when many do
+    // (tight clusters, e.g. during early exaggeration), so bh_fix's runs stay
+    // short.  The choice reads a count that the previous build posted to
+    // pinned host memory (no synchronisation; results do not depend on it).
+    int passes = 3;
+    uint32_t* eq24_host = nullptr;  // pinned: adjacent sorted keys with equal top 24 bits, last build
+    DevBuf<uint32_t> eq24;          // the same count per pairs workgroup
     DevBuf<uint32_t> keys, keys_alt, perm, perm_alt, sort_scratch;  // top key bits, radix-sorted
     DevBuf<uint64_t> kfull;      // full key per point
     DevBuf<uint64_t> k64;        // full keys, sorted
@@ -131,6 +139,13 @@ struct BhTree {
     DevBuf<RootBox> root;
     const uint32_t* sorted_perm = nullptr;  // valid after build
     void alloc(int n, MemBudget* mb);
+    BhTree() = default;
+    BhTree(const BhTree&) = delete;
+    BhTree& operator=(const BhTree&) = delete;
+    ~BhTree()
+    {
+        if (eq24_host) (void)hipHostFree(eq24_host);
+    }
 };
 
 // Bounding-box partials of Y (N x 2, T) -> part (kBoxStride doubles per block).

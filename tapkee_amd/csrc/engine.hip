@@ -1258,20 +1258,35 @@ TKA_API tka_status tka_bh_tree(const double* Y, int32_t N, int32_t precision, in
             DevBuf<double> tmp;
             DevBuf<T> y((size_t)N * 2, &mb);
             upload(tmp, Y, (size_t)N * 2, s);
-            cast_device<double, T>(tmp.get(), y.get(), (size_t)N * 2, s);
             DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
-            bbox_partials_device<T>(y.get(), N, box.get(), s);
             BhTree<T> tree;
             tree.alloc(N, &mb);
-            bh_build<T>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
+            // Two builds from the same input (the key kernel centres Y in
+            // place): the second sorts as many key bits as the first one's
+            // count of equal neighbours asks for (3 or 4 radix passes), and
+            // the trees must agree node for node.
+            std::vector<BhNode<T>> first;
             uint32_t total = 0;
             RootBox rb;
-            download(&total, tree.pre.get() + N, 1, s);
-            download(&rb, tree.root.get(), 1, s);
-            st.sync();
-            std::vector<BhNode<T>> nodes(total);
-            download(nodes.data(), tree.nodes.get(), total, s);
-            st.sync();
+            for (int b = 0; b < 2; b++) {
+                cast_device<double, T>(tmp.get(), y.get(), (size_t)N * 2, s);
+                bbox_partials_device<T>(y.get(), N, box.get(), s);
+                bh_build<T>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
+                download(&total, tree.pre.get() + N, 1, s);
+                download(&rb, tree.root.get(), 1, s);
+                st.sync();
+                std::vector<BhNode<T>> nodes(total);
+                download(nodes.data(), tree.nodes.get(), total, s);
+                st.sync();
+                if (b == 0) {
+                    first.swap(nodes);
+                } else {
+                    require(nodes.size() == first.size() &&
+                                std::memcmp(nodes.data(), first.data(), nodes.size() * sizeof(BhNode<T>)) == 0,
+                            TKA_E_INTERNAL, "Barnes-Hut trees built with different radix widths differ");
+                }
+            }
+            std::vector<BhNode<T>>& nodes = first;
             *n_nodes = (int32_t)total;
             for (uint32_t t = 0; t < total; t++) {
                 const bool leaf = (nodes[t].link & kLeafBit) != 0;
