@@ -1080,6 +1080,24 @@ TKA_DEV double exact_sqdist(const double* __restrict__ xq, const double* __restr
 {
     double dd = 0.0;
     int d = 0;
+    if ((((uintptr_t)xq | (uintptr_t)xj) & 15) == 0) {  // 16-byte aligned rows: pairs of values per load,
+                                                         // sums still in index order
+        for (; d + 8 <= D; d += 8) {
+            double2 a[4], b[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                a[k] = *reinterpret_cast<const double2*>(xq + d + 2 * k);
+                b[k] = *reinterpret_cast<const double2*>(xj + d + 2 * k);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const double t0 = dadd_sep(a[k].x, -b[k].x);
+                dd = dadd_sep(dd, dmul_sep(t0, t0));
+                const double t1 = dadd_sep(a[k].y, -b[k].y);
+                dd = dadd_sep(dd, dmul_sep(t1, t1));
+            }
+        }
+    }
     for (; d + 8 <= D; d += 8) {  // loads of a group issued together; sums stay in index order
         double a[8], b[8];
 #pragma unroll
