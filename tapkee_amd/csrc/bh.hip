@@ -611,23 +611,17 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
 }
 
 // ---------------------------------------------------------------- traverse
-// Node t of a block of four consecutive nodes held in registers (a register
-// array indexed by a per-lane value would go to scratch memory).
-template <class T>
-TKA_DEV BhNode<T> pick4(const BhNode<T> (&b)[4], int k)
-{
-    const BhNode<T> lo = (k & 1) ? b[1] : b[0];
-    const BhNode<T> hi = (k & 1) ? b[3] : b[2];
-    return (k & 2) ? hi : lo;
-}
-
 // One thread per point, in Morton order, so the lanes of a wave walk similar
 // paths and their node gathers mostly hit the same cache lines.
 //
-// The walk is a chain of dependent loads; two things shorten it:
-//  - nodes are fetched in aligned blocks of four (one cache line of FP32
-//    nodes): descending and leaf steps go to t+1, which the block usually
-//    holds, so only jumps past accepted cells and block ends wait on memory;
+// The walk is a chain of dependent loads; three things shorten it:
+//  - the next node's index is known once the acceptance test is done, so its
+//    load (one 16-byte node) is issued before the current node's division
+//    and accumulation and arrives under them (a block of four nodes per
+//    load, picked from registers, cost more in selects and load issue than
+//    it saved: 57 against 44 us at N=100k);
+//  - each workgroup first pulls its slice of its XCD's node range into L2
+//    (below);
 //  - workgroups are dispatched to the 8 XCDs round-robin, so workgroup b is
 //    given the points of logical block x*q + min(x, r) + k (x = b % 8,
 //    k = b / 8, grid = 8q + r): every XCD walks one contiguous Morton range,
@@ -677,11 +671,9 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
     }
     T fx = 0, fy = 0, z = 0;
     if (active) {
-        // the node buffer is padded to whole blocks of four
-        BhNode<T> blk[4] = {nodes[0], nodes[1], nodes[2], nodes[3]};
-        int t = 0, base = 0;
+        BhNode<T> nd = nodes[0];
+        int t = 0;
         while (t < total) {
-            const BhNode<T> nd = pick4(blk, t & 3);
             const T dx = px - nd.x, dy = py - nd.y;
             const T D = dx * dx + dy * dy;
             bool acc;
@@ -696,18 +688,13 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
                 const int sk = (int)nd.link;
                 next = (acc && sk > t) ? sk : t + 1;  // skip pointers point forward; never loop
             }
-            // the next node's block is requested before this node's arithmetic
-            if ((next & ~3) != base && next < total) {
-                base = next & ~3;
-                const BhNode<T>* b4 = nodes + base;
-                blk[0] = b4[0];
-                blk[1] = b4[1];
-                blk[2] = b4[2];
-                blk[3] = b4[3];
-            }
+            // the next node is requested as soon as its index is known, so
+            // its load runs under this node's division and accumulation
+            const uint32_t count = nd.cl & ((1u << kCountBits) - 1u);
+            if (next < total) nd = nodes[next];
             if (acc) {  // a leaf's count is 1: count*q and count*q*q are exactly q and q*q
                 const T q = (T)1 / ((T)1 + D);
-                const T cq = (T)(nd.cl & ((1u << kCountBits) - 1u)) * q;
+                const T cq = (T)count * q;
                 z += cq;
                 const T m = cq * q;
                 fx += m * dx;
@@ -766,7 +753,7 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     sy.alloc(n + 1, mb);
     part.alloc(2 * (size_t)ceil_div(n, kTreeScanTile), mb);
     tile_cnt.alloc(ceil_div(n, kTreeScanTile), mb);
-    nodes.alloc(2 * (size_t)n + 4, mb);  // whole blocks of four for the traversal's loads
+    nodes.alloc(2 * (size_t)n, mb);
     root.alloc(1, mb);
 }
 
