@@ -676,18 +676,16 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
         while (t < total) {
             const T dx = px - nd.x, dy = py - nd.y;
             const T D = dx * dx + dy * dy;
-            bool acc;
-            int next;
-            if (nd.link & kLeafBit) {
-                acc = (nd.link & ~kLeafBit) != p;
-                next = t + 1;
-            } else {
-                const int lvl = (int)(nd.cl >> kCountBits);
-                const T size2 = ldexp(size0sq, -2 * lvl);  // (size0 2^-lvl)^2, exact scaling
-                acc = size2 < theta2 * D;
-                const int sk = (int)nd.link;
-                next = (acc && sk > t) ? sk : t + 1;  // skip pointers point forward; never loop
-            }
+            // leaf or cell by selects, not a branch (lanes at leaves and at
+            // cells share every instruction; a branch costs more than the
+            // cell test a leaf does not need)
+            const bool leaf = (nd.link & kLeafBit) != 0u;
+            const int lvl = (int)(nd.cl >> kCountBits);
+            const T size2 = ldexp(size0sq, -2 * lvl);  // (size0 2^-lvl)^2, exact scaling
+            const bool cell_acc = size2 < theta2 * D;
+            const bool acc = leaf ? (nd.link & ~kLeafBit) != p : cell_acc;
+            const int sk = (int)nd.link;
+            const int next = (!leaf && cell_acc && sk > t) ? sk : t + 1;  // skip pointers point forward
             // the next node is requested as soon as its index is known, so
             // its load runs under this node's division and accumulation
             const uint32_t count = nd.cl & ((1u << kCountBits) - 1u);
