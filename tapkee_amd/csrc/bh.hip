@@ -140,11 +140,11 @@ __global__ __launch_bounds__(kRsBlock) void bh_keys(T* __restrict__ Y, int N, co
     const RootBox rb = reduce_root_box<kRsBlock>(box_part, nbox, N);
     if (blockIdx.x == 0 && threadIdx.x == 0) *root = rb;
     const double mx = rb.mean_x, my = rb.mean_y, hw = rb.hw, hh = rb.hh;
-    uint32_t hk[kRsItems];
-    bool ok[kRsItems];
-    const int base = blockIdx.x * kRsBlock * kRsItems + threadIdx.x;
+    uint32_t hk[kRsItemsCounted];
+    bool ok[kRsItemsCounted];
+    const int base = blockIdx.x * kRsBlock * kRsItemsCounted + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
+    for (int k = 0; k < kRsItemsCounted; k++) {
         const int i = base + k * kRsBlock;
         ok[k] = i < N;
         hk[k] = 0u;
@@ -738,7 +738,7 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     __atomic_store_n(eq24_host, 0u, __ATOMIC_RELAXED);
     k64.alloc(n, mb);
     p2.alloc(n, mb);
-    sort_scratch.alloc(radix_sort_scratch_words(n), mb);
+    sort_scratch.alloc(radix_sort_scratch_words(n, kRsItemsCounted), mb);
     // on the stream the buffer was allocated on (ordered after the allocation)
     TKA_HIP(hipMemsetAsync(sort_scratch.get(), 0, sort_scratch.size() * sizeof(uint32_t), alloc_stream()));
     mask.alloc(n, mb);
@@ -771,7 +771,7 @@ void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr,
         if (tr.passes == 3 && eq > (uint32_t)N / 8) tr.passes = 4;
         else if (tr.passes == 4 && eq < (uint32_t)N / 16) tr.passes = 3;
     }
-    const int tiles = (int)radix_tiles((size_t)N);
+    const int tiles = (int)radix_tiles((size_t)N, kRsItemsCounted);
     hipLaunchKernelGGL(bh_keys<T>, dim3(tiles), dim3(kRsBlock), 0, s, Y, N, box_part, nbox, tr.root.get(),
                        tr.passes, tr.keys.get(), tr.kfull.get(), tr.perm.get(), tr.mask.get(), tr.tile_cnt.get(),
                        rs_layout(tr.sort_scratch.get()), tiles);

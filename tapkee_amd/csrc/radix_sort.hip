@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys,
             ok[k] = i < n;
             key[k] = ok[k] ? keys[i] : K(0);
         }
-        rs_tile_histogram<K>(key, ok, passes, h);
+        rs_tile_histogram<K, kRsItems>(key, ok, passes, h);
     }
     __syncthreads();
     rs_count_flush(passes, L, h);
@@ -271,6 +271,7 @@ bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit
     const int tiles = (int)ceil_div(n, (size_t)kRsBlock * ITEMS);
     const int passes = (end_bit + 7) / 8;
     const RsLayout L = rs_layout(scratch);
+    require(counted || ITEMS == kRsItems, TKA_E_INTERNAL, "radix sort: the counting launch uses kRsItems tiles");
     if (!counted) {
         hipLaunchKernelGGL((rs_count<K>), dim3(std::min(tiles, kRsCountBlocks)), dim3(kRsBlock), 0, s, keys, n, passes, L,
                            tiles);
@@ -292,7 +293,10 @@ bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit
 
 }  // namespace
 
-size_t radix_sort_scratch_words(size_t n) { return kRsHeader + (size_t)kRsMaxPasses * 256 * radix_tiles(n ? n : 1); }
+size_t radix_sort_scratch_words(size_t n, int items)
+{
+    return kRsHeader + (size_t)kRsMaxPasses * 256 * radix_tiles(n ? n : 1, items);
+}
 
 template <class K, class V>
 bool radix_sort_pairs(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch,
@@ -311,7 +315,7 @@ bool radix_sort_pairs_counted(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t
     if (n <= 1 || end_bit <= 0) return false;
     require(n <= kRsMaxItems, TKA_E_UNSUPPORTED, "radix sort: more than 2^30 - 1 items");
     require(end_bit <= 8 * (int)sizeof(K), TKA_E_INTERNAL, "radix sort: end_bit beyond the key width");
-    return sort_impl<K, V, kRsItems>(keys, vals, keys_alt, vals_alt, n, end_bit, scratch, true, stream);
+    return sort_impl<K, V, kRsItemsCounted>(keys, vals, keys_alt, vals_alt, n, end_bit, scratch, true, stream);
 }
 
 template bool radix_sort_pairs_counted<uint32_t, uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, size_t, int,

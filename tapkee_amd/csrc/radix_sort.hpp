@@ -13,11 +13,19 @@ constexpr int kRsMaxPasses = 8;                       // 64-bit keys
 constexpr size_t kRsMaxItems = ((size_t)1 << 30) - 1;  // counts share a word with a 2-bit status flag
 
 constexpr int kRsItems = 8;                           // keys per thread: tile = 2048 keys
-inline size_t radix_tiles(size_t n) { return (n + (size_t)kRsBlock * kRsItems - 1) / ((size_t)kRsBlock * kRsItems); }
+// Keys per thread of the counted sort (radix_sort_pairs_counted): its callers
+// sort ~1e5 keys per launch, where the tiles' serial ranking and not the
+// look-back over more tiles sets the time, so tiles are smaller (1024 keys).
+constexpr int kRsItemsCounted = 4;
+inline size_t radix_tiles(size_t n, int items = kRsItems)
+{
+    return (n + (size_t)kRsBlock * items - 1) / ((size_t)kRsBlock * items);
+}
 
-// Scratch (uint32 words) for n pairs; must be zero-initialised once before
-// the first use (every sort leaves its counters at zero).
-size_t radix_sort_scratch_words(size_t n);
+// Scratch (uint32 words) for n pairs sorted in tiles of kRsBlock x items
+// keys; must be zero-initialised once before the first use (every sort
+// leaves its counters at zero).
+size_t radix_sort_scratch_words(size_t n, int items = kRsItems);
 
 // Sorts keys[0..n) (only bits [0, end_bit) are examined) carrying vals.
 // keys_alt/vals_alt are ping-pong buffers of the same size.  Returns true if
@@ -93,13 +101,12 @@ TKA_DEV uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total)
 // LDS histograms h.  Keys that arrive nearly sorted (CSR edges, Morton keys
 // of points kept in Morton order) give whole waves one digit: those add once,
 // from one lane, instead of 64 atomics serialised on a single bin.
-template <class K>
-TKA_DEV void rs_tile_histogram(const K (&key)[kRsItems], const bool (&ok)[kRsItems], int passes,
-                               uint32_t (*h)[256])
+template <class K, int ITEMS>
+TKA_DEV void rs_tile_histogram(const K (&key)[ITEMS], const bool (&ok)[ITEMS], int passes, uint32_t (*h)[256])
 {
     const int lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < kRsItems; k++) {
+    for (int k = 0; k < ITEMS; k++) {
         const uint64_t act = __ballot(ok[k]);
         for (int p = 0; p < passes; p++) {
             const uint32_t d = (uint32_t)(key[k] >> (8 * p)) & 255u;
@@ -126,14 +133,16 @@ TKA_DEV void rs_count_flush(int passes, RsLayout L, uint32_t (*h)[256])
     }
 }
 
-// Counting step of the sort for one tile, called by all kRsBlock threads of
-// the workgroup that holds tile `tile` (keys tile*kRsBlock*kRsItems + k*kRsBlock
-// + threadIdx.x, k < kRsItems; ok[k] false past n), one workgroup per tile.
+// Counting step of the counted sort for one tile, called by all kRsBlock
+// threads of the workgroup that holds tile `tile` (keys
+// tile*kRsBlock*kRsItemsCounted + k*kRsBlock + threadIdx.x, k <
+// kRsItemsCounted; ok[k] false past n), one workgroup per tile.
 // Adds the tile's digit histograms of all passes to the global counts and
 // clears the tile's status words.  The sort's passes must follow: they clear
 // the counts again.
 template <class K>
-TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems], int passes, RsLayout L, int tile,
+TKA_DEV void rs_count_tile(const K (&key)[kRsItemsCounted], const bool (&ok)[kRsItemsCounted], int passes, RsLayout L,
+                           int tile,
                            int tiles)
 {
     __shared__ uint32_t h[kRsMaxPasses][256];
@@ -144,7 +153,7 @@ TKA_DEV void rs_count_tile(const K (&key)[kRsItems], const bool (&ok)[kRsItems],
     }
     if (tile == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
     __syncthreads();
-    rs_tile_histogram<K>(key, ok, passes, h);
+    rs_tile_histogram<K, kRsItemsCounted>(key, ok, passes, h);
     __syncthreads();
     rs_count_flush(passes, L, h);
 }
