@@ -290,6 +290,8 @@ struct LoopState {
 };
 
 // Accumulates HIP-event timings of one kernel class on the session stream.
+// The events carry no system-scope fence (they only time device work), so
+// recording them adds no cache writeback between the kernels they bracket.
 struct EventTimer {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
     hipEvent_t pending = nullptr;
@@ -297,13 +299,13 @@ struct EventTimer {
     int64_t launches = 0;
     void begin(hipStream_t s)
     {
-        TKA_HIP(hipEventCreate(&pending));
+        TKA_HIP(hipEventCreateWithFlags(&pending, hipEventDisableSystemFence));
         TKA_HIP(hipEventRecord(pending, s));
     }
     void end(hipStream_t s)
     {
         hipEvent_t e;
-        TKA_HIP(hipEventCreate(&e));
+        TKA_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         TKA_HIP(hipEventRecord(e, s));
         ev.emplace_back(pending, e);
         pending = nullptr;
@@ -465,8 +467,9 @@ struct Session : tka_session {
         if (!exact) {
             tree.alloc(N, m);
             TKA_HIP(hipStreamCreate(&s2));
-            TKA_HIP(hipEventCreateWithFlags(&ev_y, hipEventDisableTiming));
-            TKA_HIP(hipEventCreateWithFlags(&ev_attr, hipEventDisableTiming));
+            // both streams are on this device: device-scope release suffices
+            TKA_HIP(hipEventCreateWithFlags(&ev_y, hipEventDisableTiming | hipEventDisableSystemFence));
+            TKA_HIP(hipEventCreateWithFlags(&ev_attr, hipEventDisableTiming | hipEventDisableSystemFence));
         }
         nbox = nbox0;
         Ycur = L.Ya.get();
