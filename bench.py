@@ -291,11 +291,13 @@ def main():
                         "the walk is bound by dependent node loads, not by HBM (traffic_GBps)"}
     kernels = {"tree_build_ms": round(build_ms, 5), "repulsion_ms": round(rep_ms, 5),
                "update_ms": round(upd_ms, 5),
-               "attraction_ms": round(att_ms, 5), "attraction_concurrent_with": "tree build (sort and node kernels)",
+               "attraction_ms": round(att_ms, 5),
+               "attraction_runs_in": "extra workgroups of the tree build's sort and node launches (one stream)",
                "attraction_GBps": round(att_gbs, 1),
                "attraction_hbm_frac": round(att_gbs / PEAK_HBM_GBS, 4),
-               "attraction_note": "one workgroup per CU by design, finishing inside the tree build; "
-                                  "at full width it streams faster but slows the sort kernels beside it"}
+               "attraction_note": "per-kernel timings come from a profiling pass in which the tree build runs "
+                                  "without the attraction rows and the attraction runs alone at full width; "
+                                  "algorithmic bytes (the Y gathers mostly hit L2)"}
 
     # ---- accuracy against FP64 at this size
     accuracy = None

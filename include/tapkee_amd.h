@@ -143,7 +143,8 @@ TKA_API tka_status tka_tsne_run(const double* X, int32_t N, int32_t D, double* Y
 typedef struct tka_session tka_session;
 
 typedef struct tka_kernel_timing {
-    int64_t launches_build;      /* tree builds (keys, sort, nodes) timed */
+    int64_t launches_build;      /* tree builds (keys, sort, nodes) timed; while timing, the build runs
+                                    without the attraction rows it otherwise carries */
     int64_t launches_repulsion;  /* Barnes-Hut traversals timed */
     int64_t launches_step;       /* optimiser updates timed (with the attraction fused when theta = 0) */
     int64_t launches_exact;      /* exact all-pairs repulsion launches timed */
@@ -155,8 +156,8 @@ typedef struct tka_kernel_timing {
     int64_t nnz;                 /* entries of P (edges per attraction pass) */
     int32_t n;
     int32_t dims;
-    int64_t launches_attraction; /* Barnes-Hut loop: attraction passes timed (second stream,
-                                    concurrent with the tree's sort and node kernels) */
+    int64_t launches_attraction; /* Barnes-Hut loop: attraction passes timed as launches of their own
+                                    (untimed steps run these rows inside the tree build's launches) */
     double ms_attraction;
 } tka_kernel_timing;
 

@@ -48,6 +48,9 @@
 // skip pointer; otherwise descend to t+1.
 #include "bh.hpp"
 
+#include <algorithm>
+#include <vector>
+
 #include "radix_sort.hpp"
 
 namespace tka {
@@ -137,20 +140,29 @@ __global__ __launch_bounds__(kRsBlock) void bh_keys(T* __restrict__ Y, int N, co
         const int g = blockIdx.x * kRsBlock + threadIdx.x;
         if (g < (N + kTreeScanTile - 1) / kTreeScanTile) tile_cnt[g] = 0u;
     }
+    // the tile's points are requested before the box reduction, so both
+    // loads are in flight together
+    using V2 = typename Vec2<T>::type;
+    const int base = blockIdx.x * kRsBlock * kRsItemsCounted + threadIdx.x;
+    V2 yv[kRsItemsCounted];
+#pragma unroll
+    for (int k = 0; k < kRsItemsCounted; k++) {
+        const int i = base + k * kRsBlock;
+        yv[k] = i < N ? reinterpret_cast<const V2*>(Y)[i] : V2{};
+    }
     const RootBox rb = reduce_root_box<kRsBlock>(box_part, nbox, N);
     if (blockIdx.x == 0 && threadIdx.x == 0) *root = rb;
     const double mx = rb.mean_x, my = rb.mean_y, hw = rb.hw, hh = rb.hh;
     uint32_t hk[kRsItemsCounted];
     bool ok[kRsItemsCounted];
-    const int base = blockIdx.x * kRsBlock * kRsItemsCounted + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < kRsItemsCounted; k++) {
         const int i = base + k * kRsBlock;
         ok[k] = i < N;
         hk[k] = 0u;
         if (ok[k]) {
-            const T x = (T)((double)Y[2 * (size_t)i] - mx);
-            const T y = (T)((double)Y[2 * (size_t)i + 1] - my);
+            const T x = (T)((double)yv[k].x - mx);
+            const T y = (T)((double)yv[k].y - my);
             Y[2 * (size_t)i] = x;
             Y[2 * (size_t)i + 1] = y;
             const uint32_t qx = quantise(((double)x + hw) / (2.0 * hw));
@@ -234,8 +246,13 @@ template <class T>
 __global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ ktop, const uint32_t* __restrict__ perm,
                                               const uint64_t* __restrict__ kfull, const T* __restrict__ Y, int N,
                                               uint64_t* __restrict__ k64, uint32_t* __restrict__ p2,
-                                              typename Vec2<T>::type* __restrict__ ys)
+                                              typename Vec2<T>::type* __restrict__ ys, int nmain,
+                                              AttractSlice<T> side)
 {
+    if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
+        side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
+        return;
+    }
     __shared__ uint64_t sk[4][kWaveRun];
     __shared__ uint32_t sp[4][kWaveRun];
     const int j = blockIdx.x * 256 + threadIdx.x;
@@ -411,8 +428,12 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
                                                 unsigned long long* __restrict__ mask, int* __restrict__ node_a,
                                                 int* __restrict__ node_b, int* __restrict__ node_lvl,
                                                 uint32_t* __restrict__ tile_cnt, double* __restrict__ part,
-                                                uint32_t* __restrict__ eq24)
+                                                uint32_t* __restrict__ eq24, int nmain, AttractSlice<T> side)
 {
+    if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
+        side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
+        return;
+    }
     __shared__ uint64_t win[kPairWin];
     __shared__ uint32_t own_cnt[2];  // owners with a in tile blockIdx.x - 1, blockIdx.x
     __shared__ uint32_t eq_cnt;      // pairs whose keys share their top 24 bits
@@ -517,8 +538,13 @@ __global__ __launch_bounds__(256) void bh_scan_write(const unsigned long long* _
                                                      const typename Vec2<T>::type* __restrict__ ys, int N,
                                                      const uint32_t* __restrict__ tile_cnt,
                                                      const double* __restrict__ part, uint32_t* __restrict__ pre,
-                                                     double* __restrict__ psx, double* __restrict__ psy)
+                                                     double* __restrict__ psx, double* __restrict__ psy, int nmain,
+                                                     AttractSlice<T> side)
 {
+    if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
+        side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
+        return;
+    }
     __shared__ double sh[4];
     __shared__ double wsum[3][4];
     // prefix of earlier tiles: nodes = points + internal nodes starting there
@@ -572,12 +598,17 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
                                                const uint32_t* __restrict__ perm, int N,
                                                const uint32_t* __restrict__ pre, const double* __restrict__ psx,
                                                const double* __restrict__ psy, BhNode<T>* __restrict__ nodes,
-                                               const uint32_t* __restrict__ eq24, uint32_t* __restrict__ eq24_host)
+                                               const uint32_t* __restrict__ eq24, uint32_t* __restrict__ eq24_host,
+                                               int nmain, AttractSlice<T> side)
 {
+    if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
+        side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
+        return;
+    }
     if (blockIdx.x == 0) {  // the pairs kernel's count, posted to the host for the next build's pass count
         __shared__ double sh[4];
         double c = 0.0;
-        for (int b = threadIdx.x; b < (int)gridDim.x; b += 256) c += (double)eq24[b];
+        for (int b = threadIdx.x; b < nmain; b += 256) c += (double)eq24[b];
         c = block_sum(c, sh);
         if (threadIdx.x == 0) __hip_atomic_store(eq24_host, (uint32_t)c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
@@ -786,29 +817,60 @@ void bh_build(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipS
 }
 
 template <class T>
-void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s)
+void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s, const AttractSlice<T>* attract)
 {
     const unsigned g = ceil_div(N, 256);
-    const bool alt = radix_sort_pairs_counted<uint32_t, uint32_t>(
-        tr.keys.get(), tr.perm.get(), tr.keys_alt.get(), tr.perm_alt.get(), N, 8 * tr.passes, tr.sort_scratch.get(), s);
+    // The attraction rows are split over the sort passes and the node kernels
+    // in proportion to their durations at N = 100k (a pass 12 us; pairs 10,
+    // scan 5, emit 4), one row group per extra workgroup, so each launch's
+    // extra workgroups finish within it on the CUs its own workgroups leave
+    // idle.  bh_fix takes none: its 48 KB of LDS per workgroup would hold
+    // back the extra workgroups.  Other splits (passes 8 or 16; node kernels
+    // 14/7/6) measured the same within 0.5%.
+    const int passes = N > 1 ? tr.passes : 0;
+    constexpr int kWPass = 12, kWFix = 0, kWPairs = 10, kWScan = 5, kWEmit = 4;
+    std::vector<int> weight(passes, kWPass);
+    weight.insert(weight.end(), {kWFix, kWPairs, kWScan, kWEmit});
+    const int nk = (int)weight.size();
+    std::vector<AttractSlice<T>> slice(nk);
+    std::vector<int> wgs(nk, 0);
+    if (attract) {
+        int wsum = 0, wacc = 0;
+        for (int w : weight) wsum += w;
+        const int r0 = attract->row0, rows = attract->row1 - attract->row0;
+        for (int k = 0; k < nk; k++) {
+            slice[k] = *attract;
+            slice[k].row0 = r0 + (int)((int64_t)rows * wacc / wsum);
+            wacc += weight[k];
+            slice[k].row1 = r0 + (int)((int64_t)rows * wacc / wsum);
+            const int groups = (int)ceil_div((size_t)std::max(0, slice[k].row1 - slice[k].row0), kRowsPerBlock);
+            wgs[k] = groups;
+        }
+    }
+    bool alt = false;
+    if (passes > 0)
+        alt = radix_sort_pairs_counted_side<uint32_t, uint32_t, AttractSlice<T>>(
+            tr.keys.get(), tr.perm.get(), tr.keys_alt.get(), tr.perm_alt.get(), N, 8 * passes, tr.sort_scratch.get(),
+            slice.data(), wgs.data(), s);
     const uint32_t* hk = alt ? tr.keys_alt.get() : tr.keys.get();
     const uint32_t* hp = alt ? tr.perm_alt.get() : tr.perm.get();
-    hipLaunchKernelGGL(bh_fix<T>, dim3(g), dim3(256), 0, s, hk, hp, tr.kfull.get(), Y, N, tr.k64.get(), tr.p2.get(),
-                       tr.ys.get());
+    const int kf = passes, kp = passes + 1, ks = passes + 2, ke = passes + 3;
+    hipLaunchKernelGGL(bh_fix<T>, dim3(g + wgs[kf]), dim3(256), 0, s, hk, hp, tr.kfull.get(), Y, N, tr.k64.get(),
+                       tr.p2.get(), tr.ys.get(), (int)g, slice[kf]);
     TKA_LAUNCH_CHECK();
     const uint32_t* p = tr.p2.get();
     tr.sorted_perm = p;
     static_assert(kTreeScanTile == 256, "one scan tile per pairs workgroup");
-    hipLaunchKernelGGL(bh_pairs<T>, dim3(g), dim3(256), 0, s, tr.k64.get(), tr.ys.get(), N, tr.mask.get(),
+    hipLaunchKernelGGL(bh_pairs<T>, dim3(g + wgs[kp]), dim3(256), 0, s, tr.k64.get(), tr.ys.get(), N, tr.mask.get(),
                        tr.node_a.get(), tr.node_b.get(), tr.node_lvl.get(), tr.tile_cnt.get(), tr.part.get(),
-                       tr.eq24.get());
+                       tr.eq24.get(), (int)g, slice[kp]);
     TKA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bh_scan_write<T>, dim3(g), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N,
-                       tr.tile_cnt.get(), tr.part.get(), tr.pre.get(), tr.sx.get(), tr.sy.get());
+    hipLaunchKernelGGL(bh_scan_write<T>, dim3(g + wgs[ks]), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N,
+                       tr.tile_cnt.get(), tr.part.get(), tr.pre.get(), tr.sx.get(), tr.sy.get(), (int)g, slice[ks]);
     TKA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bh_emit<T>, dim3(g), dim3(256), 0, s, tr.mask.get(), tr.node_a.get(), tr.node_b.get(),
-                       tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(), tr.nodes.get(),
-                       tr.eq24.get(), tr.eq24_host);
+    hipLaunchKernelGGL(bh_emit<T>, dim3(g + wgs[ke]), dim3(256), 0, s, tr.mask.get(), tr.node_a.get(),
+                       tr.node_b.get(), tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(),
+                       tr.nodes.get(), tr.eq24.get(), tr.eq24_host, (int)g, slice[ke]);
     TKA_LAUNCH_CHECK();
 }
 
@@ -832,8 +894,8 @@ template void bh_build<float>(float*, int, const double*, int, BhTree<float>&, h
 template void bh_build<double>(double*, int, const double*, int, BhTree<double>&, hipStream_t);
 template void bh_build_keys<float>(float*, int, const double*, int, BhTree<float>&, hipStream_t);
 template void bh_build_keys<double>(double*, int, const double*, int, BhTree<double>&, hipStream_t);
-template void bh_build_tree<float>(const float*, int, BhTree<float>&, hipStream_t);
-template void bh_build_tree<double>(const double*, int, BhTree<double>&, hipStream_t);
+template void bh_build_tree<float>(const float*, int, BhTree<float>&, hipStream_t, const AttractSlice<float>*);
+template void bh_build_tree<double>(const double*, int, BhTree<double>&, hipStream_t, const AttractSlice<double>*);
 template int bh_repulsion<float>(const BhTree<float>&, const float*, double, float*, double*, unsigned long long*,
                                  hipStream_t);
 template int bh_repulsion<double>(const BhTree<double>&, const double*, double, double*, double*,

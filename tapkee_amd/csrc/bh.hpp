@@ -2,6 +2,7 @@
 // keys (replaces quadtree.hpp's pointer tree and its per-point recursion).
 #pragma once
 
+#include "attract.hpp"
 #include "common.hpp"
 
 namespace tka {
@@ -160,11 +161,13 @@ template <class T>
 void bh_build(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s);
 
 // bh_build in two halves: root cell, centring of Y and keys (after which Y
-// is final for this iteration); then sort and nodes.
+// is final for this iteration); then sort and nodes.  bh_build_tree can carry
+// the attraction rows of `attract` (of this final Y) in extra workgroups of
+// its launches, on CUs that the sort and node kernels leave idle.
 template <class T>
 void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s);
 template <class T>
-void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s);
+void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s, const AttractSlice<T>* attract = nullptr);
 
 // Repulsive forces: neg (N x 2) = sum over accepted cells of
 // count*q^2*(y - com) and zpart[block] = partial sums of count*q

@@ -391,30 +391,33 @@ struct Session : tka_session {
     DevBuf<T> val2, state_tmp;
     DevBuf<uint32_t> rn_scratch;
 
-    // Barnes-Hut loop: the attraction pass runs on a second stream beside
-    // the tree's sort and node kernels.  ev_y marks "Y centred and final for
-    // this iteration"; the update waits for ev_attr.
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_y = nullptr, ev_attr = nullptr;
-
-    ~Session() override
+    // Barnes-Hut loop: one stream.  The attraction rows ride in extra
+    // workgroups of the tree build's sort and node launches, which leave most
+    // CUs idle; no second stream, so the overlap does not depend on how the
+    // runtime maps streams onto hardware queues (two streams that share a
+    // queue run one after the other).
+    AttractSlice<T> attract_job() const
     {
-        if (s2) {
-            (void)hipStreamSynchronize(s2);
-            (void)hipStreamDestroy(s2);
-        }
-        if (ev_y) (void)hipEventDestroy(ev_y);
-        if (ev_attr) (void)hipEventDestroy(ev_attr);
+        AttractSlice<T> a;
+        a.Y = Ycur;
+        a.row_ptr = L.row.get();
+        a.col = L.col.get();
+        a.val = L.val.get();
+        a.exag = (T)exag;
+        a.pos = L.pos.get();
+        a.row0 = 0;
+        a.row1 = N;
+        return a;
     }
 
-    void launch_attraction()
+    // The attraction as a launch of its own (after a KL evaluation, whose
+    // tree the step reuses, and in profiling mode, where it is timed alone).
+    void attraction_alone()
     {
-        TKA_HIP(hipEventRecord(ev_y, s()));
-        TKA_HIP(hipStreamWaitEvent(s2, ev_y, 0));
-        if (profile) timers[kTimAttract].begin(s2);
-        attraction_device<T>(dims, Ycur, L.row.get(), L.col.get(), L.val.get(), exag, N, L.pos.get(), s2);
-        if (profile) timers[kTimAttract].end(s2);
-        TKA_HIP(hipEventRecord(ev_attr, s2));
+        hipStream_t s = this->s();
+        if (profile) timers[kTimAttract].begin(s);
+        attraction_device<T>(dims, Ycur, L.row.get(), L.col.get(), L.val.get(), exag, N, L.pos.get(), s);
+        if (profile) timers[kTimAttract].end(s);
     }
 
     void renumber(const uint32_t* perm)
@@ -464,13 +467,7 @@ struct Session : tka_session {
         if (L.dense || !exact) L.pos.alloc(nd, m);
         // tsne.hpp:75-80: uY = 0, gains = 1; Y0 is not centred before iteration 0
         bbox_partials_device<T>(L.Ya.get(), N, L.box.get(), s);
-        if (!exact) {
-            tree.alloc(N, m);
-            TKA_HIP(hipStreamCreate(&s2));
-            // both streams are on this device: device-scope release suffices
-            TKA_HIP(hipEventCreateWithFlags(&ev_y, hipEventDisableTiming | hipEventDisableSystemFence));
-            TKA_HIP(hipEventCreateWithFlags(&ev_attr, hipEventDisableTiming | hipEventDisableSystemFence));
-        }
+        if (!exact) tree.alloc(N, m);
         nbox = nbox0;
         Ycur = L.Ya.get();
         Ynext = L.Yb.get();
@@ -497,21 +494,23 @@ struct Session : tka_session {
             } else if (!tree_ready) {
                 if (iter > 0 && iter % kRenumberEvery == 0 && tree.sorted_perm != nullptr)
                     renumber(tree.sorted_perm);  // order of the previous iteration's tree
+                // Y is final after the key kernel: the (HBM-bound) attraction
+                // runs inside the sort and node launches, which occupy few
+                // CUs; beside the traversal it would slow the traversal's
+                // node loads by about its own duration.  Profiling times the
+                // build without it, then the attraction alone.
+                const AttractSlice<T> job = attract_job();
                 if (profile) timers[kTimBuild].begin(s);
                 bh_build_keys<T>(Ycur, N, L.box.get(), nbox, tree, s);
-                // Y is final after the key kernel: the (HBM-bound) attraction
-                // runs beside the sort and node kernels, which occupy few
-                // CUs; beside the traversal it would slow the traversal's
-                // node loads by about its own duration
-                launch_attraction();
-                bh_build_tree<T>(Ycur, N, tree, s);
+                bh_build_tree<T>(Ycur, N, tree, s, profile ? nullptr : &job);
                 if (profile) timers[kTimBuild].end(s);
+                if (profile) attraction_alone();
                 if (profile) timers[kTimRepulsion].begin(s);
                 bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(),
                                 profile ? interactions.get() : nullptr, s);
                 if (profile) timers[kTimRepulsion].end(s);
             } else {
-                launch_attraction();  // tree and repulsion came with the last KL evaluation
+                attraction_alone();  // tree and repulsion came with the last KL evaluation
             }
             tree_ready = false;
             // after renumbering: the CSR arrays were swapped for their renumbered copies
@@ -526,7 +525,6 @@ struct Session : tka_session {
                                            nullptr, s);
                 if (profile) timers[kTimStep].end(s);
             } else {
-                TKA_HIP(hipStreamWaitEvent(s, ev_attr, 0));
                 if (profile) timers[kTimStep].begin(s);
                 nbox = tsne_update_device<T>(dims, Ycur, Ynext, L.uY.get(), L.gains.get(), L.neg.get(), L.pos.get(),
                                              L.zpart.get(), nz, N, sp, L.box.get(), nullptr, s);

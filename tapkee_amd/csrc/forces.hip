@@ -3,27 +3,16 @@
 
 #include <cfloat>
 
+#include "attract.hpp"
 #include "radix_sort.hpp"
 
 namespace tka {
 namespace {
 
-constexpr int kG = 16;                 // lanes per CSR row
-constexpr int kRowsPerBlock = 256 / kG;
-
 template <class T>
 TKA_DEV T sgn(T x)
 {
     return x == (T)0 ? (T)0 : (x < (T)0 ? (T)-1 : (T)1);  // tsne.hpp:53-56
-}
-
-// Sum over the kG lanes of a row group (lanes g*kG .. g*kG+kG-1).
-template <class T>
-TKA_DEV T group_sum(T v)
-{
-#pragma unroll
-    for (int o = kG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
 }
 
 TKA_DEV double z_total(const double* __restrict__ zpart, int nz, double* sh)
@@ -130,66 +119,8 @@ __global__ __launch_bounds__(256) void tsne_step(const T* __restrict__ Y, T* __r
     }
 }
 
-// Attraction alone (quadtree.hpp:357-384), for the Barnes-Hut loop where it
-// runs beside the tree build: pos_i = sum_e exag*P_e (y_i - y_col)/(1 + d^2).
-// Each lane takes its row's edges e, e+kG, e+2kG, ... as before, loading
-// kBatch column indices, then kBatch rows of Y, before the arithmetic, so a
-// lane waits on two memory latencies per kBatch edges rather than per edge.
-// Padding entries use the row itself with P = 0 and add exactly zero.
-// The grid is one workgroup per CU, each looping over row groups: run at full
-// width this HBM-bound pass slows the latency-bound sort kernels beside it by
-// more than it gains; at one workgroup per CU it takes about two thirds of
-// the tree build and leaves the sort most of the device (measured at N=100k:
-// 0.165 ms per iteration against 0.175 at full width).
-constexpr int kBatch = 4;
-
-template <class T, int DIMS>
-TKA_DEV void attract_row(const T* __restrict__ Y, const int* __restrict__ row_ptr, const int* __restrict__ col,
-                         const T* __restrict__ val, T exag, int i, int gl, T* __restrict__ pos)
-{
-    T yi[DIMS], acc[DIMS];
-#pragma unroll
-    for (int d = 0; d < DIMS; d++) {
-        yi[d] = Y[(size_t)i * DIMS + d];
-        acc[d] = (T)0;
-    }
-    const int e1 = row_ptr[i + 1];
-    for (int base = row_ptr[i] + gl; base < e1; base += kBatch * kG) {
-        int j[kBatch];
-        T v[kBatch];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            const int e = base + u * kG;
-            const bool ok = e < e1;
-            j[u] = ok ? col[e] : i;
-            v[u] = ok ? val[e] : (T)0;
-        }
-        T yj[kBatch][DIMS];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++)
-#pragma unroll
-            for (int d = 0; d < DIMS; d++) yj[u][d] = Y[(size_t)j[u] * DIMS + d];
-#pragma unroll
-        for (int u = 0; u < kBatch; u++) {
-            T diff[DIMS];
-            T D = (T)0;
-#pragma unroll
-            for (int d = 0; d < DIMS; d++) {
-                diff[d] = yi[d] - yj[u][d];
-                D += diff[d] * diff[d];
-            }
-            const T m = (exag * v[u]) / ((T)1 + D);
-#pragma unroll
-            for (int d = 0; d < DIMS; d++) acc[d] += m * diff[d];
-        }
-    }
-#pragma unroll
-    for (int d = 0; d < DIMS; d++) acc[d] = group_sum(acc[d]);
-    if (gl == 0)
-#pragma unroll
-        for (int d = 0; d < DIMS; d++) pos[(size_t)i * DIMS + d] = acc[d];
-}
-
+// Attraction alone (quadtree.hpp:357-384): one row group of kG lanes per row,
+// a grid-stride loop over row groups (attract_row).
 template <class T, int DIMS>
 __global__ __launch_bounds__(256) void attract(const T* __restrict__ Y, const int* __restrict__ row_ptr,
                                                const int* __restrict__ col, const T* __restrict__ val, T exag,
@@ -219,24 +150,37 @@ __global__ __launch_bounds__(kUpdBlock) void tsne_update(const T* __restrict__ Y
     constexpr int kWaves = kUpdBlock / 64;
     __shared__ double sh[kWaves];
     __shared__ double sbox[7][kWaves];
-    const double Z = z_total(zpart, nz, sh);
     const int i = blockIdx.x * kUpdBlock + threadIdx.x;
+    // The row's operands are requested before the Z reduction, so its loads
+    // and theirs are in flight together.
+    T rp[DIMS], rn[DIMS], rg[DIMS], ru[DIMS], ry[DIMS];
+#pragma unroll
+    for (int d = 0; d < DIMS; d++) {
+        const size_t o = (size_t)i * DIMS + d;
+        const bool in = i < N;
+        rp[d] = in ? pos[o] : (T)0;
+        rn[d] = in ? neg[o] : (T)0;
+        rg[d] = in ? gains[o] : (T)0;
+        ru[d] = in ? uY[o] : (T)0;
+        ry[d] = in ? Y[o] : (T)0;
+    }
+    const double Z = z_total(zpart, nz, sh);
     double bx[7] = {0.0, 0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
     if (i < N) {
         T yn[DIMS];
 #pragma unroll
         for (int d = 0; d < DIMS; d++) {
             const size_t o = (size_t)i * DIMS + d;
-            const T dC = pos[o] - (T)((double)neg[o] / Z);
+            const T dC = rp[d] - (T)((double)rn[d] / Z);
             if (dC_out) dC_out[o] = dC;
-            T gn = gains[o];
-            T u = uY[o];
+            T gn = rg[d];
+            T u = ru[d];
             gn = (sgn(dC) != sgn(u)) ? (gn + (T)0.2) : (gn * (T)0.8);
             if (gn < (T)0.01) gn = (T)0.01;
             u = momentum * u - eta * gn * dC;
             gains[o] = gn;
             uY[o] = u;
-            yn[d] = Y[o] + u;
+            yn[d] = ry[d] + u;
             Ynext[o] = yn[d];
             if (d < 3) bx[d] = (double)yn[d];
         }
@@ -694,7 +638,7 @@ template <class T>
 void attraction_device(int dims, const T* Y, const int* row_ptr, const int* col, const T* val, double exag, int N,
                        T* pos, hipStream_t s)
 {
-    const int nb = std::min(tsne_step_blocks(N), device_cus());
+    const int nb = tsne_step_blocks(N);  // one row group per kG lanes: full width, alone on the device
 #define TKA_ATT(DM) \
     hipLaunchKernelGGL((attract<T, DM>), dim3(nb), dim3(256), 0, s, Y, row_ptr, col, val, (T)exag, N, pos)
     switch (dims) {

@@ -27,9 +27,9 @@ int tsne_step_device(int dims, const T* Y, T* Ynext, T* uY, T* gains, const T* n
                      StepParams sp, double* box_part, T* dC_out, hipStream_t s);
 int tsne_step_blocks(int N);
 
-// The same step in two launches, so that the attraction can run on a second
-// stream beside the tree build: pos (N x dims) from the CSR, then the update
-// from pos, neg and the zpart partials of Z.  The update returns the number of
+// The same step in two parts, so that the attraction can run inside the tree
+// build's launches (AttractSlice, bh_build_tree) or alone: pos (N x dims) from
+// the CSR, then the update from pos, neg and the zpart partials of Z.  The update returns the number of
 // box partials it wrote (tsne_update_blocks(N), a few per thousand points, so
 // that the next tree build can reduce them cheaply).
 template <class T>

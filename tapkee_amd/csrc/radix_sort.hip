@@ -20,6 +20,8 @@
 //                gives up sets a fault word the host can read.
 #include "radix_sort.hpp"
 
+#include "attract.hpp"
+
 namespace tka {
 namespace {
 
@@ -63,11 +65,16 @@ __global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys,
 // Stable scatter of one tile.  Item k of lane l in wave w sits at tile
 // position w*64*ITEMS + k*64 + l, so ranking items in (w, k, l) order keeps
 // the input order among equal digits.
-template <class K, class V, int ITEMS>
+// Workgroups past the pass's tiles run the side work and take no tile.
+template <class K, class V, int ITEMS, class Side>
 __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ kin, const V* __restrict__ vin,
                                                         K* __restrict__ kout, V* __restrict__ vout, size_t n,
-                                                        int pass, RsLayout L, int tiles, int ticketed)
+                                                        int pass, RsLayout L, int tiles, int ticketed, Side side)
 {
+    if ((int)blockIdx.x >= tiles) {
+        side.run((int)blockIdx.x - tiles, (int)gridDim.x - tiles);
+        return;
+    }
     constexpr int kWaves = kRsBlock / 64;
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t base[256];
@@ -264,9 +271,9 @@ void scan_impl(const uint32_t* in, uint32_t* out, size_t n, uint32_t* scratch, b
     TKA_LAUNCH_CHECK();
 }
 
-template <class K, class V, int ITEMS>
+template <class K, class V, int ITEMS, class Side = RsNoSide>
 bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch, bool counted,
-               hipStream_t s)
+               hipStream_t s, const Side* side = nullptr, const int* side_wgs = nullptr)
 {
     const int tiles = (int)ceil_div(n, (size_t)kRsBlock * ITEMS);
     const int passes = (end_bit + 7) / 8;
@@ -282,8 +289,9 @@ bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit
     K* kb = keys_alt;
     V* vb = vals_alt;
     for (int p = 0; p < passes; p++) {
-        hipLaunchKernelGGL((rs_onesweep<K, V, ITEMS>), dim3(tiles), dim3(kRsBlock), 0, s, ka, va, kb, vb, n, p, L,
-                           tiles, tiles > kRsMaxIndexedTiles ? 1 : 0);
+        const int extra = side ? side_wgs[p] : 0;
+        hipLaunchKernelGGL((rs_onesweep<K, V, ITEMS, Side>), dim3(tiles + extra), dim3(kRsBlock), 0, s, ka, va, kb,
+                           vb, n, p, L, tiles, tiles > kRsMaxIndexedTiles ? 1 : 0, side ? side[p] : Side{});
         TKA_LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);
@@ -320,6 +328,24 @@ bool radix_sort_pairs_counted(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t
 
 template bool radix_sort_pairs_counted<uint32_t, uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, size_t, int,
                                                            uint32_t*, hipStream_t);
+
+template <class K, class V, class Side>
+bool radix_sort_pairs_counted_side(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit,
+                                   uint32_t* scratch, const Side* side, const int* side_wgs, hipStream_t stream)
+{
+    if (n <= 1 || end_bit <= 0) return false;
+    require(n <= kRsMaxItems, TKA_E_UNSUPPORTED, "radix sort: more than 2^30 - 1 items");
+    require(end_bit <= 8 * (int)sizeof(K), TKA_E_INTERNAL, "radix sort: end_bit beyond the key width");
+    return sort_impl<K, V, kRsItemsCounted, Side>(keys, vals, keys_alt, vals_alt, n, end_bit, scratch, true, stream,
+                                                  side, side_wgs);
+}
+
+template bool radix_sort_pairs_counted_side<uint32_t, uint32_t, AttractSlice<float>>(
+    uint32_t*, uint32_t*, uint32_t*, uint32_t*, size_t, int, uint32_t*, const AttractSlice<float>*, const int*,
+    hipStream_t);
+template bool radix_sort_pairs_counted_side<uint32_t, uint32_t, AttractSlice<double>>(
+    uint32_t*, uint32_t*, uint32_t*, uint32_t*, size_t, int, uint32_t*, const AttractSlice<double>*, const int*,
+    hipStream_t);
 
 template bool radix_sort_pairs<uint32_t, uint32_t>(uint32_t*, uint32_t*, uint32_t*, uint32_t*, size_t, int, uint32_t*,
                                                    hipStream_t);

@@ -40,6 +40,17 @@ template <class K, class V>
 bool radix_sort_pairs_counted(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit, uint32_t* scratch,
                               hipStream_t stream);
 
+// Work that rides along in a sort's passes: pass p launches side_wgs[p]
+// workgroups past its tiles, which call side[p].run(w, side_wgs[p]) for
+// w = 0 .. side_wgs[p]-1 and take no tile.  The side work must not depend on
+// the sort.  Instantiated for AttractSlice<float> and AttractSlice<double>.
+struct RsNoSide {
+    TKA_DEV void run(int, int) const {}
+};
+template <class K, class V, class Side>
+bool radix_sort_pairs_counted_side(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit,
+                                   uint32_t* scratch, const Side* side, const int* side_wgs, hipStream_t stream);
+
 // ---------------------------------------------------------------- device side
 // Scratch layout (words): [1] fault, [2, 2+8) tile tickets per pass, [16,
 // 16+8) per-pass count readers, then the global digit counts (8 x 256), then

@@ -314,6 +314,24 @@ def test_renumbered_run_returns_caller_order(engine):
     assert (d.argmin(1) == labels).mean() > 0.95
 
 
+@pytest.mark.parametrize("n", [3000, 70_000])
+def test_attraction_inside_build_equals_attraction_alone(engine, n):
+    # Untimed steps run the attraction rows in extra workgroups of the tree
+    # build's launches; profiling mode runs the build without them and the
+    # attraction as a launch of its own.  Each row is computed by the same
+    # code either way, so the two loops must agree bit for bit (70k points:
+    # several sort tiles and row groups per launch).
+    X = mixture(n, 16, k=6)
+    out = []
+    for prof in (False, True):
+        s = engine.Session(X, perplexity=20.0, theta=0.5, seed=4, log_every=0)
+        s.profile(prof)
+        s.step(60)
+        out.append(s.embedding())
+        s.close()
+    assert np.array_equal(out[0], out[1])
+
+
 def test_y0_replays_std_rand(engine, oracle):
     X = mixture(300, 10)
     # max_iter = 0: the output is the (centred) initial solution
