@@ -718,9 +718,12 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
             const int sk = (int)nd.link;
             const int next = (!leaf && cell_acc && sk > t) ? sk : t + 1;  // skip pointers point forward
             // the next node is requested as soon as its index is known, so
-            // its load runs under this node's division and accumulation
+            // its load runs under this node's division and accumulation.
+            // Unguarded: next <= total <= 2N - 1 (N leaves, at most N - 1
+            // internal nodes) lies inside the 2N-node buffer, and a load at
+            // next == total ends the walk unused (a guard cost 1 us of 38).
             const uint32_t count = nd.cl & ((1u << kCountBits) - 1u);
-            if (next < total) nd = nodes[next];
+            nd = nodes[next];
             if (acc) {  // a leaf's count is 1: count*q and count*q*q are exactly q and q*q
                 const T q = (T)1 / ((T)1 + D);
                 const T cq = (T)count * q;
@@ -782,7 +785,7 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     sy.alloc(n + 1, mb);
     part.alloc(2 * (size_t)ceil_div(n, kTreeScanTile), mb);
     tile_cnt.alloc(ceil_div(n, kTreeScanTile), mb);
-    nodes.alloc(2 * (size_t)n, mb);
+    nodes.alloc(2 * (size_t)n, mb);  // > the node count (<= 2n - 1): the walk's look-ahead load stays inside
     root.alloc(1, mb);
 }
 
