@@ -49,6 +49,7 @@
 #include "bh.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <vector>
 
 #include "radix_sort.hpp"
@@ -714,12 +715,12 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
             // leaf or cell by selects, not a branch (lanes at leaves and at
             // cells share every instruction; a branch costs more than the
             // cell test a leaf does not need)
-            const bool leaf = (nd.link & kLeafBit) != 0u;
             const int lvl = (int)(nd.cl >> kCountBits);
             const bool cell_acc = ldexp(reach0, -2 * lvl) < D;  // exact scaling by 4^-lvl
-            const bool acc = leaf ? (nd.link & ~kLeafBit) != p : cell_acc;
             const int sk = (int)nd.link;
-            const int next = (cell_acc && sk > t) ? sk : t + 1;  // leaves: sk < 0 (leaf bit 31); skips point forward
+            // leaves: sk < 0 (leaf bit 31); a cell's skip pointer lies past
+            // it; a max and a select keep the choice off the scalar unit
+            const int next = max(cell_acc ? sk : INT_MIN, t + 1);
             // the next node is requested as soon as its index is known, so
             // its load runs under this node's division and accumulation.
             // Unguarded: next <= total <= 2N - 1 (N leaves, at most N - 1
@@ -728,9 +729,14 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
             // A 32-bit byte offset lets the load take the node base from
             // scalar registers.
             static_assert(2ull * kMaxTreePoints * sizeof(BhNode<T>) <= 0xffffffffull, "node offsets fit 32 bits");
-            const uint32_t count = nd.cl & ((1u << kCountBits) - 1u);
+            const uint32_t link = nd.link, cl = nd.cl;
             nd = *reinterpret_cast<const BhNode<T>*>(reinterpret_cast<const char*>(nodes) +
                                                      (uint32_t)next * (uint32_t)sizeof(BhNode<T>));
+            // nothing of the rest may be scheduled ahead of the load
+            __builtin_amdgcn_sched_barrier(0);
+            const bool leaf = (link & kLeafBit) != 0u;
+            const bool acc = leaf ? (link & ~kLeafBit) != p : cell_acc;
+            const uint32_t count = cl & ((1u << kCountBits) - 1u);
             if (acc) {  // a leaf's count is 1: count*q and count*q*q are exactly q and q*q
                 const T q = (T)1 / ((T)1 + D);
                 const T cq = (T)count * q;
