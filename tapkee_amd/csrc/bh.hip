@@ -49,7 +49,9 @@
 #include "bh.hpp"
 
 #include <algorithm>
+#include <cfloat>
 #include <climits>
+#include <type_traits>
 #include <vector>
 
 #include "radix_sort.hpp"
@@ -738,7 +740,21 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
             const bool acc = leaf ? (link & ~kLeafBit) != p : cell_acc;
             const uint32_t count = cl & ((1u << kCountBits) - 1u);
             if (acc) {  // a leaf's count is 1: count*q and count*q*q are exactly q and q*q
-                const T q = (T)1 / ((T)1 + D);
+                // FP32: q = 1/(1 + D) as the hardware reciprocal (1 ulp) with
+                // one Newton-Raphson step, r + r(1 - a r); 3 instructions
+                // against 11 for the IEEE division (the N=100k swiss-roll
+                // embedding after 430 iterations came out bit-identical to the
+                // one with the division).  a is clamped to
+                // FLT_MAX so that an overflowing D gives q = 0 rather than
+                // inf * 0 = NaN in the residual.
+                T q;
+                if constexpr (std::is_same<T, float>::value) {
+                    const float a = fminf(1.0f + D, FLT_MAX);
+                    const float r = __builtin_amdgcn_rcpf(a);
+                    q = fmaf(r, fmaf(-a, r, 1.0f), r);
+                } else {
+                    q = (T)1 / ((T)1 + D);
+                }
                 const T cq = (T)count * q;
                 z += cq;
                 const T m = cq * q;
