@@ -314,13 +314,15 @@ def test_renumbered_run_returns_caller_order(engine):
     assert (d.argmin(1) == labels).mean() > 0.95
 
 
-@pytest.mark.parametrize("n", [3000, 70_000])
+@pytest.mark.parametrize("n", [3000, 70_000, 600_000])
 def test_attraction_inside_build_equals_attraction_alone(engine, n):
     # Untimed steps run the attraction rows in extra workgroups of the tree
     # build's launches; profiling mode runs the build without them and the
     # attraction as a launch of its own.  Each row is computed by the same
     # code either way, so the two loops must agree bit for bit (70k points:
-    # several sort tiles and row groups per launch).
+    # several sort tiles and row groups per launch; 600k points: more than
+    # 512 sort tiles, so tiles are numbered in arrival order and the extra
+    # workgroups take no number).
     X = mixture(n, 16, k=6)
     out = []
     for prof in (False, True):
