@@ -240,6 +240,89 @@ def test_bh_error_shrinks_with_theta(oracle):
     assert errs[1] / np.linalg.norm(exact) < 0.05
 
 
+# ---------------------------------------------------------------- Barnes-Hut acceptance
+# Eight points on the integer grid.  Their mean (-1.375, 4.75) is exact, so
+# the root box is exact up to the +1e-5 of quadtree.hpp:128-129: it is taller
+# than wide (hw = 17.625 + 1e-5, hh = 34.75 + 1e-5).  Points 1 and 3,
+# (10, 36) and (13, 36), share a level-2 cell C with half-sizes (hw/4, hh/4)
+# whose two children are single-point leaves; C's centre of mass (11.5, 36)
+# is exact (two points: the online update multiplies by 0 and 1/2).  For the
+# query points p in BH_QUERIES every other cell decision of every point lies
+# more than 5% away from C's ratio, so moving theta across that ratio flips
+# exactly one decision: p sees C as one summary or as its two leaves.
+BH_Y = np.array([[-1, 0], [10, 36], [-19, -30], [13, 36], [-10, -16], [15, -6], [-9, 26], [-10, -8]], float)
+BH_QUERIES = (2, 4, 5, 7)
+
+
+def _bh_neg_z(oracle, Y, theta):
+    # with an empty P, dC = -neg_f / sum_Q (tsne.hpp:346)
+    row = np.zeros(len(Y) + 1, np.int32)
+    dC, z = oracle.gradient_bh(row, np.zeros(0, np.int32), np.zeros(0), Y, theta)
+    return -dC * z, z
+
+
+@pytest.mark.parametrize("transpose", [False, True], ids=["tall_root", "wide_root"])
+@pytest.mark.parametrize("p", BH_QUERIES)
+def test_bh_acceptance_boundary_by_hand(oracle, p, transpose):
+    """quadtree.hpp:335: a cell is a summary iff max(hh, hw) / sqrt(D) < theta,
+    D the squared distance to its centre of mass.  Worked out by hand for
+    cell C and query p: the change of Z and of p's repulsion when theta
+    crosses C's ratio is one summary term minus two leaf terms.  A cell test
+    with the full width, with only hw (or only hh), with D for sqrt(D), or with
+    <= for < fails here; the transposed point set makes the root wider than
+    tall."""
+    Y = BH_Y[:, ::-1].copy() if transpose else BH_Y.copy()
+    m = Y.mean(0)
+    half = (np.abs(Y - m).max(0) + 1e-5) / 4.0        # C is two levels down
+    com = 0.5 * (Y[1] + Y[3])
+    d = Y[p] - com
+    D = d[0] * d[0] + d[1] * d[1]
+    r = max(half[0], half[1]) / np.sqrt(D)            # C's acceptance ratio
+
+    def q(a):
+        return 1.0 / (1.0 + np.sum((Y[p] - a) ** 2))
+
+    dz = 2 * q(com) - q(Y[1]) - q(Y[3])
+    dn = 2 * q(com) ** 2 * (Y[p] - com) - q(Y[1]) ** 2 * (Y[p] - Y[1]) - q(Y[3]) ** 2 * (Y[p] - Y[3])
+    assert abs(dz) > 1e-4 * q(com)                    # the flip is visible
+
+    n_lo, z_lo = _bh_neg_z(oracle, Y, r * (1 - 1e-9))  # C opened: two leaves
+    n_hi, z_hi = _bh_neg_z(oracle, Y, r * (1 + 1e-9))  # C accepted: one summary
+    n_at, z_at = _bh_neg_z(oracle, Y, r)               # strict <: still opened
+    np.testing.assert_allclose(z_hi - z_lo, dz, rtol=1e-8)
+    np.testing.assert_allclose((n_hi - n_lo)[p], dn, rtol=1e-8)
+    others = np.arange(len(Y)) != p
+    np.testing.assert_allclose((n_hi - n_lo)[others], 0.0, atol=1e-15)
+    assert z_at == z_lo and np.array_equal(n_at, n_lo)
+
+    # the reference KL (tsne.hpp:454-486) uses the same Z: one edge p -> 0
+    v = 0.25
+    row = np.zeros(len(Y) + 1, np.int32)
+    row[p + 1:] = 1
+    fm = np.finfo(np.float32).tiny
+    for theta, z in ((r * (1 - 1e-9), z_lo), (r * (1 + 1e-9), z_hi), (r, z_at)):
+        C = oracle.kl_bh(row, np.array([0], np.int32), np.array([v]), Y, theta)
+        qp = 1.0 / (1.0 + np.sum((Y[p] - Y[0]) ** 2))
+        assert abs(C - v * np.log((v + fm) / (qp / z + fm))) < 1e-13
+
+
+def test_bh_root_summary_closed_form(oracle):
+    """With theta above every root ratio each point sees the whole tree as
+    one summary of N points at the mean, itself included (quadtree.hpp:335-343):
+    Z = sum_p N q(p, mean), neg_f[p] = N q(p, mean)^2 (p - mean)."""
+    Y = BH_Y.copy()
+    m = Y.mean(0)
+    qm = 1.0 / (1.0 + np.sum((Y - m) ** 2, axis=1))
+    neg, z = _bh_neg_z(oracle, Y, 1e6)
+    np.testing.assert_allclose(z, len(Y) * qm.sum(), rtol=1e-13)
+    np.testing.assert_allclose(neg, len(Y) * (qm ** 2)[:, None] * (Y - m), rtol=1e-12, atol=1e-16)
+    # and just below the smallest root ratio the root is opened for that point
+    s = np.abs(Y - m).max() + 1e-5
+    rmin = (s / np.sqrt(np.sum((Y - m) ** 2, axis=1))).min()
+    _, z2 = _bh_neg_z(oracle, Y, rmin * (1 - 1e-9))
+    assert abs(z2 - z) > 1e-6 * z
+
+
 def test_quadtree_cells_by_membership(oracle):
     Y = rng(5).normal(size=(300, 2))
     t = oracle.quadtree(Y)
