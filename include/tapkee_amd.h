@@ -172,6 +172,13 @@ TKA_API tka_status tka_session_kl(tka_session* s, double* kl);
 TKA_API tka_status tka_session_stats(tka_session* s, tka_tsne_stats* stats);
 /* enable != 0: record HIP events around every kernel class in later steps. */
 TKA_API tka_status tka_session_profile(tka_session* s, int32_t enable);
+/* Test hook for the error path of the Barnes-Hut tree build: marks the
+ * session's Morton sort as failed, as a tile look-back that gave up would
+ * (the sort itself stays valid).  Later steps skip the repulsion walk, and
+ * tka_session_embedding / tka_session_kl return TKA_E_INTERNAL.  No effect
+ * at theta == 0.  (No counterpart in the reference, whose tree has no
+ * device sort.) */
+TKA_API tka_status tka_session_inject_sort_fault(tka_session* s);
 /* Synchronises, then returns and clears the accumulated timings. */
 TKA_API tka_status tka_session_kernel_timing(tka_session* s, tka_kernel_timing* t);
 TKA_API void tka_session_destroy(tka_session* s);

@@ -60,6 +60,7 @@ def lib():
     L.or_normalize.argtypes = [_dp, C.c_int, C.c_int]
     L.or_sqdist.argtypes = [_dp, C.c_int, C.c_int, _dp, C.c_int]
     L.or_knn_brute.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, _dp]
+    L.or_knn_brute_rows.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, C.c_int, _ip, _dp]
     L.or_vptree_knn.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _ip, _dp]
     L.or_vptree_knn_sample.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _dp, _f64p, _f64p]
     L.or_vptree_knn_sample.restype = C.c_int
@@ -140,6 +141,16 @@ def knn_brute(X, K):
     idx = np.empty((N, K), np.int32)
     dist = np.empty((N, K))
     lib().or_knn_brute(X, N, X.shape[1], K, idx, dist)
+    return idx, dist
+
+
+def knn_brute_rows(X, K, rows):
+    """Exact kNN of the query rows `rows` only (FP64, ties by index)."""
+    X = _f64(X)
+    rows = _i32(rows)
+    idx = np.empty((len(rows), K), np.int32)
+    dist = np.empty((len(rows), K))
+    lib().or_knn_brute_rows(X, X.shape[0], X.shape[1], K, rows, len(rows), idx, dist)
     return idx, dist
 
 

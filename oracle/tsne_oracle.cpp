@@ -141,21 +141,35 @@ OR_API void or_sqdist(const double* X, int N, int D, double* DD, int compat)
 // self excluded.  This is the exact answer tapkee's VP-tree approximates
 // (tsne.hpp:628-646 searches K+1 and drops the first hit as "self").
 // ---------------------------------------------------------------------------
+static void knn_brute_row(const double* X, int N, int D, int K, int n, std::vector<std::pair<double, int>>& row,
+                          int* idx, double* dist)
+{
+    int c = 0;
+    for (int m = 0; m < N; m++) {
+        if (m == n) continue;
+        row[c++] = std::make_pair(sqdist_rows(X + (size_t)n * D, X + (size_t)m * D, D), m);
+    }
+    std::partial_sort(row.begin(), row.begin() + K, row.begin() + c);
+    for (int k = 0; k < K; k++) {
+        idx[k] = row[k].second;
+        dist[k] = row[k].first;
+    }
+}
+
 OR_API void or_knn_brute(const double* X, int N, int D, int K, int* idx, double* dist)
 {
     std::vector<std::pair<double, int>> row(N > 0 ? N - 1 : 0);
-    for (int n = 0; n < N; n++) {
-        int c = 0;
-        for (int m = 0; m < N; m++) {
-            if (m == n) continue;
-            row[c++] = std::make_pair(sqdist_rows(X + (size_t)n * D, X + (size_t)m * D, D), m);
-        }
-        std::partial_sort(row.begin(), row.begin() + K, row.begin() + c);
-        for (int k = 0; k < K; k++) {
-            idx[(size_t)n * K + k] = row[k].second;
-            dist[(size_t)n * K + k] = row[k].first;
-        }
-    }
+    for (int n = 0; n < N; n++) knn_brute_row(X, N, D, K, n, row, idx + (size_t)n * K, dist + (size_t)n * K);
+}
+
+// The same for the query rows rows[0..nrows) only (output row r belongs to
+// query rows[r]): exact neighbours of a sample of a large problem.
+OR_API void or_knn_brute_rows(const double* X, int N, int D, int K, const int* rows, int nrows, int* idx,
+                              double* dist)
+{
+    std::vector<std::pair<double, int>> row(N > 0 ? N - 1 : 0);
+    for (int r = 0; r < nrows; r++)
+        knn_brute_row(X, N, D, K, rows[r], row, idx + (size_t)r * K, dist + (size_t)r * K);
 }
 
 // VP-tree restatement (vptree.hpp:103-293).  Items are point indices into X;

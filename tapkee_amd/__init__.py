@@ -302,6 +302,11 @@ class Session:
         _lib.check(_lib.lib().tka_session_embedding(self._h, Y))
         return Y
 
+    def inject_sort_fault(self):
+        """Test hook: mark the Barnes-Hut tree's sort as failed (see
+        tka_session_inject_sort_fault); embedding() and kl() then raise."""
+        _lib.check(_lib.lib().tka_session_inject_sort_fault(self._h))
+
     def kl(self):
         v = C.c_double(0)
         _lib.check(_lib.lib().tka_session_kl(self._h, C.byref(v)))

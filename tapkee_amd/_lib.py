@@ -125,7 +125,7 @@ SYMBOLS = (
     "tka_perplexity", "tka_perplexity_dense", "tka_symmetrize", "tka_build_p", "tka_gradient", "tka_gradient_dense", "tka_kl",
     "tka_kl_dense", "tka_update", "tka_bh_tree", "tka_gradient_dev", "tka_kl_dev", "tka_session_create",
     "tka_session_step", "tka_session_synchronize", "tka_session_embedding", "tka_session_kl", "tka_session_stats",
-    "tka_session_profile", "tka_session_kernel_timing", "tka_session_destroy",
+    "tka_session_profile", "tka_session_kernel_timing", "tka_session_destroy", "tka_session_inject_sort_fault",
 )
 
 
@@ -172,6 +172,7 @@ def lib():
     L.tka_session_stats.argtypes = [C.c_void_p, C.POINTER(Stats)]
     L.tka_session_profile.argtypes = [C.c_void_p, C.c_int32]
     L.tka_session_kernel_timing.argtypes = [C.c_void_p, C.POINTER(KernelTiming)]
+    L.tka_session_inject_sort_fault.argtypes = [C.c_void_p]
     L.tka_session_destroy.argtypes = [C.c_void_p]
     L.tka_session_destroy.restype = None
     for name in SYMBOLS:

@@ -286,9 +286,13 @@ __global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ ktop,
             e = hi;
         }
     }
+    // Sorted point indices are clamped to N-1 when read: a sort whose tile
+    // look-back gave up (its fault word set) may leave positions unwritten,
+    // and every gather and scatter downstream then still stays in bounds.
+    const uint32_t pmax = (uint32_t)(N - 1);
     if (start && e - j <= kShortRun) {
         for (int i = j; i < e; i++) {
-            const uint32_t p = perm[i];
+            const uint32_t p = min(perm[i], pmax);
             const uint64_t key = kfull[p];
             int k = i;
             while (k > j && k64[k - 1] > key) {
@@ -317,7 +321,7 @@ __global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ ktop,
             uint64_t* K = sk[w];
             uint32_t* P = sp[w];
             for (int i = lane; i < len; i += 64) {
-                const uint32_t p = perm[rj + i];
+                const uint32_t p = min(perm[rj + i], pmax);
                 P[i] = p;
                 K[i] = kfull[p];
             }
@@ -337,7 +341,7 @@ __global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ ktop,
             uint64_t* K = k64 + rj;
             uint32_t* P = p2 + rj;
             for (int i = lane; i < len; i += 64) {
-                const uint32_t p = perm[rj + i];
+                const uint32_t p = min(perm[rj + i], pmax);
                 P[i] = p;
                 K[i] = kfull[p];
             }
@@ -667,13 +671,17 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
                                                    const uint32_t* __restrict__ perm, int N, T theta2,
                                                    const RootBox* __restrict__ root, T* __restrict__ neg,
                                                    double* __restrict__ zpart,
-                                                   unsigned long long* __restrict__ interactions)
+                                                   unsigned long long* __restrict__ interactions,
+                                                   const uint32_t* __restrict__ sort_fault)
 {
     __shared__ double sh[4];
     const int nb = (int)gridDim.x, xcd = (int)blockIdx.x & 7, kx = (int)blockIdx.x >> 3;
     const int lb = xcd * (nb >> 3) + min(xcd, nb & 7) + kx;
     const int j = lb * 256 + threadIdx.x;
-    const bool active = j < N;
+    // No walk over a tree whose Morton sort gave up (its nodes may hold
+    // entries that no kernel wrote): forces stay unwritten, Z partials are
+    // 0, and the host reports the fault word.
+    const bool active = j < N && *sort_fault == 0u;
     double zd = 0.0;
     unsigned cnt = 0;
     const int total = (int)pre[N];
@@ -913,13 +921,32 @@ int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* 
     (void)Y;
     const int g = bh_zblocks(tr.N);
     hipLaunchKernelGGL((bh_traverse<T>), dim3(g), dim3(256), 0, s, tr.nodes.get(), tr.pre.get(), tr.ys.get(),
-                       tr.sorted_perm, tr.N, (T)(theta * theta), tr.root.get(), neg, zpart, interactions);
+                       tr.sorted_perm, tr.N, (T)(theta * theta), tr.root.get(), neg, zpart, interactions,
+                       tr.sort_fault());
     TKA_LAUNCH_CHECK();
     return g;
 }
 
+template <class T>
+bool bh_sort_failed(const BhTree<T>& tr, hipStream_t s)
+{
+    return radix_sort_fault(tr.sort_scratch.get(), s) != 0u;
+}
+
+template <class T>
+void bh_inject_sort_fault(BhTree<T>& tr, hipStream_t s)
+{
+    const uint32_t one = 1u;
+    TKA_HIP(hipMemcpyAsync(tr.sort_scratch.get() + 1, &one, sizeof(one), hipMemcpyHostToDevice, s));
+    TKA_HIP(hipStreamSynchronize(s));
+}
+
 template struct BhTree<float>;
 template struct BhTree<double>;
+template bool bh_sort_failed<float>(const BhTree<float>&, hipStream_t);
+template bool bh_sort_failed<double>(const BhTree<double>&, hipStream_t);
+template void bh_inject_sort_fault<float>(BhTree<float>&, hipStream_t);
+template void bh_inject_sort_fault<double>(BhTree<double>&, hipStream_t);
 template void bbox_partials_device<float>(const float*, int, double*, hipStream_t);
 template void bbox_partials_device<double>(const double*, int, double*, hipStream_t);
 template void bh_build<float>(float*, int, const double*, int, BhTree<float>&, hipStream_t);

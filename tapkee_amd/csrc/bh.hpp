@@ -139,6 +139,9 @@ struct BhTree {
     DevBuf<BhNode<T>> nodes;     // pre-order, 2N - 1 at most
     DevBuf<RootBox> root;
     const uint32_t* sorted_perm = nullptr;  // valid after build
+    // Word the Morton sort sets when a tile's bounded look-back gives up
+    // (sticky: no later build clears it).
+    const uint32_t* sort_fault() const { return sort_scratch.get() + 1; }
     void alloc(int n, MemBudget* mb);
     BhTree() = default;
     BhTree(const BhTree&) = delete;
@@ -176,5 +179,15 @@ template <class T>
 int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* zpart,
                  unsigned long long* interactions, hipStream_t s);
 inline int bh_zblocks(int N) { return (int)ceil_div(N, 256); }
+
+// Synchronises the stream and tells whether any sort of this tree's builds
+// gave up (the forces of such builds are not valid).
+template <class T>
+bool bh_sort_failed(const BhTree<T>& tr, hipStream_t s);
+
+// Sets the tree's sort fault word from the host, as a sort whose look-back
+// gave up would (for tests of the error path: the sort itself stays valid).
+template <class T>
+void bh_inject_sort_fault(BhTree<T>& tr, hipStream_t s);
 
 }  // namespace tka

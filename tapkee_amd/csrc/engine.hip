@@ -355,6 +355,7 @@ struct tka_session {
     virtual void step(int n) = 0;
     virtual void embedding(double* Y) = 0;
     virtual double kl_exact() = 0;
+    virtual void inject_sort_fault() {}
     hipStream_t s() const { return stream.s; }
 };
 
@@ -436,7 +437,7 @@ struct Session : tka_session {
             renumbered = true;
         }
         T* state[3] = {Ycur, L.uY.get(), L.gains.get()};
-        renumber_points_device<T>(N, dims, perm, state, 3, state_tmp.get(), L.row.get(), L.col.get(), L.val.get(),
+        renumber_points_device<T>(N, dims, perm, tree.sort_fault(), state, 3, state_tmp.get(), L.row.get(), L.col.get(), L.val.get(),
                                   row2.get(), col2.get(), val2.get(), orig.get(), orig_tmp.get(), inv.get(),
                                   rn_scratch.get(), s);
         std::swap(L.row, row2);
@@ -584,8 +585,13 @@ struct Session : tka_session {
     // left a fault word (never expected: every wait is on a running tile).
     void check_sort()
     {
-        if (!exact && radix_sort_fault(tree.sort_scratch.get(), s()))
+        if (!exact && tree.N > 0 && bh_sort_failed(tree, s()))
             fail(TKA_E_INTERNAL, "radix sort look-back gave up; the tree of some iteration was wrong");
+    }
+
+    void inject_sort_fault() override
+    {
+        if (!exact && tree.N > 0) bh_inject_sort_fault(tree, s());
     }
 
     void embedding(double* Y) override
@@ -812,6 +818,7 @@ void gradient_impl(const double* Yh, int N, int dims, const int* rowh, const int
         tree.alloc(N, &mb);
         bh_build<T>(Y.get(), N, box.get(), bbox_blocks(N), tree, s);
         bh_repulsion<T>(tree, Y.get(), theta, neg.get(), zpart.get(), nullptr, s);
+        require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
         gradient_combine_device<T>(dims, Y.get(), neg.get(), nullptr, row.get(), col.get(), val.get(), 1.0,
                                    zpart.get(), nz, N, dC.get(), s);
     } else {
@@ -956,6 +963,14 @@ TKA_API tka_status tka_session_profile(tka_session* s, int32_t enable)
     return guard([&] {
         require(s != nullptr, TKA_E_WRONG_PARAMETER, "null session");
         s->profile = enable != 0;
+    });
+}
+
+TKA_API tka_status tka_session_inject_sort_fault(tka_session* s)
+{
+    return guard([&] {
+        require(s != nullptr, TKA_E_WRONG_PARAMETER, "null session");
+        s->inject_sort_fault();
     });
 }
 
@@ -1178,6 +1193,7 @@ TKA_API tka_status tka_kl(const double* Y, int32_t N, int32_t no_dims, const int
             tree.alloc(N, &mb);
             bh_build<double>(Yd.get(), N, box.get(), bbox_blocks(N), tree, s);
             bh_repulsion<double>(tree, Yd.get(), theta, neg.get(), zpart.get(), nullptr, s);
+            require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
             const double Z = sum_device(zpart.get(), nz, &mb, s);
             *kl = kl_edges_reference_device<double>(no_dims, Yd.get(), row.get(), colv.get(), vd.get(), 1.0, Z, N, &mb, s);
         } else {
@@ -1273,6 +1289,7 @@ TKA_API tka_status tka_bh_tree(const double* Y, int32_t N, int32_t precision, in
                 cast_device<double, T>(tmp.get(), y.get(), (size_t)N * 2, s);
                 bbox_partials_device<T>(y.get(), N, box.get(), s);
                 bh_build<T>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
+                require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
                 download(&total, tree.pre.get() + N, 1, s);
                 download(&rb, tree.root.get(), 1, s);
                 st.sync();
@@ -1341,6 +1358,7 @@ TKA_API tka_status tka_gradient_dev(const float* Y, int32_t N, int32_t no_dims, 
             tree.alloc(N, &mb);
             bh_build<float>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
             bh_repulsion<float>(tree, y.get(), theta, neg.get(), zpart.get(), nullptr, s);
+            require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
             gradient_combine_device<float>(no_dims, y.get(), neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(),
                                            nz, N, dY, s);
             const double z = sum_device(zpart.get(), nz, &mb, s);
@@ -1380,6 +1398,7 @@ TKA_API tka_status tka_kl_dev(const float* Y, int32_t N, int32_t no_dims, const 
             tree.alloc(N, &mb);
             bh_build<float>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
             bh_repulsion<float>(tree, y.get(), theta, neg.get(), zpart.get(), nullptr, s);
+            require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
             Z = sum_device(zpart.get(), nz, &mb, s);
         } else {
             Z = exact_z_device<float>(no_dims, Y, N, &mb, s);

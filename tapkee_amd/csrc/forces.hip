@@ -452,6 +452,17 @@ __global__ __launch_bounds__(256) void sum_fixed(const double* __restrict__ v, i
 }
 
 // ---------------------------------------------------------------- renumbering
+// The renumbering's permutation: the tree's sorted point order, or the
+// identity when that tree's sort gave up (its order may repeat or miss
+// points, which would leave holes in the inverse).
+__global__ __launch_bounds__(256) void rn_checked_perm(const uint32_t* __restrict__ perm,
+                                                       const uint32_t* __restrict__ sort_fault, int N,
+                                                       uint32_t* __restrict__ out)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j < N) out[j] = *sort_fault == 0u ? perm[j] : (uint32_t)j;
+}
+
 __global__ __launch_bounds__(256) void rn_inverse(const uint32_t* __restrict__ perm, int N, int* __restrict__ inv)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
@@ -532,16 +543,22 @@ void dispatch_exact(const T* Y, int N, int dd_mode, const T* P, double exag, T* 
 
 int tsne_step_blocks(int N) { return (int)ceil_div(N, kRowsPerBlock); }
 
-size_t renumber_scan_scratch_words(int N) { return (size_t)N + (size_t)N + 1 + exclusive_scan_scratch_words(N); }
+size_t renumber_scan_scratch_words(int N)
+{
+    return (size_t)N + (size_t)N + 1 + exclusive_scan_scratch_words(N) + (size_t)N;
+}
 
 template <class T>
-void renumber_points_device(int N, int dims, const uint32_t* perm, T* const* state, int nstate, T* tmp,
-                            const int* row, const int* col, const T* val, int* row2, int* col2, T* val2,
-                            int* orig, int* orig_tmp, int* inv, uint32_t* scan_scratch, hipStream_t s)
+void renumber_points_device(int N, int dims, const uint32_t* sorted, const uint32_t* sort_fault, T* const* state,
+                            int nstate, T* tmp, const int* row, const int* col, const T* val, int* row2, int* col2,
+                            T* val2, int* orig, int* orig_tmp, int* inv, uint32_t* scan_scratch, hipStream_t s)
 {
     const unsigned g = ceil_div(N, 256);
     const size_t nd = (size_t)N * dims;
     const unsigned gn = (unsigned)std::min<size_t>(8192, ceil_div(nd, 256));
+    uint32_t* perm = scan_scratch + renumber_scan_scratch_words(N) - N;
+    hipLaunchKernelGGL(rn_checked_perm, dim3(g), dim3(256), 0, s, sorted, sort_fault, N, perm);
+    TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(rn_inverse, dim3(g), dim3(256), 0, s, perm, N, inv);
     TKA_LAUNCH_CHECK();
     for (int k = 0; k < nstate; k++) {
@@ -808,8 +825,9 @@ double kl_dense_device(int dims, const T* Y, const T* P, double exag, int N, int
 #define TKA_INST(T)                                                                                                  \
     template void center_device<T>(int, T*, int, const double*, int, hipStream_t);                                \
     template void center_fresh_device<T>(int, T*, int, MemBudget*, hipStream_t);                                  \
-    template void renumber_points_device<T>(int, int, const uint32_t*, T* const*, int, T*, const int*, const int*,  \
-                                            const T*, int*, int*, T*, int*, int*, int*, uint32_t*, hipStream_t);     \
+    template void renumber_points_device<T>(int, int, const uint32_t*, const uint32_t*, T* const*, int, T*,         \
+                                            const int*, const int*, const T*, int*, int*, T*, int*, int*, int*,      \
+                                            uint32_t*, hipStream_t);                                                 \
     template void unpermute_rows_device<T>(int, int, const T*, const int*, T*, hipStream_t);                        \
     template int tsne_step_device<T>(int, const T*, T*, T*, T*, const T*, const T*, const int*, const int*, const T*, \
                                      const double*, int, int, StepParams, double*, T*, hipStream_t);                 \

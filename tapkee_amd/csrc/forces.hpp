@@ -79,13 +79,15 @@ double kl_edges_exact_device(int dims, const T* Y, const int* row_ptr, const int
                              double Z, int N, MemBudget* mb, hipStream_t s);
 
 // Renumbering of the points for memory locality: new index j holds old
-// point perm[j].  Gathers the per-point state (dims values each) and
-// rebuilds the CSR with rows and columns in the new numbering; orig maps
-// new indices to the caller's original indices and is permuted alongside.
+// point sorted[j] (the identity instead when *sort_fault is set: the sort
+// that produced `sorted` gave up).  Gathers the per-point state (dims values
+// each) and rebuilds the CSR with rows and columns in the new numbering;
+// orig maps new indices to the caller's original indices and is permuted
+// alongside.
 template <class T>
-void renumber_points_device(int N, int dims, const uint32_t* perm, T* const* state, int nstate, T* tmp,
-                            const int* row, const int* col, const T* val, int* row2, int* col2, T* val2,
-                            int* orig, int* orig_tmp, int* inv, uint32_t* scan_scratch, hipStream_t s);
+void renumber_points_device(int N, int dims, const uint32_t* sorted, const uint32_t* sort_fault, T* const* state,
+                            int nstate, T* tmp, const int* row, const int* col, const T* val, int* row2, int* col2,
+                            T* val2, int* orig, int* orig_tmp, int* inv, uint32_t* scan_scratch, hipStream_t s);
 size_t renumber_scan_scratch_words(int N);
 
 // Y_out[orig[j]] = Y[j] (N x dims).

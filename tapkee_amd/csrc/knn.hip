@@ -1155,11 +1155,22 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     __shared__ uint64_t s_list[4][kCap];
     const int w = threadIdx.x >> 6;
     uint64_t u[8];
+    // Every slot below cnt holds a stored candidate (seed chunks store all
+    // their slots, appends store every slot they reserve, compaction rewrites
+    // the kept prefix), so a row index >= N cannot come from the tiles; it is
+    // still never gathered: such an entry is dropped and the row is reported
+    // as failed instead of reading X out of bounds.
+    bool foreign = false;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
         const int p = r * 64 + lane;
         u[r] = p < c ? qbuf[p] : ~0ull;
+        if (u[r] != ~0ull && (uint32_t)u[r] >= (uint32_t)a.N) {
+            foreign = true;
+            u[r] = ~0ull;
+        }
     }
+    foreign = __ballot(foreign) != 0ull;
     // v_(K) exactly (radix select), then the candidates within thr, packed
     const uint32_t vk_bits = wave_select<8>(u, K, 4);
     const double thr = c >= K ? (double)ord2f(vk_bits) + 2.0 * eps : HUGE_VAL;
@@ -1204,7 +1215,7 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     if (lane == 0) {
         const float tau = a.tau[slot];
         int bad = 0;
-        if (c < K) {
+        if (c < K || foreign) {
             bad = 1;  // cannot happen with K <= N-1 unless values were non-finite
         } else if (isfinite(tau)) {
             bad = !(dk < (double)tau + qn2 - eps);
@@ -1550,6 +1561,8 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         hipLaunchKernelGGL(morton3, dim3(nbr), dim3(256), 0, s, P.get(), N, part.get(), nbr, k0.get(), v0.get());
         TKA_LAUNCH_CHECK();
         const bool alt = radix_sort_pairs<uint64_t, uint32_t>(k0.get(), v0.get(), k1.get(), v1.get(), N, 63, scr.get(), s);
+        // the order gathers rows of X: never from a sort that gave up
+        require(radix_sort_fault(scr.get(), s) == 0, TKA_E_INTERNAL, "radix sort look-back gave up");
         perm.alloc(N, mb);
         TKA_HIP(hipMemcpyAsync(perm.get(), alt ? v1.get() : v0.get(), (size_t)N * sizeof(uint32_t),
                                hipMemcpyDeviceToDevice, s));

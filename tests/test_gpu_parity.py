@@ -314,6 +314,28 @@ def test_renumbered_run_returns_caller_order(engine):
     assert (d.argmin(1) == labels).mean() > 0.95
 
 
+def test_sort_fault_is_reported(engine):
+    # The Morton sort's look-back is bounded; when it gives up it sets a
+    # fault word.  The word is set here from the host (the sort itself stays
+    # valid): steps must run to the end without touching the GPU's memory out
+    # of bounds (no walk, identity renumbering at iteration 50, a KL
+    # evaluation at 50) and the embedding and the KL must then raise.
+    X = mixture(3000, 16, k=4)
+    s = engine.Session(X, perplexity=20.0, theta=0.5, seed=3)
+    s.inject_sort_fault()
+    s.step(60)
+    with pytest.raises(engine._lib.EngineError, match="radix sort"):
+        s.embedding()
+    with pytest.raises(engine._lib.EngineError, match="radix sort"):
+        s.kl()
+    s.close()
+    # a session without the fault is unaffected
+    s = engine.Session(X, perplexity=20.0, theta=0.5, seed=3)
+    s.step(60)
+    assert np.all(np.isfinite(s.embedding()))
+    s.close()
+
+
 @pytest.mark.parametrize("n", [3000, 70_000, 600_000])
 def test_attraction_inside_build_equals_attraction_alone(engine, n):
     # Untimed steps run the attraction rows in extra workgroups of the tree

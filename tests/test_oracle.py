@@ -88,6 +88,11 @@ def test_knn_brute_matches_scipy(oracle):
     order = np.lexsort((np.broadcast_to(np.arange(400), D.shape), D), axis=1)[:, :K]
     np.testing.assert_array_equal(idx, order)
     np.testing.assert_allclose(dist, np.take_along_axis(D, order, 1), rtol=1e-12)
+    # the sampled-rows form answers the same for the rows asked
+    rows = np.array([399, 0, 17, 17, 250])
+    ri, rd = oracle.knn_brute_rows(X, K, rows)
+    np.testing.assert_array_equal(ri, order[rows])
+    np.testing.assert_array_equal(rd, dist[rows])
 
 
 def test_knn_brute_tiny_by_hand(oracle):
