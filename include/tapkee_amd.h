@@ -270,6 +270,22 @@ TKA_API tka_status tka_kl_dev(const float* Y, int32_t N, int32_t no_dims, const 
                               const int32_t* col, const float* val, double theta, double* kl, double* sum_q,
                               void* stream);
 
+/* Allocation-free variant of tka_gradient_dev: the same arguments, kernels
+ * and results (bit for bit), with every buffer beyond the inputs and outputs
+ * taken from a fixed device region that the library holds on each GPU it
+ * runs on (about 20 MB, until the process ends) instead of allocated.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).
+ * N <= TKA_NOALLOC_MAX_POINTS, else TKA_E_WRONG_PARAMETER.  The region's
+ * address is looked up on the first call on a device (that call is the
+ * set-up; later calls allocate nothing).  Calls of one process are
+ * serialised and each synchronises its stream; the region keeps the last
+ * call's contents. */
+#define TKA_NOALLOC_MAX_POINTS 131072
+TKA_API tka_status tka_gradient_dev_noalloc(const float* Y, int32_t N, int32_t no_dims, const int32_t* row_ptr,
+                                            const int32_t* col, const float* val, double theta, float* dY,
+                                            double* sum_q, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -20,6 +20,7 @@ LIB_PATH = os.path.join(HERE, "libtapkee_amd.so")
 
 # tka_status
 OK, E_WRONG_PARAMETER, E_NO_DATA, E_NOT_ENOUGH_MEMORY, E_CANCELLED, E_UNSUPPORTED, E_DEVICE, E_INTERNAL = range(8)
+NOALLOC_MAX_POINTS = 131072  # TKA_NOALLOC_MAX_POINTS
 # tka_dist_dtype
 DIST_FP32, DIST_BF16, DIST_FP8 = 0, 1, 2
 DIST_NAMES = {"fp32": DIST_FP32, "bf16": DIST_BF16, "fp8": DIST_FP8}
@@ -126,6 +127,7 @@ SYMBOLS = (
     "tka_kl_dense", "tka_update", "tka_bh_tree", "tka_gradient_dev", "tka_kl_dev", "tka_session_create",
     "tka_session_step", "tka_session_synchronize", "tka_session_embedding", "tka_session_kl", "tka_session_stats",
     "tka_session_profile", "tka_session_kernel_timing", "tka_session_destroy", "tka_session_inject_sort_fault",
+    "tka_gradient_dev_noalloc",
 )
 
 
@@ -161,6 +163,7 @@ def lib():
     L.tka_bh_tree.argtypes = [_dp, C.c_int32, C.c_int32, _i32p, _ip, _ip, _ip, _ip, _dp, _dp, _dp]
     L.tka_gradient_dev.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_double, C.c_void_p, _f64p, C.c_void_p]
+    L.tka_gradient_dev_noalloc.argtypes = L.tka_gradient_dev.argtypes
     L.tka_kl_dev.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                              C.c_double, _f64p, _f64p, C.c_void_p]
     L.tka_session_create.argtypes = [_dp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,

@@ -612,7 +612,8 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
         side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
         return;
     }
-    if (blockIdx.x == 0) {  // the pairs kernel's count, posted to the host for the next build's pass count
+    if (blockIdx.x == 0 && eq24_host != nullptr) {  // the pairs kernel's count, posted to the host for the next
+                                                     // build's pass count
         __shared__ double sh[4];
         double c = 0.0;
         for (int b = threadIdx.x; b < nmain; b += 256) c += (double)eq24[b];
@@ -827,6 +828,53 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
 }
 
 template <class T>
+void BhTree<T>::carve(int n, Arena& a)
+{
+    require(n <= kMaxTreePoints, TKA_E_UNSUPPORTED, "Barnes-Hut tree: at most 2^26 - 1 points");
+    N = n;
+    a.take(keys, n);
+    a.take(keys_alt, n);
+    a.take(perm, n);
+    a.take(perm_alt, n);
+    a.take(kfull, n);
+    passes = n <= (1 << 22) ? 3 : 4;
+    a.take(eq24, ceil_div(n, 256));
+    eq24_host = nullptr;
+    a.take(k64, n);
+    a.take(p2, n);
+    a.take(sort_scratch, radix_sort_scratch_words(n, kRsItemsCounted));
+    a.take(mask, n);
+    a.take(node_a, n);
+    a.take(node_b, n);
+    a.take(node_lvl, n);
+    a.take(ys, n);
+    a.take(pre, n + 1);
+    a.take(sx, n + 1);
+    a.take(sy, n + 1);
+    a.take(part, 2 * (size_t)ceil_div(n, kTreeScanTile));
+    a.take(tile_cnt, ceil_div(n, kTreeScanTile));
+    a.take(nodes, 2 * (size_t)n);
+    a.take(root, 1);
+}
+
+static __global__ __launch_bounds__(256) void fill_words(uint32_t* __restrict__ p, int n, uint32_t v)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = v;
+}
+
+template <class T>
+void BhTree<T>::clear_sort_state(hipStream_t s)
+{
+    // the sort keeps its per-pass counts and tickets cleared after each
+    // sort and clears the tile status words while counting; the fault word
+    // and those counters must start at zero (a fill kernel, not a memset
+    // node, so that the clearing can be captured in a graph)
+    const int n = (int)kRsHeader;
+    hipLaunchKernelGGL(fill_words, dim3(ceil_div(n, 256)), dim3(256), 0, s, sort_scratch.get(), n, 0u);
+    TKA_LAUNCH_CHECK();
+}
+
+template <class T>
 void bbox_partials_device(const T* Y, int N, double* part, hipStream_t s)
 {
     hipLaunchKernelGGL(bbox_part<T>, dim3(bbox_blocks(N)), dim3(kBoxBlock), 0, s, Y, N, part);
@@ -837,7 +885,7 @@ template <class T>
 void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s)
 {
     require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "tree storage too small");
-    if (N <= (1 << 22)) {  // hysteresis between 1/16 and 1/8 of the points
+    if (N <= (1 << 22) && tr.eq24_host) {  // hysteresis between 1/16 and 1/8 of the points
         const uint32_t eq = __atomic_load_n(tr.eq24_host, __ATOMIC_RELAXED);
         if (tr.passes == 3 && eq > (uint32_t)N / 8) tr.passes = 4;
         else if (tr.passes == 4 && eq < (uint32_t)N / 16) tr.passes = 3;

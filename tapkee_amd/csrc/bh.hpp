@@ -143,6 +143,12 @@ struct BhTree {
     // (sticky: no later build clears it).
     const uint32_t* sort_fault() const { return sort_scratch.get() + 1; }
     void alloc(int n, MemBudget* mb);
+    // The same buffers as views into a caller's region (nothing allocated):
+    // no pinned count for the adaptive sort width (3 radix passes up to 2^22
+    // points, as a fresh alloc() starts), and the sort's state words must be
+    // cleared with clear_sort_state() before the first build.
+    void carve(int n, Arena& a);
+    void clear_sort_state(hipStream_t s);
     BhTree() = default;
     BhTree(const BhTree&) = delete;
     BhTree& operator=(const BhTree&) = delete;

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <new>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -41,6 +42,30 @@ struct Error : std::runtime_error {
 inline void require(bool ok, tka_status s, const std::string& m)
 {
     if (!ok) fail(s, m);
+}
+
+// Message of the last failed entry point on this thread (tka_last_error).
+std::string& last_error();
+
+// Runs an entry point's body: exceptions become a tka_status and the message
+// of tka_last_error().
+template <class F>
+tka_status guard(F&& f)
+{
+    try {
+        f();
+        last_error().clear();
+        return TKA_OK;
+    } catch (const Error& e) {
+        last_error() = e.what();
+        return e.status;
+    } catch (const std::bad_alloc&) {
+        last_error() = "Not enough memory";
+        return TKA_E_NOT_ENOUGH_MEMORY;
+    } catch (const std::exception& e) {
+        last_error() = e.what();
+        return TKA_E_INTERNAL;
+    }
 }
 
 // Device memory comes from a process-wide cache of blocks (engine.hip):
@@ -109,9 +134,18 @@ class DevBuf {
         bytes_ = bytes;
         if (b) b->used += bytes;
     }
+    // Non-owning view of n elements at p (memory the caller keeps alive);
+    // release() forgets it without freeing.
+    void view(T* p, size_t n)
+    {
+        release();
+        p_ = p;
+        n_ = n;
+        block_ = 0;
+    }
     void release()
     {
-        if (p_) {
+        if (p_ && block_ != 0) {
             dev_free(p_, block_, dev_, stream_);
             if (budget_) budget_->used -= bytes_;
         }
@@ -145,6 +179,26 @@ class DevBuf {
 };
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Buffers carved one after another (256-byte aligned) out of one region that
+// the caller owns.  With a null base nothing is carved and `used` counts the
+// bytes a region would need.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    template <class T>
+    void take(DevBuf<T>& b, size_t n)
+    {
+        used = (used + 255) & ~(size_t)255;
+        const size_t bytes = (n ? n : 1) * sizeof(T);
+        if (base) {
+            require(used + bytes <= cap, TKA_E_INTERNAL, "scratch region too small");
+            b.view(reinterpret_cast<T*>(base + used), n);
+        }
+        used += bytes;
+    }
+};
 
 // Compute units of the current device (256 on MI355X).
 int device_cus();

@@ -68,6 +68,12 @@ void trim_cache(int dev)
 }
 }  // namespace
 
+std::string& last_error()
+{
+    thread_local std::string msg;
+    return msg;
+}
+
 int device_cus()
 {
     static int cus[64] = {};  // per device, filled on first use
@@ -138,7 +144,6 @@ void dev_free(void* p, size_t bb, int dev, hipStream_t stream) noexcept
 
 namespace {
 
-thread_local std::string g_last_error;
 
 double now_ms()
 {
@@ -768,25 +773,6 @@ void tsne_run_impl(const double* Xh, int N, int D, double* Yh, int no_dims, doub
     if (st) *st = sess->st;
 }
 
-template <class F>
-tka_status guard(F&& f)
-{
-    try {
-        f();
-        g_last_error.clear();
-        return TKA_OK;
-    } catch (const Error& e) {
-        g_last_error = e.what();
-        return e.status;
-    } catch (const std::bad_alloc&) {
-        g_last_error = "Not enough memory";
-        return TKA_E_NOT_ENOUGH_MEMORY;
-    } catch (const std::exception& e) {
-        g_last_error = e.what();
-        return TKA_E_INTERNAL;
-    }
-}
-
 template <class T>
 void gradient_impl(const double* Yh, int N, int dims, const int* rowh, const int* colh, const double* valh,
                    double theta, double* dYh, double* sum_q)
@@ -898,7 +884,7 @@ TKA_API void tka_default_options(tka_tsne_options* o)
     o->cancel = nullptr;
 }
 
-TKA_API const char* tka_last_error(void) { return g_last_error.c_str(); }
+TKA_API const char* tka_last_error(void) { return last_error().c_str(); }
 
 TKA_API int32_t tka_version(void) { return 10000; }
 

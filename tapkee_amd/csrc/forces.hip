@@ -719,6 +719,12 @@ int exact_forces_device(int dims, const T* Y, int N, int dd_mode, const T* P, do
     return exact_zblocks(N);
 }
 
+void sum_into_device(const double* v, int n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(sum_fixed, dim3(1), dim3(256), 0, s, v, n, out);
+    TKA_LAUNCH_CHECK();
+}
+
 double sum_device(const double* v, int n, MemBudget* mb, hipStream_t s)
 {
     DevBuf<double> out(1, mb);

@@ -64,6 +64,8 @@ inline int exact_zblocks(int N) { return (int)ceil_div(N, 256); }
 
 // Sum of a double array (deterministic order); result copied to the host.
 double sum_device(const double* v, int n, MemBudget* mb, hipStream_t s);
+// The same sum (same order) into out[0] on the device, asynchronously.
+void sum_into_device(const double* v, int n, double* out, hipStream_t s);
 
 // KL over CSR edges as tsne.hpp:454-486: sum p*log((p + FLT_MIN)/(q/Z + FLT_MIN)),
 // p = exag*val.  Z given.

@@ -111,3 +111,9 @@ def test_cpp_dropin_header_cpu_cases(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "all drop-in checks passed" in r.stdout
+
+
+def test_noalloc_limit_matches_header():
+    text = open(os.path.join(ROOT, "include", "tapkee_amd.h")).read()
+    m = re.search(r"#define TKA_NOALLOC_MAX_POINTS (\d+)", text)
+    assert m and int(m.group(1)) == _lib.NOALLOC_MAX_POINTS
