@@ -10,8 +10,10 @@ repulsion, CSR attraction, gains/momentum update), on P and Y resident in
 HBM; `value` is iterations per second.  The same JSON line also reports:
   knn_p_build   kNN + P build time with FP32 and with BF16 distance tiles
                 (device work; the host-to-device copy of X is reported apart)
-  accuracy      errors against FP64: kNN recall, max |dP|, gradient at a
-                fixed Y, final KL of the FP32 loop against the FP64 loop
+  accuracy      errors against FP64: kNN recall, max |dP|, the gradient at
+                the final embedding of the 1000-iteration run (FP32 vs FP64,
+                FP64 vs the oracle, Barnes-Hut vs exact), final KL of the
+                FP32 loop against the FP64 loop
   pipeline      two full 1000-iteration runs (FP32 and FP64 loops) from the
                 same P and Y0, host arrays in and out
   precision_sweep  kNN + P at N=250,000, D=128 (anisotropic Gaussian
@@ -19,10 +21,13 @@ HBM; `value` is iterations per second.  The same JSON line also reports:
   parametric    N=10,000, D=784: a 784-256-256-2 MLP trained by Adam on the
                 engine's KL autograd function vs non-parametric t-SNE
   roofline      of the dominant kernel (HIP events on the engine's stream)
-  cpu_baseline  the FP64 CPU restatement of the reference on one host core:
-                loop iterations timed, the VP-tree kNN and the perplexity
-                calibration timed on a strided sample of the queries, the
-                symmetrisation timed whole; full-run time extrapolated.
+  cpu_baseline  the FP64 CPU restatement of the reference on one host core
+                (built -march=native on the host that runs it): loop
+                iterations timed from embeddings at iterations 50, 300 and
+                900 and weighted over the run, the VP-tree kNN and the
+                perplexity calibration timed on a strided sample of the
+                queries, the symmetrisation timed whole; full-run time
+                extrapolated; CPU model and usable cores reported.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N]
 """
@@ -59,9 +64,10 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_loop(row, col, val, Y0, theta, budget_s=10.0, max_iters=40):
+def cpu_loop(row, col, val, Y0, theta, exaggeration, momentum, budget_s=4.0, max_iters=20):
     """The FP64 single-thread restatement of the reference loop (oracle),
-    timed on this host: tree build + forces + update per iteration."""
+    timed on this host from the embedding Y0: tree build + forces + update
+    per iteration, P scaled by the phase's exaggeration."""
     import numpy as np
 
     from oracle import oracle as O
@@ -69,15 +75,33 @@ def cpu_loop(row, col, val, Y0, theta, budget_s=10.0, max_iters=40):
     Y = Y0.copy()
     uY = np.zeros_like(Y)
     gains = np.ones_like(Y)
-    v = val * 1.0   # after the exaggeration phase
+    v = val * exaggeration
     n = 0
     t0 = time.perf_counter()
     while n < max_iters and (time.perf_counter() - t0) < budget_s:
         dY, _ = O.gradient_bh(row, col, v, Y, theta)
-        uY, gains, Y = O.update(dY, uY, gains, Y, 0.8)
+        uY, gains, Y = O.update(dY, uY, gains, Y, momentum)
         n += 1
     dt = time.perf_counter() - t0
     return n / dt, n, dt
+
+
+def host_cpu():
+    """CPU model and the cores this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "nproc": usable, "logical_cpus": os.cpu_count()}
 
 
 def cpu_knn_p(X, perp, idx_full, stride=50):
@@ -283,12 +307,13 @@ def main():
     att_bytes = nnz * (4 + 4 + 8) + N * (8 + 4 + 8)
     att_gbs = att_bytes / (att_ms * 1e-3) / 1e9
     traffic = int((TRAVERSE_FETCH_KIB * FETCH_SCALE + TRAVERSE_WRITE_KIB * WRITE_SCALE) * 1024)
-    roofline = {"kernel": "bh_traverse", "bound": "mfma", "achieved": round(achieved_tflops, 3),
+    roofline = {"kernel": "bh_traverse", "bound": "latency", "achieved": round(achieved_tflops, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved_tflops / PEAK_FP32_TFLOPS, 5),
                 "traffic": traffic, "interactions_per_launch": int(inter), "ms_per_launch": round(rep_ms, 5),
                 "traffic_GBps": round(traffic / (rep_ms * 1e-3) / 1e9, 1),
-                "note": "FP32 VALU work (14 flop per cell interaction) against the published FP32 peak; "
-                        "the walk is bound by dependent node loads, not by HBM (traffic_GBps)"}
+                "priced_against": "FP32 VALU peak (the kernel issues no MFMA)",
+                "note": "14 FP32 flop per cell interaction; neither the VALU nor HBM (traffic_GBps) bounds the "
+                        "walk: it is a chain of dependent node loads (load latency)"}
     kernels = {"tree_build_ms": round(build_ms, 5), "repulsion_ms": round(rep_ms, 5),
                "update_ms": round(upd_ms, 5),
                "attraction_ms": round(att_ms, 5),
@@ -299,9 +324,26 @@ def main():
                                   "without the attraction rows and the attraction runs alone at full width; "
                                   "algorithmic bytes (the Y gathers mostly hit L2)"}
 
-    # ---- accuracy against FP64 at this size
+    # ---- full runs: 1000 iterations FP32 loop vs FP64 loop from the same P and Y0
+    pipeline = None
+    y0 = np.random.Generator(np.random.PCG64(5)).normal(size=(N, 2)) * 1e-4
+    finals = {}
+    if not a.no_pipeline:
+        runs = {}
+        for prec in ("fp32", "fp64"):
+            t = time.perf_counter()
+            Y, st = tapkee_amd.tsne(X, perplexity=perp, theta=theta, dist_dtype="bf16", loop_precision=prec, y0=y0)
+            wall = time.perf_counter() - t
+            finals[prec] = Y
+            runs[prec] = {"wall_s": round(wall, 3), "loop_ms": round(st["ms_loop"], 1),
+                          "kl_final_exact": st["kl_final_exact"], "kl_trace_last": st["kl"][-1]}
+            log(f"[bench] full run {prec}: {runs[prec]}")
+        d_kl = (runs["fp32"]["kl_final_exact"] - runs["fp64"]["kl_final_exact"]) / runs["fp64"]["kl_final_exact"]
+        pipeline = {"fp32_loop": runs["fp32"], "fp64_loop": runs["fp64"], "delta_kl_rel_vs_fp64": d_kl,
+                    "iters": 1000, "note": "wall_s: host X in, host Y out, kNN (BF16 tiles) + P + 1000 iterations"}
+
+    # ---- accuracy against FP64 at this size, gradients at the final embedding
     accuracy = None
-    Yfix = sess.embedding()
     if not a.no_accuracy:
         idx32, _, f32 = tapkee_amd.knn(X, K, dist_dtype="fp32")
         idx16, _, f16 = tapkee_amd.knn(X, K, dist_dtype="bf16")
@@ -309,37 +351,30 @@ def main():
         P32 = tapkee_amd.build_p(X, perp, dist_dtype="fp32")
         P16 = tapkee_amd.build_p(X, perp, dist_dtype="bf16")
         row, col, val = P16
-        g32, z32 = tapkee_amd.gradient(Yfix, row, col, val, theta, precision="fp32")
-        g64, z64 = tapkee_amd.gradient(Yfix, row, col, val, theta, precision="fp64")
-        gex, zex = tapkee_amd.gradient(Yfix, row, col, val, 0.0, precision="fp64")
+        if "fp32" in finals:
+            Yacc, where = finals["fp32"], "the final embedding of the 1000-iteration FP32 run"
+        else:
+            Yacc, where = sess.embedding(), f"the embedding after {W + K_steps + min(K_steps, 100)} iterations"
+        g32, z32 = tapkee_amd.gradient(Yacc, row, col, val, theta, precision="fp32")
+        g64, z64 = tapkee_amd.gradient(Yacc, row, col, val, theta, precision="fp64")
+        gex, zex = tapkee_amd.gradient(Yacc, row, col, val, 0.0, precision="fp64")
+        from oracle import oracle as O
+        gor, zor = O.gradient_bh(row, col, val, Yacc, theta)
         rel = lambda u, v: float(np.linalg.norm(u - v) / np.linalg.norm(v))  # noqa: E731
         accuracy = {
             "knn_recall_bf16_vs_exact": r16, "knn_rows_exact_bf16": e16,
             "knn_certificate_failures": {"fp32": f32, "bf16": f16},
             "max_abs_dP_bf16_vs_fp32": max_dp(P16, P32),
             "gradient_rel_l2_fp32_vs_fp64": rel(g32, g64), "z_rel_fp32_vs_fp64": abs(z32 - z64) / z64,
-            "gradient_rel_l2_bh_vs_exact_fp64": rel(g64, gex),
+            "gradient_rel_l2_fp64_vs_oracle": rel(g64, gor), "z_rel_fp64_vs_oracle": abs(z64 - zor) / zor,
+            "gradient_rel_l2_bh_vs_exact_fp64": rel(g64, gex), "z_rel_bh_vs_exact_fp64": (z64 - zex) / zex,
+            "embedding_extent": float(np.abs(Yacc).max()),
             "note": "FP32 distance tiles are certified exact (FP64 re-rank + error bound), so they are the "
-                    "FP64 reference for recall; gradients at the embedding after the timed steps, theta=0.5"}
+                    f"FP64 reference for recall; gradients (theta={theta}) at {where}; the oracle is the FP64 "
+                    "pointer-quadtree restatement of the reference"}
+        if pipeline is not None:
+            accuracy["kl_rel_fp32_loop_vs_fp64_loop"] = pipeline["delta_kl_rel_vs_fp64"]
         log(f"[bench] accuracy: {accuracy}")
-
-    # ---- full runs: 1000 iterations FP32 loop vs FP64 loop from the same P and Y0
-    pipeline = None
-    if not a.no_pipeline:
-        y0 = np.random.Generator(np.random.PCG64(5)).normal(size=(N, 2)) * 1e-4
-        runs = {}
-        for prec in ("fp32", "fp64"):
-            t = time.perf_counter()
-            Y, st = tapkee_amd.tsne(X, perplexity=perp, theta=theta, dist_dtype="bf16", loop_precision=prec, y0=y0)
-            wall = time.perf_counter() - t
-            runs[prec] = {"wall_s": round(wall, 3), "loop_ms": round(st["ms_loop"], 1),
-                          "kl_final_exact": st["kl_final_exact"], "kl_trace_last": st["kl"][-1]}
-            log(f"[bench] full run {prec}: {runs[prec]}")
-        d_kl = (runs["fp32"]["kl_final_exact"] - runs["fp64"]["kl_final_exact"]) / runs["fp64"]["kl_final_exact"]
-        pipeline = {"fp32_loop": runs["fp32"], "fp64_loop": runs["fp64"], "delta_kl_rel_vs_fp64": d_kl,
-                    "iters": 1000, "note": "wall_s: host X in, host Y out, kNN (BF16 tiles) + P + 1000 iterations"}
-        if accuracy is not None:
-            accuracy["kl_rel_fp32_loop_vs_fp64_loop"] = d_kl
 
     sweep = None
     if not a.no_sweep:
@@ -348,19 +383,45 @@ def main():
     if not a.no_parametric:
         param = parametric_leg(tapkee_amd, datasets, perp)
 
-    # ---- CPU baseline: the reference restated in FP64, one core
+    # ---- CPU baseline: the reference restated in FP64, one core, built for
+    # this host's CPU; the loop timed at three points of a run (embeddings
+    # of an engine run from the same Y0 at iterations 50, 300 and 900: the
+    # walk's cost follows the embedding's shape)
     cpu = None
     if not a.no_cpu:
+        from oracle import oracle as O
+        O.use_native()
         row, col, val = tapkee_amd.build_p(X, perp, dist_dtype="bf16")
-        rate, n_it, dt = cpu_loop(row, col, val, Yfix, theta)
+        snap = {}
+        s2 = tapkee_amd.Session(X, perplexity=perp, theta=theta, dist_dtype="bf16", y0=y0, log_every=0)
+        done = 0
+        for it in (50, 300, 900):
+            s2.step(it - done)
+            done = it
+            snap[it] = s2.embedding()
+        s2.close()
+        phases = {50: (12.0, 0.5, 250), 300: (1.0, 0.8, 350), 900: (1.0, 0.8, 400)}  # exaggeration, momentum, iters
+        rates, total_s, n_all, t_all = {}, 0.0, 0, 0.0
+        for it, (ex, mom, span) in phases.items():
+            r, n_it, dt = cpu_loop(row, col, val, snap[it], theta, ex, mom)
+            rates[it] = round(r, 4)
+            total_s += span / r
+            n_all += n_it
+            t_all += dt
+        rate = 1000.0 / total_s
         idx_full = tapkee_amd.knn(X, K, dist_dtype="bf16")[0]
         knn_p_s, parts = cpu_knn_p(X, perp, idx_full)
-        full_s = knn_p_s + 1000.0 / rate
+        full_s = knn_p_s + total_s
+        hc = host_cpu()
         cpu = {"value": round(rate, 4), "unit": "iters/s", "cores": 1, "kind": "port",
-               "sample": f"{n_it} Barnes-Hut iterations (quadtree build + forces + update, FP64, one thread) "
-                         f"at N={N} on the engine's P from the embedding after {W + K_steps + min(K_steps, 100)} "
-                         f"iterations ({dt:.1f} s); VP-tree over all {N} points with {parts['queries_timed']} "
-                         f"strided queries searched and calibrated, symmetrisation of the full graph",
+               "cpu_model": hc["model"], "nproc": hc["nproc"], "logical_cpus": hc["logical_cpus"],
+               "build": "g++ -O3 -march=native -ffp-contract=off, one thread",
+               "sample": f"{n_all} Barnes-Hut iterations (quadtree build + forces + update, FP64, one thread) at "
+                         f"N={N} on the engine's P, timed from the embeddings of a run at iterations 50 (P x12), "
+                         f"300 and 900 ({t_all:.1f} s) and weighted over iterations 0-249, 250-599 and 600-999; "
+                         f"VP-tree over all {N} points with {parts['queries_timed']} strided queries searched and "
+                         f"calibrated, symmetrisation of the full graph",
+               "rates_at_iteration": rates,
                "knn_p_s_est": round(knn_p_s, 3), "knn_p_parts": parts, "full_run_s_est": round(full_s, 2)}
         log(f"[bench] cpu baseline: {cpu}")
     sess.close()

@@ -32,6 +32,18 @@ def build(force: bool = False) -> str:
     return _SO
 
 
+def use_native() -> str:
+    """Build the restatement for this host's CPU (-march=native, same
+    -ffp-contract=off arithmetic) and load it instead of the portable build;
+    for timing on the machine that runs it.  Returns the library path."""
+    global _SO, _lib
+    subprocess.run(["make", "-B", "-C", _HERE, "native"], check=True, stdout=subprocess.DEVNULL)
+    _SO = os.path.join(_HERE, "_build", "liboracle_tsne_native.so")
+    _lib = None
+    lib()
+    return _SO
+
+
 class RunOptions(C.Structure):
     _fields_ = [
         ("max_iter", C.c_int), ("stop_lying_iter", C.c_int), ("mom_switch_iter", C.c_int),
