@@ -18,6 +18,8 @@ HBM; `value` is iterations per second.  The same JSON line also reports:
                 same P and Y0, host arrays in and out
   precision_sweep  kNN + P at N=250,000, D=128 (anisotropic Gaussian
                 mixture) with FP32, BF16 and FP8 e4m3 distance tiles
+  exact_theta0  workload 1 (theta=0, N=2000): exact-loop iterations/s with the
+                dense and the kNN P, all-pairs force kernel time (and at N=10k)
   parametric    N=10,000, D=784: a 784-256-256-2 MLP trained by Adam on the
                 engine's KL autograd function vs non-parametric t-SNE
   roofline      of the dominant kernel (HIP events on the engine's stream)
@@ -189,6 +191,36 @@ def precision_sweep(tapkee_amd, datasets, perp=30.0):
     return out
 
 
+def exact_leg(tapkee_amd, datasets, perp=30.0, steps=500):
+    """Workload 1 (theta=0, N=2000, D=50): iterations/s of the exact loop with
+    the reference's dense P and with the kNN P (K=90), and the all-pairs
+    force launches (tiles + slice reduction, HIP events) at N=2000 and, on
+    workload 5's data, N=10000; FP32 work 20 flop per ordered pair."""
+    out = {"workload": "gaussian mixture N=2000 D=50 perplexity=30 theta=0 (workload 1)"}
+    X1, _ = datasets.gaussian_mixture()
+    X5, _ = datasets.autograd_mixture()
+    for name, X, mode in (("dense_p", X1, "dense"), ("knn_p", X1, "knn"), ("knn_p_n10000", X5, "knn")):
+        n = X.shape[0]
+        s = tapkee_amd.Session(X, perplexity=perp, theta=0.0, exact_p=mode, seed=1, log_every=0)
+        s.step(20)
+        s.synchronize()
+        k = steps if n <= 2000 else 100
+        t = time.perf_counter()
+        s.step(k)
+        s.synchronize()
+        dt = time.perf_counter() - t
+        s.profile(True)
+        s.step(50)
+        kt = s.kernel_timing()
+        s.close()
+        ms = kt["ms_exact"] / max(kt["launches_exact"], 1)
+        tflops = 20.0 * n * (n - 1) / (ms * 1e-3) / 1e12
+        out[name] = {"n": n, "iters_per_s": round(k / dt, 1), "ms_per_iter": round(dt * 1e3 / k, 4),
+                     "exact_forces_ms": round(ms, 5), "exact_forces_tflops": round(tflops, 2)}
+        log(f"[bench] exact {name}: {out[name]}")
+    return out
+
+
 def parametric_leg(tapkee_amd, datasets, perp=30.0, steps=500):
     """Workload 5: a 784-256-256-2 FP32 MLP trained by full-batch Adam
     (lr 1e-3, 500 steps) on KL(P||Q) through the engine's autograd function
@@ -228,6 +260,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the N=250k precision sweep")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error-vs-FP64 measurements")
     ap.add_argument("--no-parametric", action="store_true", help="skip the parametric (MLP) t-SNE leg")
+    ap.add_argument("--no-exact", action="store_true", help="skip the theta=0 (workload 1) leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run 3 plain device-to-device copies of MIB MiB (a known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -379,6 +412,9 @@ def main():
     sweep = None
     if not a.no_sweep:
         sweep = precision_sweep(tapkee_amd, datasets, perp)
+    exact = None
+    if not a.no_exact:
+        exact = exact_leg(tapkee_amd, datasets, perp)
     param = None
     if not a.no_parametric:
         param = parametric_leg(tapkee_amd, datasets, perp)
@@ -442,6 +478,7 @@ def main():
         "cpu_baseline": cpu,
         "pipeline": pipeline,
         "precision_sweep": sweep,
+        "exact_theta0": exact,
         "parametric": param,
     }
     if cpu:

@@ -270,6 +270,18 @@ TKA_API tka_status tka_kl_dev(const float* Y, int32_t N, int32_t no_dims, const 
                               const int32_t* col, const float* val, double theta, double* kl, double* sum_q,
                               void* stream);
 
+/* KL(P||Q) and its gradient in one call on device arrays: the forward of
+ * the autograd function, which keeps the gradient for its backward.  theta
+ * == 0: one all-pairs pass gives the repulsion and Z, and the KL
+ * (sum p log(p Z (1 + d^2)) over P's pattern) uses that Z; theta > 0: the
+ * Barnes-Hut tree gives both.  dY receives pos - neg/Z (no factor 4, as
+ * tsne.hpp:325-407).  precision TKA_FP32: Y, val, dY are float; TKA_FP64:
+ * double.  *kl and *sum_q (may be NULL) on the host; synchronises the
+ * stream. */
+TKA_API tka_status tka_kl_grad_dev(const void* Y, int32_t N, int32_t no_dims, const int32_t* row_ptr,
+                                   const int32_t* col, const void* val, double theta, int32_t precision, double* kl,
+                                   void* dY, double* sum_q, void* stream);
+
 /* Allocation-free variant of tka_gradient_dev: the same arguments, kernels
  * and results (bit for bit), with every buffer beyond the inputs and outputs
  * taken from a fixed device region that the library holds on each GPU it

@@ -127,7 +127,7 @@ SYMBOLS = (
     "tka_kl_dense", "tka_update", "tka_bh_tree", "tka_gradient_dev", "tka_kl_dev", "tka_session_create",
     "tka_session_step", "tka_session_synchronize", "tka_session_embedding", "tka_session_kl", "tka_session_stats",
     "tka_session_profile", "tka_session_kernel_timing", "tka_session_destroy", "tka_session_inject_sort_fault",
-    "tka_gradient_dev_noalloc",
+    "tka_gradient_dev_noalloc", "tka_kl_grad_dev",
 )
 
 
@@ -164,6 +164,8 @@ def lib():
     L.tka_gradient_dev.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_double, C.c_void_p, _f64p, C.c_void_p]
     L.tka_gradient_dev_noalloc.argtypes = L.tka_gradient_dev.argtypes
+    L.tka_kl_grad_dev.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_double, C.c_int32, _f64p, C.c_void_p, _f64p, C.c_void_p]
     L.tka_kl_dev.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                              C.c_double, _f64p, _f64p, C.c_void_p]
     L.tka_session_create.argtypes = [_dp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,

@@ -1,11 +1,15 @@
 """KL(P||Q) as a torch.autograd.Function on ROCm tensors.
 
 forward  : KL(P||Q) = sum_ij p_ij log(p_ij / q_ij) over the CSR pattern of P,
-           q_ij = (1 + |y_i - y_j|^2)^-1 / Z with Z recomputed on the device
-           (exact FP64 pair sum at theta == 0, Barnes-Hut estimate at theta > 0).
-backward : dKL/dY = 4 * (F_attr - F_rep / Z), computed by the engine's force
-           kernels (the gradient tapkee returns, tsne.hpp:325-407, times 4).
-Nothing of size N^2 is stored: Y, P and theta are saved, Z is recomputed.
+           q_ij = (1 + |y_i - y_j|^2)^-1 / Z, and in the same call its
+           gradient (tka_kl_grad_dev): at theta == 0 one all-pairs pass on
+           the device gives the repulsion and Z, which the KL reuses; at
+           theta > 0 the Barnes-Hut tree gives both.
+backward : dKL/dY = 4 * (F_attr - F_rep / Z), the gradient the forward left
+           (the one tapkee returns, tsne.hpp:325-407, times 4), scaled by the
+           incoming gradient.
+Nothing of size N^2 is stored: the forward keeps Y's gradient (N x d).
+float32 and float64 tensors are both accepted (P's values in Y's dtype).
 
 `TSNELoss` wraps it for a fixed P built from data by the engine, so any
 torch optimiser or a parametric map X -> Y can be driven by it.
@@ -32,8 +36,8 @@ def _stream(t: torch.Tensor) -> int:
 def _check_inputs(Y, row_ptr, col, val):
     if not Y.is_cuda:
         raise _lib.DeviceError("KLDivergence runs on the GPU: Y must be a device tensor")
-    if Y.dtype != torch.float32 or val.dtype != torch.float32:
-        raise _lib.WrongParameterError("Y and P values must be float32")
+    if Y.dtype not in (torch.float32, torch.float64) or val.dtype != Y.dtype:
+        raise _lib.WrongParameterError("Y must be float32 or float64 and P's values of the same dtype")
     if row_ptr.dtype != torch.int32 or col.dtype != torch.int32:
         raise _lib.WrongParameterError("row_ptr and col must be int32")
     if Y.dim() != 2 or row_ptr.numel() != Y.shape[0] + 1:
@@ -45,24 +49,21 @@ class KLDivergence(torch.autograd.Function):
     def forward(ctx, Y, row_ptr, col, val, theta: float = 0.0):
         _check_inputs(Y, row_ptr, col, val)
         Y = Y.contiguous()
+        val = val.contiguous()
         kl = C.c_double(0.0)
         z = C.c_double(0.0)
         N, d = Y.shape
-        _lib.check(_lib.lib().tka_kl_dev(Y.data_ptr(), N, d, row_ptr.data_ptr(), col.data_ptr(), val.data_ptr(),
-                                         float(theta), C.byref(kl), C.byref(z), _stream(Y)))
-        ctx.save_for_backward(Y, row_ptr, col, val)
-        ctx.theta = float(theta)
+        dY = torch.empty_like(Y)
+        prec = _lib.FP64 if Y.dtype == torch.float64 else _lib.FP32
+        _lib.check(_lib.lib().tka_kl_grad_dev(Y.data_ptr(), N, d, row_ptr.data_ptr(), col.data_ptr(),
+                                              val.data_ptr(), float(theta), prec, C.byref(kl), dY.data_ptr(),
+                                              C.byref(z), _stream(Y)))
+        ctx.save_for_backward(dY)
         return torch.tensor(kl.value, dtype=Y.dtype, device=Y.device)
 
     @staticmethod
     def backward(ctx, grad_out):
-        Y, row_ptr, col, val = ctx.saved_tensors
-        N, d = Y.shape
-        dY = torch.empty_like(Y)
-        z = C.c_double(0.0)
-        _lib.check(_lib.lib().tka_gradient_dev(Y.data_ptr(), N, d, row_ptr.data_ptr(), col.data_ptr(),
-                                               val.data_ptr(), ctx.theta, dY.data_ptr(), C.byref(z),
-                                               _stream(Y)))
+        (dY,) = ctx.saved_tensors
         return 4.0 * dY * grad_out, None, None, None, None
 
 
@@ -81,6 +82,9 @@ class TSNELoss(torch.nn.Module):
         self.register_buffer("row_ptr", torch.from_numpy(row).to(device))
         self.register_buffer("col", torch.from_numpy(col).to(device))
         self.register_buffer("val", torch.from_numpy(val.astype(np.float32)).to(device))
+        self.register_buffer("val64", torch.from_numpy(val).to(device))
 
     def forward(self, Y):
-        return KLDivergence.apply(Y, self.row_ptr, self.col, self.val, self.theta)
+        val = self.val64 if Y.dtype == torch.float64 else self.val
+        return KLDivergence.apply(Y, self.row_ptr, self.col, val, self.theta)
+

@@ -288,6 +288,7 @@ void validate(int N, int D, int no_dims, double perplexity, double theta)
 template <class T>
 struct LoopState {
     DevBuf<T> Ya, Yb, uY, gains, neg, pos;
+    DevBuf<T> work;  // theta == 0: per-slice row sums of the all-pairs forces
     DevBuf<double> zpart, box;
     DevBuf<int> row, col;
     DevBuf<T> val;   // CSR values (sparse) or dense P (N x N)
@@ -471,6 +472,7 @@ struct Session : tka_session {
         nz = exact ? exact_zblocks(N) : bh_zblocks(N);
         L.zpart.alloc(nz, m);
         if (L.dense || !exact) L.pos.alloc(nd, m);
+        if (exact) L.work.alloc(exact_work_elems(N, dims, L.dense), m);
         // tsne.hpp:75-80: uY = 0, gains = 1; Y0 is not centred before iteration 0
         bbox_partials_device<T>(L.Ya.get(), N, L.box.get(), s);
         if (!exact) tree.alloc(N, m);
@@ -495,7 +497,7 @@ struct Session : tka_session {
             if (exact) {
                 if (profile) timers[kTimExact].begin(s);
                 exact_forces_device<T>(dims, Ycur, N, dd_mode, L.dense ? L.val.get() : nullptr, exag, L.neg.get(),
-                                       L.dense ? L.pos.get() : nullptr, L.zpart.get(), s);
+                                       L.dense ? L.pos.get() : nullptr, L.zpart.get(), L.work.get(), s);
                 if (profile) timers[kTimExact].end(s);
             } else if (!tree_ready) {
                 if (iter > 0 && iter % kRenumberEvery == 0 && tree.sorted_perm != nullptr)
@@ -810,7 +812,9 @@ void gradient_impl(const double* Yh, int N, int dims, const int* rowh, const int
     } else {
         nz = exact_zblocks(N);
         zpart.alloc(nz);
-        exact_forces_device<T>(dims, Y.get(), N, TKA_DD_TEXTBOOK, nullptr, 1.0, neg.get(), nullptr, zpart.get(), s);
+        DevBuf<T> work(exact_work_elems(N, dims, false), &mb);
+        exact_forces_device<T>(dims, Y.get(), N, TKA_DD_TEXTBOOK, nullptr, 1.0, neg.get(), nullptr, zpart.get(),
+                               work.get(), s);
         gradient_combine_device<T>(dims, Y.get(), neg.get(), nullptr, row.get(), col.get(), val.get(), 1.0,
                                    zpart.get(), nz, N, dC.get(), s);
     }
@@ -837,7 +841,8 @@ void gradient_dense_impl(const double* Ph, const double* Yh, int N, int dims, in
     cast_device<double, T>(ptmp.get(), P.get(), (size_t)N * N, s);
     const int nz = exact_zblocks(N);
     DevBuf<double> zpart(nz);
-    exact_forces_device<T>(dims, Y.get(), N, dd_mode, P.get(), 1.0, neg.get(), pos.get(), zpart.get(), s);
+    DevBuf<T> work(exact_work_elems(N, dims, true), &mb);
+    exact_forces_device<T>(dims, Y.get(), N, dd_mode, P.get(), 1.0, neg.get(), pos.get(), zpart.get(), work.get(), s);
     gradient_combine_device<T>(dims, Y.get(), neg.get(), pos.get(), nullptr, nullptr, nullptr, 1.0, zpart.get(), nz,
                                N, dC.get(), s);
     DevBuf<double> out(nd);
@@ -1352,7 +1357,9 @@ TKA_API tka_status tka_gradient_dev(const float* Y, int32_t N, int32_t no_dims, 
         } else {
             nz = exact_zblocks(N);
             zpart.alloc(nz);
-            exact_forces_device<float>(no_dims, Y, N, TKA_DD_TEXTBOOK, nullptr, 1.0, neg.get(), nullptr, zpart.get(), s);
+            DevBuf<float> work(exact_work_elems(N, no_dims, false));
+            exact_forces_device<float>(no_dims, Y, N, TKA_DD_TEXTBOOK, nullptr, 1.0, neg.get(), nullptr, zpart.get(),
+                                       work.get(), s);
             gradient_combine_device<float>(no_dims, Y, neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(), nz, N,
                                            dY, s);
             const double z = sum_device(zpart.get(), nz, &mb, s);
@@ -1391,6 +1398,74 @@ TKA_API tka_status tka_kl_dev(const float* Y, int32_t N, int32_t no_dims, const 
         }
         *kl = kl_edges_exact_device<float>(no_dims, Y, row_ptr, col, val, 1.0, Z, N, &mb, s);
         if (sum_q) *sum_q = Z;
+    });
+}
+
+}  // extern "C"
+
+namespace tka {
+namespace {
+
+// KL(P||Q) and dC = pos - neg/Z at one Y on the caller's stream, sharing Z.
+template <class T>
+void kl_grad_impl(const T* Y, int N, int dims, const int* row_ptr, const int* col, const T* val, double theta,
+                  double* kl, T* dY, double* sum_q, hipStream_t s)
+{
+    MemBudget mb;
+    const size_t nd = (size_t)N * dims;
+    DevBuf<T> neg(nd);
+    DevBuf<double> zpart;
+    double Z;
+    if (theta > 0.0) {
+        require(dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut KL needs 2 dimensions");
+        DevBuf<T> y(nd);  // the tree build centres its input
+        TKA_HIP(hipMemcpyAsync(y.get(), Y, nd * sizeof(T), hipMemcpyDeviceToDevice, s));
+        const int nz = bh_zblocks(N);
+        zpart.alloc(nz);
+        DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
+        bbox_partials_device<T>(y.get(), N, box.get(), s);
+        BhTree<T> tree;
+        tree.alloc(N, &mb);
+        bh_build<T>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
+        bh_repulsion<T>(tree, y.get(), theta, neg.get(), zpart.get(), nullptr, s);
+        require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
+        gradient_combine_device<T>(2, y.get(), neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(), nz, N, dY, s);
+        Z = sum_device(zpart.get(), nz, &mb, s);
+    } else {
+        const int nz = exact_zblocks(N);
+        zpart.alloc(nz);
+        DevBuf<T> work(exact_work_elems(N, dims, false));
+        exact_forces_device<T>(dims, Y, N, TKA_DD_TEXTBOOK, nullptr, 1.0, neg.get(), nullptr, zpart.get(), work.get(),
+                               s);
+        gradient_combine_device<T>(dims, Y, neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(), nz, N, dY, s);
+        Z = sum_device(zpart.get(), nz, &mb, s);
+    }
+    *kl = kl_edges_exact_device<T>(dims, Y, row_ptr, col, val, 1.0, Z, N, &mb, s);
+    if (sum_q) *sum_q = Z;
+}
+
+}  // namespace
+}  // namespace tka
+
+extern "C" {
+
+TKA_API tka_status tka_kl_grad_dev(const void* Y, int32_t N, int32_t no_dims, const int32_t* row_ptr,
+                                   const int32_t* col, const void* val, double theta, int32_t precision, double* kl,
+                                   void* dY, double* sum_q, void* stream)
+{
+    return guard([&] {
+        require(N >= 2 && no_dims >= 1 && no_dims <= 3 && Y && row_ptr && col && val && kl && dY,
+                TKA_E_WRONG_PARAMETER, "bad arguments");
+        require(theta >= 0.0, TKA_E_WRONG_PARAMETER, "theta must be non-negative");
+        require(precision == TKA_FP32 || precision == TKA_FP64, TKA_E_WRONG_PARAMETER, "precision must be FP32 or FP64");
+        Stream st(stream);
+        AllocStream as(st.s);  // temporaries ordered on the caller's stream, which may be non-blocking
+        if (precision == TKA_FP64)
+            kl_grad_impl<double>(static_cast<const double*>(Y), N, no_dims, row_ptr, col,
+                                 static_cast<const double*>(val), theta, kl, static_cast<double*>(dY), sum_q, st.s);
+        else
+            kl_grad_impl<float>(static_cast<const float*>(Y), N, no_dims, row_ptr, col, static_cast<const float*>(val),
+                                theta, kl, static_cast<float*>(dY), sum_q, st.s);
     });
 }
 

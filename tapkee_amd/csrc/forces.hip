@@ -1,6 +1,8 @@
 // Attractive forces + optimiser step (fused), exact repulsion, KL.
 #include "forces.hpp"
 
+#include <type_traits>
+
 #include <cfloat>
 
 #include "attract.hpp"
@@ -272,15 +274,37 @@ __global__ __launch_bounds__(256) void grad_combine(const T* __restrict__ Y, con
     }
 }
 
-// Exact all-pairs repulsion; one thread per row, columns streamed through LDS.
+// 1/(1 + D).  FP32: the hardware reciprocal (1 ulp) with one Newton-Raphson
+// step, r + r(1 - a r) (3 instructions against 11 for the IEEE division); a
+// is clamped to FLT_MAX so that an overflowing D gives 0, not inf * 0 = NaN.
+// FP64: the division.
+template <class T>
+TKA_DEV T recip_one_plus(T D)
+{
+    if constexpr (std::is_same<T, float>::value) {
+        const float a = fminf(1.0f + D, FLT_MAX);
+        const float r = __builtin_amdgcn_rcpf(a);
+        return fmaf(r, fmaf(-a, r, 1.0f), r);
+    } else {
+        return (T)1 / ((T)1 + D);
+    }
+}
+
+// Exact all-pairs repulsion, tiled in two dimensions so that small N still
+// fills the device: workgroup (x, y) takes rows 256x .. 256x+255 (one per
+// thread) against the columns of slice y, streamed through LDS 256 at a time.
+// Each slice leaves its per-row sums in part (slice-major, N x DIMS each) and
+// its Z partial in zpart[y * gridDim.x + x]; exact_reduce adds the slices in
+// order, so results do not depend on scheduling.
 template <class T, int DIMS, bool kCompat, bool kDense>
-__global__ __launch_bounds__(256) void exact_forces(const T* __restrict__ Y, int N, const T* __restrict__ P, T exag,
-                                                    T* __restrict__ neg, T* __restrict__ pos,
-                                                    double* __restrict__ zpart)
+__global__ __launch_bounds__(256) void exact_tiles(const T* __restrict__ Y, int N, int cols, const T* __restrict__ P,
+                                                   T exag, T* __restrict__ part_rep, T* __restrict__ part_att,
+                                                   double* __restrict__ zpart)
 {
     __shared__ T sY[256 * DIMS];
     __shared__ double sh[4];
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = blockIdx.y * cols, c1 = min(N, c0 + cols);
     T yi[DIMS], rep[DIMS], att[DIMS];
 #pragma unroll
     for (int d = 0; d < DIMS; d++) {
@@ -288,14 +312,11 @@ __global__ __launch_bounds__(256) void exact_forces(const T* __restrict__ Y, int
         rep[d] = att[d] = (T)0;
     }
     T z = (T)0;
-    for (int j0 = 0; j0 < N; j0 += 256) {
+    for (int j0 = c0; j0 < c1; j0 += 256) {
+        const int jn = min(256, c1 - j0);
         __syncthreads();
-        for (int e = threadIdx.x; e < 256 * DIMS; e += 256) {
-            const size_t src = (size_t)j0 * DIMS + e;
-            sY[e] = (src < (size_t)N * DIMS) ? Y[src] : (T)0;
-        }
+        for (int e = threadIdx.x; e < jn * DIMS; e += 256) sY[e] = Y[(size_t)j0 * DIMS + e];
         __syncthreads();
-        const int jn = (N - j0) < 256 ? (N - j0) : 256;
         if (i < N) {
             for (int jj = 0; jj < jn; jj++) {
                 const int j = j0 + jj;
@@ -311,7 +332,7 @@ __global__ __launch_bounds__(256) void exact_forces(const T* __restrict__ Y, int
                         D += diff[d] * diff[d];
                 }
                 if (kCompat) D = (T)-2 * D;
-                const T q = (T)1 / ((T)1 + D);
+                const T q = recip_one_plus(D);
                 z += q;
                 const T q2 = q * q;
                 T pq = (T)0;
@@ -325,35 +346,52 @@ __global__ __launch_bounds__(256) void exact_forces(const T* __restrict__ Y, int
         }
     }
     if (i < N) {
+        const size_t o = ((size_t)blockIdx.y * N + i) * DIMS;
 #pragma unroll
         for (int d = 0; d < DIMS; d++) {
-            neg[(size_t)i * DIMS + d] = rep[d];
-            if (kDense) pos[(size_t)i * DIMS + d] = att[d];
+            part_rep[o + d] = rep[d];
+            if (kDense) part_att[o + d] = att[d];
         }
     }
     const double zs = block_sum((double)z, sh);
-    if (threadIdx.x == 0) zpart[blockIdx.x] = zs;
+    if (threadIdx.x == 0) zpart[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = zs;
 }
 
-// FP64 all-pairs sum of 1/(1+d^2) (textbook) or 1/(1-2<y_i,y_j>) (compat).
+// neg (and pos) = the slices' per-row sums, added in slice order.
+template <class T, bool kDense>
+__global__ __launch_bounds__(256) void exact_reduce(const T* __restrict__ part_rep, const T* __restrict__ part_att,
+                                                    size_t nd, int slices, T* __restrict__ neg, T* __restrict__ pos)
+{
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < nd; e += (size_t)gridDim.x * 256) {
+        T r = part_rep[e], a = kDense ? part_att[e] : (T)0;
+        for (int sl = 1; sl < slices; sl++) {
+            r += part_rep[(size_t)sl * nd + e];
+            if (kDense) a += part_att[(size_t)sl * nd + e];
+        }
+        neg[e] = r;
+        if (kDense) pos[e] = a;
+    }
+}
+
+// FP64 all-pairs sum of 1/(1+d^2) (textbook) or 1/(1-2<y_i,y_j>) (compat),
+// on the exact_tiles grid: zpart[y * gridDim.x + x] holds the sum over rows
+// of block x and the columns of slice y.
 template <class T, int DIMS, bool kCompat>
-__global__ __launch_bounds__(256) void pair_z(const T* __restrict__ Y, int N, double* __restrict__ zpart)
+__global__ __launch_bounds__(256) void pair_z(const T* __restrict__ Y, int N, int cols, double* __restrict__ zpart)
 {
     __shared__ double sY[256 * DIMS];
     __shared__ double sh[4];
     const int i = blockIdx.x * 256 + threadIdx.x;
+    const int c0 = blockIdx.y * cols, c1 = min(N, c0 + cols);
     double yi[DIMS];
 #pragma unroll
     for (int d = 0; d < DIMS; d++) yi[d] = (i < N) ? (double)Y[(size_t)i * DIMS + d] : 0.0;
     double z = 0.0;
-    for (int j0 = 0; j0 < N; j0 += 256) {
+    for (int j0 = c0; j0 < c1; j0 += 256) {
+        const int jn = min(256, c1 - j0);
         __syncthreads();
-        for (int e = threadIdx.x; e < 256 * DIMS; e += 256) {
-            const size_t src = (size_t)j0 * DIMS + e;
-            sY[e] = (src < (size_t)N * DIMS) ? (double)Y[src] : 0.0;
-        }
+        for (int e = threadIdx.x; e < jn * DIMS; e += 256) sY[e] = (double)Y[(size_t)j0 * DIMS + e];
         __syncthreads();
-        const int jn = (N - j0) < 256 ? (N - j0) : 256;
         if (i < N) {
             for (int jj = 0; jj < jn; jj++) {
                 if (j0 + jj == i) continue;
@@ -373,7 +411,7 @@ __global__ __launch_bounds__(256) void pair_z(const T* __restrict__ Y, int N, do
         }
     }
     const double zs = block_sum(z, sh);
-    if (threadIdx.x == 0) zpart[blockIdx.x] = zs;
+    if (threadIdx.x == 0) zpart[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = zs;
 }
 
 // mode 0: p log((p + FLT_MIN)/(q/Z + FLT_MIN))   (tsne.hpp:481-482)
@@ -514,28 +552,34 @@ __global__ __launch_bounds__(256) void rn_copy_rows(const uint32_t* __restrict__
 }
 
 template <class T, int DIMS, bool kCompat, bool kDense>
-void launch_exact(const T* Y, int N, const T* P, double exag, T* neg, T* pos, double* zpart, hipStream_t s)
+void launch_exact(const T* Y, int N, const T* P, double exag, T* neg, T* pos, double* zpart, T* work, hipStream_t s)
 {
-    hipLaunchKernelGGL((exact_forces<T, DIMS, kCompat, kDense>), dim3(exact_zblocks(N)), dim3(256), 0, s, Y, N, P,
-                       (T)exag, neg, pos, zpart);
+    const ExactGrid g = exact_grid(N);
+    const size_t nd = (size_t)N * DIMS;
+    T* part_att = kDense ? work + (size_t)g.slices * nd : nullptr;
+    hipLaunchKernelGGL((exact_tiles<T, DIMS, kCompat, kDense>), dim3(g.row_blocks, g.slices), dim3(256), 0, s, Y, N,
+                       g.cols, P, (T)exag, work, part_att, zpart);
+    TKA_LAUNCH_CHECK();
+    hipLaunchKernelGGL((exact_reduce<T, kDense>), dim3(std::min<size_t>(2048, ceil_div(nd, 256))), dim3(256), 0, s,
+                       work, part_att, nd, g.slices, neg, pos);
     TKA_LAUNCH_CHECK();
 }
 
 template <class T, int DIMS>
-void dispatch_exact(const T* Y, int N, int dd_mode, const T* P, double exag, T* neg, T* pos, double* zpart,
+void dispatch_exact(const T* Y, int N, int dd_mode, const T* P, double exag, T* neg, T* pos, double* zpart, T* work,
                     hipStream_t s)
 {
     const bool compat = dd_mode == TKA_DD_TAPKEE_COMPAT;
     if (P) {
         if (compat)
-            launch_exact<T, DIMS, true, true>(Y, N, P, exag, neg, pos, zpart, s);
+            launch_exact<T, DIMS, true, true>(Y, N, P, exag, neg, pos, zpart, work, s);
         else
-            launch_exact<T, DIMS, false, true>(Y, N, P, exag, neg, pos, zpart, s);
+            launch_exact<T, DIMS, false, true>(Y, N, P, exag, neg, pos, zpart, work, s);
     } else {
         if (compat)
-            launch_exact<T, DIMS, true, false>(Y, N, P, exag, neg, pos, zpart, s);
+            launch_exact<T, DIMS, true, false>(Y, N, P, exag, neg, pos, zpart, work, s);
         else
-            launch_exact<T, DIMS, false, false>(Y, N, P, exag, neg, pos, zpart, s);
+            launch_exact<T, DIMS, false, false>(Y, N, P, exag, neg, pos, zpart, work, s);
     }
 }
 
@@ -708,12 +752,12 @@ void gradient_combine_device(int dims, const T* Y, const T* neg, const T* pos_pr
 
 template <class T>
 int exact_forces_device(int dims, const T* Y, int N, int dd_mode, const T* P, double exag, T* neg, T* pos,
-                        double* zpart, hipStream_t s)
+                        double* zpart, T* work, hipStream_t s)
 {
     switch (dims) {
-        case 1: dispatch_exact<T, 1>(Y, N, dd_mode, P, exag, neg, pos, zpart, s); break;
-        case 2: dispatch_exact<T, 2>(Y, N, dd_mode, P, exag, neg, pos, zpart, s); break;
-        case 3: dispatch_exact<T, 3>(Y, N, dd_mode, P, exag, neg, pos, zpart, s); break;
+        case 1: dispatch_exact<T, 1>(Y, N, dd_mode, P, exag, neg, pos, zpart, work, s); break;
+        case 2: dispatch_exact<T, 2>(Y, N, dd_mode, P, exag, neg, pos, zpart, work, s); break;
+        case 3: dispatch_exact<T, 3>(Y, N, dd_mode, P, exag, neg, pos, zpart, work, s); break;
         default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
     }
     return exact_zblocks(N);
@@ -759,11 +803,13 @@ template <class T>
 double exact_z_device(int dims, const T* Y, int N, MemBudget* mb, hipStream_t s)
 {
     const int nb = exact_zblocks(N);
+    const ExactGrid g = exact_grid(N);
+    const dim3 grid(g.row_blocks, g.slices);
     DevBuf<double> part(nb, mb);
     switch (dims) {
-        case 1: hipLaunchKernelGGL((pair_z<T, 1, false>), dim3(nb), dim3(256), 0, s, Y, N, part.get()); break;
-        case 2: hipLaunchKernelGGL((pair_z<T, 2, false>), dim3(nb), dim3(256), 0, s, Y, N, part.get()); break;
-        case 3: hipLaunchKernelGGL((pair_z<T, 3, false>), dim3(nb), dim3(256), 0, s, Y, N, part.get()); break;
+        case 1: hipLaunchKernelGGL((pair_z<T, 1, false>), grid, dim3(256), 0, s, Y, N, g.cols, part.get()); break;
+        case 2: hipLaunchKernelGGL((pair_z<T, 2, false>), grid, dim3(256), 0, s, Y, N, g.cols, part.get()); break;
+        case 3: hipLaunchKernelGGL((pair_z<T, 3, false>), grid, dim3(256), 0, s, Y, N, g.cols, part.get()); break;
         default: fail(TKA_E_WRONG_PARAMETER, "no_dims must be 1, 2 or 3");
     }
     TKA_LAUNCH_CHECK();
@@ -795,12 +841,14 @@ double kl_dense_device(int dims, const T* Y, const T* P, double exag, int N, int
 {
     const bool compat = dd_mode == TKA_DD_TAPKEE_COMPAT;
     const int nbz = exact_zblocks(N);
+    const ExactGrid g = exact_grid(N);
+    const dim3 grid(g.row_blocks, g.slices);
     DevBuf<double> zp(nbz, mb);
 #define TKA_PZ(DM)                                                                                              \
     if (compat)                                                                                                 \
-        hipLaunchKernelGGL((pair_z<T, DM, true>), dim3(nbz), dim3(256), 0, s, Y, N, zp.get());                  \
+        hipLaunchKernelGGL((pair_z<T, DM, true>), grid, dim3(256), 0, s, Y, N, g.cols, zp.get());               \
     else                                                                                                        \
-        hipLaunchKernelGGL((pair_z<T, DM, false>), dim3(nbz), dim3(256), 0, s, Y, N, zp.get());
+        hipLaunchKernelGGL((pair_z<T, DM, false>), grid, dim3(256), 0, s, Y, N, g.cols, zp.get());
     switch (dims) {
         case 1: TKA_PZ(1); break;
         case 2: TKA_PZ(2); break;
@@ -843,7 +891,8 @@ double kl_dense_device(int dims, const T* Y, const T* P, double exag, int N, int
                                        StepParams, double*, T*, hipStream_t);                                        \
     template void gradient_combine_device<T>(int, const T*, const T*, const T*, const int*, const int*, const T*,    \
                                              double, const double*, int, int, T*, hipStream_t);                      \
-    template int exact_forces_device<T>(int, const T*, int, int, const T*, double, T*, T*, double*, hipStream_t);    \
+    template int exact_forces_device<T>(int, const T*, int, int, const T*, double, T*, T*, double*, T*,             \
+                                        hipStream_t);                                                                \
     template double kl_edges_reference_device<T>(int, const T*, const int*, const int*, const T*, double, double,    \
                                                  int, MemBudget*, hipStream_t);                                       \
     template double exact_z_device<T>(int, const T*, int, MemBudget*, hipStream_t);                                  \

@@ -3,6 +3,8 @@
 // theta = 0, and the KL reductions.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace tka {
@@ -54,13 +56,41 @@ void gradient_combine_device(int dims, const T* Y, const T* neg, const T* pos_pr
                              const int* col, const T* val, double exag, const double* zpart, int nz, int N, T* dC,
                              hipStream_t s);
 
+// Grid of the all-pairs kernels: row blocks of 256 rows times column slices
+// of `cols` columns, about kExactTargetWgs workgroups in all (every CU busy
+// at N = 2000 and N = 10000 alike), slices of at least kExactMinCols columns.
+constexpr int kExactTargetWgs = 2048;
+constexpr int kExactMinCols = 128;
+struct ExactGrid {
+    int row_blocks, slices, cols;
+};
+inline ExactGrid exact_grid(int N)
+{
+    const int rb = (int)ceil_div(N, 256);
+    int slices = std::min((int)ceil_div(N, kExactMinCols), std::max(1, (int)ceil_div(kExactTargetWgs, rb)));
+    slices = std::max(1, slices);
+    const int cols = (int)ceil_div(N, slices);
+    return {rb, (int)ceil_div(N, cols), cols};
+}
+// Z partials of the all-pairs kernels (one per workgroup).
+inline int exact_zblocks(int N)
+{
+    const ExactGrid g = exact_grid(N);
+    return g.row_blocks * g.slices;
+}
+// Elements of T the all-pairs forces need as workspace (per-slice row sums).
+inline size_t exact_work_elems(int N, int dims, bool dense)
+{
+    return (size_t)exact_grid(N).slices * N * dims * (dense ? 2 : 1);
+}
+
 // Exact repulsion over all pairs (tsne.hpp:353-407): neg_i = sum_j q^2 (y_i -
-// y_j), zpart = per-block sums of q.  With a dense symmetric P (N x N) also
-// pos_i = sum_j exag*P_ij q (y_i - y_j).  dd_mode as tka_dd_mode.
+// y_j), zpart = per-workgroup sums of q (exact_zblocks(N) of them).  With a
+// dense symmetric P (N x N) also pos_i = sum_j exag*P_ij q (y_i - y_j).
+// dd_mode as tka_dd_mode.  work: exact_work_elems(N, dims, P != nullptr).
 template <class T>
 int exact_forces_device(int dims, const T* Y, int N, int dd_mode, const T* P, double exag, T* neg, T* pos,
-                        double* zpart, hipStream_t s);
-inline int exact_zblocks(int N) { return (int)ceil_div(N, 256); }
+                        double* zpart, T* work, hipStream_t s);
 
 // Sum of a double array (deterministic order); result copied to the host.
 double sum_device(const double* v, int n, MemBudget* mb, hipStream_t s);

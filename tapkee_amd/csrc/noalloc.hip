@@ -55,7 +55,7 @@ char* device_scratch()
 }
 
 struct Views {
-    DevBuf<float> y, neg;
+    DevBuf<float> y, neg, work;
     DevBuf<double> zpart, box, zsum;
     BhTree<float> tree;
 };
@@ -74,6 +74,7 @@ void carve(int N, int dims, bool bh, Arena& a, Views& v)
         v.tree.carve(N, a);
     } else {
         a.take(v.zpart, exact_zblocks(N));
+        a.take(v.work, exact_work_elems(N, dims, false));
     }
 }
 
@@ -105,6 +106,10 @@ TKA_API tka_status tka_gradient_dev_noalloc(const float* Y, int32_t N, int32_t n
             Views w;
             carve(TKA_NOALLOC_MAX_POINTS, 2, true, probe, w);
             require(probe.used <= kScratchBytes, TKA_E_INTERNAL, "no-alloc scratch region too small");
+            Arena probe0;
+            Views w0;
+            carve(TKA_NOALLOC_MAX_POINTS, 3, false, probe0, w0);
+            require(probe0.used <= kScratchBytes, TKA_E_INTERNAL, "no-alloc scratch region too small");
         }
         Arena a;
         a.base = device_scratch();
@@ -123,7 +128,7 @@ TKA_API tka_status tka_gradient_dev_noalloc(const float* Y, int32_t N, int32_t n
             sum_into_device(v.zpart.get(), bh_zblocks(N), v.zsum.get(), s);
         } else {
             const int nz = exact_forces_device<float>(no_dims, Y, N, TKA_DD_TEXTBOOK, nullptr, 1.0, v.neg.get(),
-                                                      nullptr, v.zpart.get(), s);
+                                                      nullptr, v.zpart.get(), v.work.get(), s);
             gradient_combine_device<float>(no_dims, Y, v.neg.get(), nullptr, row_ptr, col, val, 1.0, v.zpart.get(),
                                            nz, N, dY, s);
             sum_into_device(v.zpart.get(), nz, v.zsum.get(), s);

@@ -105,3 +105,52 @@ def test_gradient_at_workload5_size(engine, oracle):
     gb, _ = engine.gradient(Y, row, col, val, 0.5, precision="fp64")
     rb, _ = oracle.gradient_bh(row, col, val, Y, 0.5)
     assert np.linalg.norm(gb - rb) / np.linalg.norm(rb) < 1e-9
+
+
+def test_autograd_fp64_gradcheck(engine):
+    """torch.autograd.gradcheck of the FP64 Function at theta=0 (central
+    differences of the forward against the backward, every coordinate)."""
+    import torch
+
+    from tapkee_amd.autograd import kl_divergence
+    X = mixture(48, 6)
+    row, col, val = engine.build_p(X, 5.0)
+    dev = torch.device("cuda:0")
+    Y = torch.tensor(rng(7).normal(size=(48, 2)) * 2.0, dtype=torch.float64, device=dev, requires_grad=True)
+    r, c = torch.tensor(row, device=dev), torch.tensor(col, device=dev)
+    v = torch.tensor(val, dtype=torch.float64, device=dev)
+    assert torch.autograd.gradcheck(lambda y: kl_divergence(y, r, c, v, 0.0), (Y,), eps=1e-6, atol=1e-7, rtol=1e-5)
+
+
+def test_autograd_fp64_at_workload5_size(engine, oracle):
+    """N=10,000, D=784 (workload 5) through the FP64 Function: the backward
+    against 4x the oracle's exact gradient, and against central differences
+    (h=1e-6) of the Function's own forward; the forward against the
+    oracle's KL."""
+    import torch
+
+    from tapkee_amd import datasets
+    from tapkee_amd.autograd import kl_divergence
+    X, _ = datasets.autograd_mixture()
+    X = X.astype(np.float64)
+    N = X.shape[0]
+    row, col, val = engine.build_p(X, 30.0)
+    Yh = rng(3).normal(size=(N, 2)) * 3.0
+    dev = torch.device("cuda:0")
+    r, c = torch.tensor(row, device=dev), torch.tensor(col, device=dev)
+    v = torch.tensor(val, dtype=torch.float64, device=dev)
+    Y = torch.tensor(Yh, dtype=torch.float64, device=dev, requires_grad=True)
+    loss = kl_divergence(Y, r, c, v, 0.0)
+    loss.backward()
+    g = Y.grad.cpu().numpy()
+    ref, _ = oracle.gradient_exact_sparse(row, col, val, Yh)
+    assert np.linalg.norm(g - 4 * ref) / np.linalg.norm(4 * ref) < 1e-10
+    assert abs(loss.item() - oracle.kl_exact_sparse(row, col, val, Yh)) < 1e-10 * abs(loss.item())
+    h = 1e-6
+    with torch.no_grad():
+        for (i, d) in [(0, 0), (4321, 1), (9999, 0)]:
+            Yp, Ym = Y.detach().clone(), Y.detach().clone()
+            Yp[i, d] += h
+            Ym[i, d] -= h
+            fd = (kl_divergence(Yp, r, c, v, 0.0).item() - kl_divergence(Ym, r, c, v, 0.0).item()) / (2 * h)
+            assert abs(fd - g[i, d]) < 1e-6 * max(1.0, abs(fd)), (i, d, fd, g[i, d])
