@@ -121,6 +121,32 @@ typedef struct tka_tsne_stats {
     double ms_upload;            /* host-to-device copy of X (included in ms_normalize) */
 } tka_tsne_stats;
 
+/* Logging, as the reference's Logging singleton and LoggerImplementation
+ * (include/tapkee/utils/logging.hpp:45-144): five levels, each enabled or
+ * not; a message of an enabled level goes to the installed function, by
+ * default one that writes "[level] message" to stderr.  The engine emits
+ * the reference's t-SNE messages (tsne.hpp:66,68,191,605) and, at the
+ * benchmark level, "Input similarities computation took <s> seconds." and
+ * "Main t-SNE loop took <s> seconds." (tsne.hpp:87,142; utils/time.hpp:29-42).
+ * A call whose options set `verbose` emits its info and benchmark messages
+ * whatever the levels. */
+typedef enum tka_log_level {
+    TKA_LOG_INFO = 0,
+    TKA_LOG_WARNING = 1,
+    TKA_LOG_DEBUG = 2,
+    TKA_LOG_ERROR = 3,
+    TKA_LOG_BENCHMARK = 4
+} tka_log_level;
+#define TKA_LOG_DEFAULT_LEVELS ((1u << TKA_LOG_WARNING) | (1u << TKA_LOG_ERROR)) /* as logging.hpp:96-98 */
+typedef void (*tka_log_fn)(int32_t level, const char* message, void* user);
+/* fn == NULL restores the stderr logger.  enabled_levels: bit (1 << level)
+ * per enabled level.  Process-wide; fn may be called from any thread that
+ * calls the engine. */
+TKA_API void tka_set_logger(tka_log_fn fn, void* user, uint32_t enabled_levels);
+/* Emits one message through the installed logger if its level is enabled
+ * (for front ends that log as the reference's Logging::message_* does). */
+TKA_API void tka_log(int32_t level, const char* message);
+
 TKA_API void tka_default_options(tka_tsne_options* opt);
 TKA_API const char* tka_last_error(void);
 TKA_API int32_t tka_version(void);
@@ -195,6 +221,15 @@ TKA_API tka_status tka_normalize(double* X, int32_t N, int32_t D);
  * failed and were re-solved in FP64. */
 TKA_API tka_status tka_knn(const double* X, int32_t N, int32_t D, int32_t K, int32_t dist_dtype,
                            int32_t candidates, int32_t* idx, double* dist, int32_t* certificate_failures);
+
+/* Connectivity of a neighbour graph as tapkee's is_connected
+ * (include/tapkee/neighbors/connected.hpp:19-61): depth-first search from
+ * object 0 along the directed lists idx (N x k); *connected = 1 when every
+ * object is reached.  Host only.  With tka_knn it makes tapkee's
+ * find_neighbors (neighbors.hpp:172-216) for feature vectors under the
+ * Euclidean distance: exact neighbours replace the Brute, VpTree and
+ * CoverTree searches (include/tapkee_amd/neighbors.hpp). */
+TKA_API tka_status tka_neighbors_connected(const int32_t* idx, int32_t N, int32_t k, int32_t* connected);
 
 /* Perplexity calibration of kNN rows: bisection on beta in FP64
  * (tsne.hpp:648-711).  dist: N x K squared distances.  val: N x K row

@@ -19,8 +19,10 @@
 #include <any>
 #include <cstdint>
 #include <functional>
+#include <iostream>
 #include <list>
 #include <map>
+#include <memory>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -257,6 +259,103 @@ static const stichwort_like::ParameterKeyword<IndexType> num_neighbors("number o
 static const stichwort_like::ParameterKeyword<IndexType> max_iteration("maximal iteration", 100);
 static const stichwort_like::ParameterKeyword<bool (*)()> cancel_function("cancel function", nullptr);
 static const stichwort_like::ParameterKeyword<void (*)(double)> progress_function("progress function", nullptr);
+// ---------------------------------------------------------------- logging
+// The reference's logger interface (include/tapkee/utils/logging.hpp:45-144):
+// a LoggerImplementation receives the messages of the enabled levels; the
+// Logging singleton holds it and the levels (info, debug and benchmark off,
+// warning and error on by default).  The engine's own messages (the t-SNE
+// lines of tsne.hpp and the "... took <s> seconds." benchmark lines) reach
+// the installed implementation through tka_set_logger.
+class LoggerImplementation {
+  public:
+    LoggerImplementation() = default;
+    virtual ~LoggerImplementation() = default;
+    LoggerImplementation(const LoggerImplementation&) = delete;
+    LoggerImplementation& operator=(const LoggerImplementation&) = delete;
+    virtual void message_info(const std::string& msg) = 0;
+    virtual void message_warning(const std::string& msg) = 0;
+    virtual void message_debug(const std::string& msg) = 0;
+    virtual void message_error(const std::string& msg) = 0;
+    virtual void message_benchmark(const std::string& msg) = 0;
+};
+
+// Writes "[level] message" lines to std::cerr.
+class DefaultLoggerImplementation : public LoggerImplementation {
+  public:
+    void message_info(const std::string& m) override { put("info", m); }
+    void message_warning(const std::string& m) override { put("warning", m); }
+    void message_debug(const std::string& m) override { put("debug", m); }
+    void message_error(const std::string& m) override { put("error", m); }
+    void message_benchmark(const std::string& m) override { put("benchmark", m); }
+
+  private:
+    static void put(const char* level, const std::string& m) { std::cerr << "[" << level << "] " << m << "\n"; }
+};
+
+class Logging {
+  public:
+    static Logging& instance()
+    {
+        static Logging s;
+        return s;
+    }
+    LoggerImplementation* get_logger_impl() const { return impl_.get(); }
+    // takes ownership, as the reference's setter (it deletes the previous one)
+    void set_logger_impl(LoggerImplementation* i)
+    {
+        impl_.reset(i);
+        sync();
+    }
+    void enable_info() { set(TKA_LOG_INFO, true); }
+    void disable_info() { set(TKA_LOG_INFO, false); }
+    bool is_info_enabled() const { return on(TKA_LOG_INFO); }
+    void message_info(const std::string& m) { if (on(TKA_LOG_INFO) && impl_) impl_->message_info(m); }
+    void enable_warning() { set(TKA_LOG_WARNING, true); }
+    void disable_warning() { set(TKA_LOG_WARNING, false); }
+    bool is_warning_enabled() const { return on(TKA_LOG_WARNING); }
+    void message_warning(const std::string& m) { if (on(TKA_LOG_WARNING) && impl_) impl_->message_warning(m); }
+    void enable_debug() { set(TKA_LOG_DEBUG, true); }
+    void disable_debug() { set(TKA_LOG_DEBUG, false); }
+    bool is_debug_enabled() const { return on(TKA_LOG_DEBUG); }
+    void message_debug(const std::string& m) { if (on(TKA_LOG_DEBUG) && impl_) impl_->message_debug(m); }
+    void enable_error() { set(TKA_LOG_ERROR, true); }
+    void disable_error() { set(TKA_LOG_ERROR, false); }
+    bool is_error_enabled() const { return on(TKA_LOG_ERROR); }
+    void message_error(const std::string& m) { if (on(TKA_LOG_ERROR) && impl_) impl_->message_error(m); }
+    void enable_benchmark() { set(TKA_LOG_BENCHMARK, true); }
+    void disable_benchmark() { set(TKA_LOG_BENCHMARK, false); }
+    bool is_benchmark_enabled() const { return on(TKA_LOG_BENCHMARK); }
+    void message_benchmark(const std::string& m) { if (on(TKA_LOG_BENCHMARK) && impl_) impl_->message_benchmark(m); }
+
+  private:
+    Logging() : impl_(new DefaultLoggerImplementation) { sync(); }
+    Logging(const Logging&) = delete;
+    Logging& operator=(const Logging&) = delete;
+    bool on(tka_log_level l) const { return (levels_ >> l) & 1u; }
+    void set(tka_log_level l, bool v)
+    {
+        levels_ = v ? (levels_ | (1u << l)) : (levels_ & ~(1u << l));
+        sync();
+    }
+    // the engine reports every enabled message here; the level was checked
+    static void deliver(int32_t level, const char* msg, void* self)
+    {
+        LoggerImplementation* i = static_cast<Logging*>(self)->impl_.get();
+        if (!i) return;
+        const std::string m(msg);
+        switch (level) {
+            case TKA_LOG_INFO: i->message_info(m); break;
+            case TKA_LOG_WARNING: i->message_warning(m); break;
+            case TKA_LOG_DEBUG: i->message_debug(m); break;
+            case TKA_LOG_ERROR: i->message_error(m); break;
+            default: i->message_benchmark(m); break;
+        }
+    }
+    void sync() { tka_set_logger(&Logging::deliver, this, levels_); }
+    std::unique_ptr<LoggerImplementation> impl_;
+    uint32_t levels_ = TKA_LOG_DEFAULT_LEVELS;
+};
+
 // Engine-only keyword: tka_tsne_options for the t-SNE path.
 static const stichwort_like::ParameterKeyword<EngineOptions> engine_options("MI355X engine options", EngineOptions());
 

@@ -21,7 +21,8 @@ __all__ = [
     "embed", "tsne", "Session", "withParameters", "ParametersSet", "Parameter", "TapkeeOutput", "DimensionReductionMethod",
     "parse_reduction_method", "knn", "normalize", "perplexity", "perplexity_dense", "symmetrize", "build_p",
     "gradient",
-    "gradient_dense", "kl", "kl_dense", "update", "bh_tree", "EngineError", "WrongParameterError", "NoDataError",
+    "gradient_dense", "kl", "kl_dense", "update", "bh_tree", "find_neighbors", "neighbors_connected", "set_logger",
+    "EngineError", "WrongParameterError", "NoDataError",
     "NotEnoughMemoryError", "CancelledError", "UnsupportedMethodError", "DeviceError", "MissedParameterError",
     "MultipleParameterError",
 ]
@@ -340,6 +341,28 @@ class Session:
         self.close()
 
 
+_logger_keep = None
+
+
+def set_logger(fn=None, levels=("warning", "error")):
+    """Route the engine's messages (tka_set_logger, mirroring tapkee's
+    Logging singleton): fn(level, message) receives each message of an
+    enabled level, level one of "info", "warning", "debug", "error",
+    "benchmark"; fn=None restores the default logger, which writes
+    "[level] message" to stderr.  The defaults enable warning and error, as
+    tapkee's (include/tapkee/utils/logging.hpp:96-98)."""
+    global _logger_keep
+    mask = 0
+    for name in levels:
+        mask |= 1 << _lib.LOG_LEVEL_NAMES.index(name)
+    if fn is None:
+        cb = C.cast(None, _lib.LOG_FN)
+    else:
+        cb = _lib.LOG_FN(lambda lvl, msg, user: fn(_lib.LOG_LEVEL_NAMES[lvl], msg.decode(errors="replace")))
+    _lib.lib().tka_set_logger(cb, None, mask)
+    _logger_keep = cb
+
+
 def normalize(X):
     X = _f64(X).copy()
     _lib.check(_lib.lib().tka_normalize(X, X.shape[0], X.shape[1]))
@@ -356,6 +379,45 @@ def knn(X, K, dist_dtype="fp32", candidates=0):
     _lib.check(_lib.lib().tka_knn(X, N, D, K, _lib.DIST_NAMES[dist_dtype], candidates, idx, dist,
                                   C.byref(fails)))
     return idx, dist, fails.value
+
+
+def neighbors_connected(idx):
+    """tapkee's is_connected (include/tapkee/neighbors/connected.hpp:19-61):
+    DFS from object 0 along the directed neighbour lists idx (N x k)."""
+    idx = _i32(idx)
+    N, k = idx.shape
+    out = C.c_int32(0)
+    _lib.check(_lib.lib().tka_neighbors_connected(idx, N, k, C.byref(out)))
+    return bool(out.value)
+
+
+def _log(level, msg):
+    _lib.lib().tka_log(level, msg.encode())
+
+
+def find_neighbors(X, k, check_connectivity=True, method="Brute-force", dist_dtype="bf16"):
+    """tapkee's find_neighbors (include/tapkee/neighbors/neighbors.hpp:172-216)
+    for the rows of X (N x D) under the Euclidean distance, on the GPU: the
+    exact k nearest other rows of each row, nearest first (ties by index),
+    whatever the method named.  k above N-1 is lowered with the reference's
+    warning; with check_connectivity a graph that is not connected is
+    recomputed with 2k neighbours.  Returns (idx N x k_used, k_used)."""
+    X = _f64(X)
+    N = X.shape[0]
+    if N == 0:
+        raise NoDataError("no data to search")
+    if k > N - 1:
+        _log(_lib.LOG_WARNING, "Number of neighbors is greater than number of objects to embed. "
+                               "Using greatest possible number of neighbors.")
+        k = N - 1
+    _log(_lib.LOG_INFO, f"Using the {method} neighbors computation method.")
+    idx, _, _ = knn(X, k, dist_dtype=dist_dtype)
+    if check_connectivity and not neighbors_connected(idx):
+        _log(_lib.LOG_WARNING, f"The neighborhood graph with {k} neighbors is not connected. "
+                               f"Recomputing with a larger number of neighbors {2 * k}.")
+        return find_neighbors(X, 2 * k, check_connectivity, method, dist_dtype)
+    _log(_lib.LOG_INFO, f"The neighborhood graph with {k} neighbors is connected.")
+    return idx, k
 
 
 def perplexity(dist, perp):

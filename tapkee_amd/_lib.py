@@ -21,6 +21,10 @@ LIB_PATH = os.path.join(HERE, "libtapkee_amd.so")
 # tka_status
 OK, E_WRONG_PARAMETER, E_NO_DATA, E_NOT_ENOUGH_MEMORY, E_CANCELLED, E_UNSUPPORTED, E_DEVICE, E_INTERNAL = range(8)
 NOALLOC_MAX_POINTS = 131072  # TKA_NOALLOC_MAX_POINTS
+LOG_INFO, LOG_WARNING, LOG_DEBUG, LOG_ERROR, LOG_BENCHMARK = range(5)
+LOG_LEVEL_NAMES = ("info", "warning", "debug", "error", "benchmark")
+LOG_DEFAULT_LEVELS = (1 << LOG_WARNING) | (1 << LOG_ERROR)
+LOG_FN = C.CFUNCTYPE(None, C.c_int32, C.c_char_p, C.c_void_p)
 # tka_dist_dtype
 DIST_FP32, DIST_BF16, DIST_FP8 = 0, 1, 2
 DIST_NAMES = {"fp32": DIST_FP32, "bf16": DIST_BF16, "fp8": DIST_FP8}
@@ -127,7 +131,7 @@ SYMBOLS = (
     "tka_kl_dense", "tka_update", "tka_bh_tree", "tka_gradient_dev", "tka_kl_dev", "tka_session_create",
     "tka_session_step", "tka_session_synchronize", "tka_session_embedding", "tka_session_kl", "tka_session_stats",
     "tka_session_profile", "tka_session_kernel_timing", "tka_session_destroy", "tka_session_inject_sort_fault",
-    "tka_gradient_dev_noalloc", "tka_kl_grad_dev",
+    "tka_gradient_dev_noalloc", "tka_kl_grad_dev", "tka_set_logger", "tka_log", "tka_neighbors_connected",
 )
 
 
@@ -180,8 +184,14 @@ def lib():
     L.tka_session_inject_sort_fault.argtypes = [C.c_void_p]
     L.tka_session_destroy.argtypes = [C.c_void_p]
     L.tka_session_destroy.restype = None
+    L.tka_set_logger.argtypes = [LOG_FN, C.c_void_p, C.c_uint32]
+    L.tka_set_logger.restype = None
+    L.tka_log.argtypes = [C.c_int32, C.c_char_p]
+    L.tka_log.restype = None
+    L.tka_neighbors_connected.argtypes = [_ip, C.c_int32, C.c_int32, _i32p]
     for name in SYMBOLS:
-        if name not in ("tka_default_options", "tka_last_error", "tka_version", "tka_session_destroy"):
+        if name not in ("tka_default_options", "tka_last_error", "tka_version", "tka_session_destroy",
+                        "tka_set_logger", "tka_log"):
             getattr(L, name).restype = st
     _lib = L
     return L

@@ -44,6 +44,16 @@ inline void require(bool ok, tka_status s, const std::string& m)
     if (!ok) fail(s, m);
 }
 
+// The engine's messages (the reference's Logging singleton,
+// include/tapkee/utils/logging.hpp:89-144): delivered to the installed
+// tka_log_fn, or written to stderr as "[level] message", when the level is
+// enabled or `force` is set (a call's verbose option).
+void log_message(tka_log_level level, const std::string& msg, bool force = false);
+
+// "<operation> took <seconds> seconds." at the benchmark level, as the
+// reference's timed_context (include/tapkee/utils/time.hpp:29-42).
+void log_took(const char* operation, double seconds, bool force = false);
+
 // Message of the last failed entry point on this thread (tka_last_error).
 std::string& last_error();
 

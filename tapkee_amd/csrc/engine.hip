@@ -74,6 +74,31 @@ std::string& last_error()
     return msg;
 }
 
+namespace {
+std::mutex g_log_mu;
+tka_log_fn g_log_fn = nullptr;
+void* g_log_user = nullptr;
+uint32_t g_log_levels = TKA_LOG_DEFAULT_LEVELS;
+const char* const kLevelNames[] = {"info", "warning", "debug", "error", "benchmark"};
+}  // namespace
+
+void log_message(tka_log_level level, const std::string& msg, bool force)
+{
+    std::lock_guard<std::mutex> lk(g_log_mu);
+    if (!force && !(g_log_levels & (1u << level))) return;
+    if (g_log_fn)
+        g_log_fn((int32_t)level, msg.c_str(), g_log_user);
+    else
+        fprintf(stderr, "[%s] %s\n", kLevelNames[level], msg.c_str());
+}
+
+void log_took(const char* operation, double seconds, bool force)
+{
+    char buf[64];
+    snprintf(buf, sizeof buf, "%.9g", seconds);
+    log_message(TKA_LOG_BENCHMARK, std::string(operation) + " took " + buf + " seconds.", force);
+}
+
 int device_cus()
 {
     static int cus[64] = {};  // per device, filled on first use
@@ -376,7 +401,9 @@ void log_kl(tka_session& c, int iter, double C)
         st.kl_value[st.kl_count] = C;
         st.kl_count++;
     }
-    if (c.o.verbose) fprintf(stderr, "Iteration %d: error is %.17g\n", iter, C);
+    char buf[96];
+    snprintf(buf, sizeof buf, "Iteration %d: error is %.17g", iter, C);  // tsne.hpp:191
+    log_message(TKA_LOG_INFO, buf, c.o.verbose != 0);
 }
 
 template <class T>
@@ -656,10 +683,11 @@ tka_session* create_session(const double* Xh, int N, int D, int no_dims, double 
     if (sparse) {
         require(K >= 1, TKA_E_WRONG_PARAMETER, "perplexity must be at least 1/3 (K = 3*perplexity neighbours)");
         require(K <= N - 1, TKA_E_WRONG_PARAMETER, "K must be below the number of points");
-        if (o.verbose && perplexity > K) fprintf(stderr, "Perplexity should be lower than K!\n");  // tsne.hpp:604-605
+        if (perplexity > K) log_message(TKA_LOG_WARNING, "Perplexity should be lower than K!");  // tsne.hpp:604-605
     }
     if (o.y0_mode == TKA_Y0_EXPLICIT) require(o.y0 != nullptr, TKA_E_WRONG_PARAMETER, "TKA_Y0_EXPLICIT needs y0");
-    if (o.verbose) fprintf(stderr, exact ? "Using exact t-SNE algorithm\n" : "Using Barnes-Hut-SNE algorithm\n");
+    log_message(TKA_LOG_INFO, exact ? "Using exact t-SNE algorithm" : "Using Barnes-Hut-SNE algorithm",  // tsne.hpp:66,68
+                o.verbose != 0);
     TKA_HIP(hipSetDevice(o.device));
     std::unique_ptr<tka_session> sess;
     if (o.loop_precision == TKA_FP64)
@@ -756,6 +784,7 @@ tka_session* create_session(const double* Xh, int N, int D, int no_dims, double 
     else
         static_cast<Session<float>*>(sess.get())->init(Y0);
     c.stream.sync();
+    log_took("Input similarities computation", (now_ms() - t0) * 1e-3, o.verbose != 0);  // tsne.hpp:87
     return sess.release();
 }
 
@@ -769,6 +798,7 @@ void tsne_run_impl(const double* Xh, int N, int D, double* Yh, int no_dims, doub
     sess->step(sess->o.max_iter);                       // tsne.hpp:143-193
     sess->embedding(Yh);
     const double t6 = now_ms();
+    log_took("Main t-SNE loop", (t6 - t5) * 1e-3, sess->o.verbose != 0);  // tsne.hpp:142
     sess->st.kl_final_exact = sess->kl_exact();
     sess->st.ms_loop = t6 - t5;
     sess->st.ms_total = t6 - t0;
@@ -859,6 +889,14 @@ void gradient_dense_impl(const double* Ph, const double* Yh, int N, int dims, in
 using namespace tka;
 
 extern "C" {
+
+TKA_API void tka_set_logger(tka_log_fn fn, void* user, uint32_t enabled_levels)
+{
+    std::lock_guard<std::mutex> lk(g_log_mu);
+    g_log_fn = fn;
+    g_log_user = fn ? user : nullptr;
+    g_log_levels = enabled_levels;
+}
 
 TKA_API void tka_default_options(tka_tsne_options* o)
 {

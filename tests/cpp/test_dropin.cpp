@@ -9,6 +9,8 @@
 #include <cstdio>
 #include <cstring>
 #include <random>
+#include <string>
+#include <vector>
 
 using namespace tapkee;
 
@@ -46,6 +48,22 @@ static DenseMatrix swissroll(int N)
 }
 
 static bool cancel_now() { return true; }
+
+// Collects the engine's messages (utils/logging.hpp:45-59 interface).
+struct CollectingLogger : LoggerImplementation {
+    std::vector<std::string> lines;
+    void message_info(const std::string& m) override { lines.push_back("info:" + m); }
+    void message_warning(const std::string& m) override { lines.push_back("warning:" + m); }
+    void message_debug(const std::string& m) override { lines.push_back("debug:" + m); }
+    void message_error(const std::string& m) override { lines.push_back("error:" + m); }
+    void message_benchmark(const std::string& m) override { lines.push_back("benchmark:" + m); }
+    bool has(const std::string& prefix) const
+    {
+        for (const auto& l : lines)
+            if (l.compare(0, prefix.size(), prefix) == 0) return true;
+        return false;
+    }
+};
 
 int main(int argc, char** argv)
 {
@@ -93,7 +111,17 @@ int main(int argc, char** argv)
                        wrong_parameter_error);
     }
 
+    // the reference's default levels (logging.hpp:96-98): warning and error on
+    if (!Logging::instance().is_warning_enabled() || !Logging::instance().is_error_enabled() ||
+        Logging::instance().is_info_enabled() || Logging::instance().is_benchmark_enabled()) {
+        fprintf(stderr, "FAIL: default logging levels\n");
+        failures++;
+    }
+    CollectingLogger* logger = new CollectingLogger;
+    Logging::instance().set_logger_impl(logger);  // owned by Logging from here
     if (gpu) {
+        Logging::instance().enable_info();
+        Logging::instance().enable_benchmark();
         // methods.cpp:16-32 smoke test: shape N x 2, here also finite and centred
         TapkeeOutput out = with((method = tDistributedStochasticNeighborEmbedding, target_dimension = 2,
                                  num_neighbors = 20, sne_perplexity = 10.0))
@@ -114,6 +142,22 @@ int main(int argc, char** argv)
         }
         printf("kl trace %d entries, last %.6f, final exact %.6f\n", out.stats.kl_count,
                out.stats.kl_value[out.stats.kl_count - 1], out.stats.kl_final_exact);
+        // tsne.hpp:68, :87, :142, :191 through the installed logger
+        for (const char* want : {"info:Using Barnes-Hut-SNE algorithm", "benchmark:Input similarities computation took ",
+                                 "benchmark:Main t-SNE loop took ", "info:Iteration 50: error is "}) {
+            if (!logger->has(want)) {
+                fprintf(stderr, "FAIL: no log line '%s'\n", want);
+                failures++;
+            }
+        }
+        Logging::instance().disable_info();
+        Logging::instance().disable_benchmark();
+        logger->lines.clear();
+        with((method = tDistributedStochasticNeighborEmbedding, sne_perplexity = 10.0)).embedUsing(X);
+        if (logger->has("info:") || logger->has("benchmark:")) {
+            fprintf(stderr, "FAIL: disabled levels still delivered\n");
+            failures++;
+        }
         // interface.cpp:245-261 not_enough_memory_error (forced by a cap)
         EngineOptions eo;
         eo.opt.device_mem_cap = 1024;

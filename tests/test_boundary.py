@@ -117,3 +117,48 @@ def test_noalloc_limit_matches_header():
     text = open(os.path.join(ROOT, "include", "tapkee_amd.h")).read()
     m = re.search(r"#define TKA_NOALLOC_MAX_POINTS (\d+)", text)
     assert m and int(m.group(1)) == _lib.NOALLOC_MAX_POINTS
+
+
+def test_logger_receives_the_reference_warning():
+    # tsne.hpp:604-605 warns when perplexity > K; the message is emitted in
+    # validation, before any device work (on a machine without a GPU the
+    # call then fails)
+    got = []
+    tapkee_amd.set_logger(lambda lvl, msg: got.append((lvl, msg)), levels=("warning", "error"))
+    try:
+        with pytest.raises(_lib.EngineError):
+            tapkee_amd.tsne(np.random.default_rng(0).normal(size=(200, 4)), perplexity=30.0, knn_k=20,
+                            device=1 << 20)
+    finally:
+        tapkee_amd.set_logger(None)
+    assert ("warning", "Perplexity should be lower than K!") in got
+    # a disabled level is not delivered
+    got.clear()
+    tapkee_amd.set_logger(lambda lvl, msg: got.append((lvl, msg)), levels=("error",))
+    try:
+        with pytest.raises(_lib.EngineError):
+            tapkee_amd.tsne(np.random.default_rng(0).normal(size=(200, 4)), perplexity=30.0, knn_k=20,
+                            device=1 << 20)
+    finally:
+        tapkee_amd.set_logger(None)
+    assert got == []
+
+
+def test_cpp_neighbors_header_cpu_cases(tmp_path):
+    exe = tmp_path / "test_neighbors"
+    src = os.path.join(ROOT, "tests", "cpp", "test_neighbors.cpp")
+    libdir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), src,
+                    "-L", libdir, "-ltapkee_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all neighbour checks passed" in r.stdout
+
+
+def test_neighbors_connected_fixed_graphs():
+    # connected.hpp:19-61: DFS from object 0 along the directed lists
+    assert tapkee_amd.neighbors_connected(np.array([[1], [2], [0]]))
+    assert not tapkee_amd.neighbors_connected(np.array([[1], [0], [0]]))
+    assert not tapkee_amd.neighbors_connected(np.array([[1, 1], [0, 0], [3, 3], [2, 2]]))
+    with pytest.raises(tapkee_amd.WrongParameterError):
+        tapkee_amd.neighbors_connected(np.array([[1], [5], [0]]))
