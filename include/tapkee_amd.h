@@ -189,6 +189,25 @@ typedef struct tka_kernel_timing {
 
 TKA_API tka_status tka_session_create(const double* X, int32_t N, int32_t D, int32_t no_dims, double perplexity,
                                       double theta, const tka_tsne_options* opt, tka_session** out);
+/* Sessions from a precomputed similarity structure instead of features
+ * (the counterpart of tapkee's precomputed callbacks,
+ * include/tapkee/callbacks/precomputed_callbacks.hpp:20-42, for t-SNE):
+ *  - tka_session_create_knn: a neighbour graph, idx (N x K int32, the
+ *    object itself excluded) and dist (N x K squared distances); the
+ *    perplexity calibration, symmetrisation and division by the sum follow
+ *    (tsne.hpp:601-718, :207-322, :119-124);
+ *  - tka_session_create_p: a joint P in CSR (row_ptr N+1, col and val
+ *    row_ptr[N] entries), used after division by its sum (tsne.hpp:119-124);
+ *    it should be symmetric, as the reference's is.
+ * Both give a sparse P also at theta == 0 (exact repulsion, CSR attraction).
+ * Indices, distances and values are checked on the host (out of range,
+ * negative or non-finite -> TKA_E_WRONG_PARAMETER).  Y0 as in the options;
+ * TKA_Y0_RAND_COMPAT draws from std::rand() with no VP-tree draws before. */
+TKA_API tka_status tka_session_create_knn(const int32_t* idx, const double* dist, int32_t N, int32_t K, int32_t no_dims,
+                                          double perplexity, double theta, const tka_tsne_options* opt,
+                                          tka_session** out);
+TKA_API tka_status tka_session_create_p(const int32_t* row_ptr, const int32_t* col, const double* val, int32_t N,
+                                        int32_t no_dims, double theta, const tka_tsne_options* opt, tka_session** out);
 TKA_API tka_status tka_session_step(tka_session* s, int32_t iterations);
 TKA_API tka_status tka_session_synchronize(tka_session* s);
 /* Current embedding, zero-mean, N x no_dims FP64. */

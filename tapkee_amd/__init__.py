@@ -292,6 +292,41 @@ class Session:
         _lib.check(_lib.lib().tka_session_create(X, N, D, no_dims, perplexity, theta, C.byref(o), C.byref(h)))
         self._h = h
 
+    @classmethod
+    def from_knn(cls, idx, dist, no_dims=2, perplexity=30.0, theta=0.5, **options):
+        """A session on a precomputed neighbour graph (tka_session_create_knn):
+        idx (N x K, the object itself excluded) and dist (N x K squared
+        distances) replace the kNN search."""
+        idx = _i32(idx)
+        dist = _f64(dist)
+        if idx.ndim != 2 or idx.shape != dist.shape:
+            raise WrongParameterError("idx and dist must both be N x K")
+        N, K = idx.shape
+        self = cls.__new__(cls)
+        self.N, self.dims = N, no_dims
+        o, self._keep = _options(N, no_dims, **options)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().tka_session_create_knn(idx, dist, N, K, no_dims, perplexity, theta, C.byref(o),
+                                                     C.byref(h)))
+        self._h = h
+        return self
+
+    @classmethod
+    def from_p(cls, row, col, val, no_dims=2, theta=0.5, **options):
+        """A session on a precomputed joint P in CSR (tka_session_create_p),
+        used after division by its sum."""
+        row, col, val = _i32(row), _i32(col), _f64(val)
+        N = len(row) - 1
+        if len(col) != len(val) or len(col) < row[-1]:
+            raise WrongParameterError("col and val must hold row_ptr[N] entries")
+        self = cls.__new__(cls)
+        self.N, self.dims = N, no_dims
+        o, self._keep = _options(N, no_dims, **options)
+        h = C.c_void_p()
+        _lib.check(_lib.lib().tka_session_create_p(row, col, val, N, no_dims, theta, C.byref(o), C.byref(h)))
+        self._h = h
+        return self
+
     def step(self, iterations=1):
         _lib.check(_lib.lib().tka_session_step(self._h, int(iterations)))
 

@@ -132,6 +132,7 @@ SYMBOLS = (
     "tka_session_step", "tka_session_synchronize", "tka_session_embedding", "tka_session_kl", "tka_session_stats",
     "tka_session_profile", "tka_session_kernel_timing", "tka_session_destroy", "tka_session_inject_sort_fault",
     "tka_gradient_dev_noalloc", "tka_kl_grad_dev", "tka_set_logger", "tka_log", "tka_neighbors_connected",
+    "tka_session_create_knn", "tka_session_create_p",
 )
 
 
@@ -174,6 +175,10 @@ def lib():
                              C.c_double, _f64p, _f64p, C.c_void_p]
     L.tka_session_create.argtypes = [_dp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                      C.POINTER(Options), C.POINTER(C.c_void_p)]
+    L.tka_session_create_knn.argtypes = [_ip, _dp, C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                         C.POINTER(Options), C.POINTER(C.c_void_p)]
+    L.tka_session_create_p.argtypes = [_ip, _ip, _dp, C.c_int32, C.c_int32, C.c_double, C.POINTER(Options),
+                                       C.POINTER(C.c_void_p)]
     L.tka_session_step.argtypes = [C.c_void_p, C.c_int32]
     L.tka_session_synchronize.argtypes = [C.c_void_p]
     L.tka_session_embedding.argtypes = [C.c_void_p, _dp]

@@ -661,9 +661,57 @@ struct Session : tka_session {
     }
 };
 
+// What a session's P is computed from: feature vectors (the reference's
+// path), a precomputed neighbour graph (indices and squared distances), or
+// a precomputed joint P.
+struct SimilarityInput {
+    enum Kind { kFeatures, kKnn, kJointP } kind = kFeatures;
+    const double* X = nullptr;  // kFeatures: N x D
+    int D = 0;
+    const int* idx = nullptr;   // kKnn: N x K neighbours (no self), K
+    const double* dist = nullptr;
+    int K = 0;
+    const int* row = nullptr;   // kJointP: CSR
+    const int* col = nullptr;
+    const double* val = nullptr;
+};
+
+// Host checks of a precomputed graph before any device work.
+void validate_graph(const SimilarityInput& in, int N)
+{
+    if (in.kind == SimilarityInput::kKnn) {
+        require(in.idx && in.dist, TKA_E_WRONG_PARAMETER, "null neighbour arrays");
+        require(in.K >= 1 && in.K <= N - 1, TKA_E_WRONG_PARAMETER, "K must be in [1, N-1]");
+        for (int i = 0; i < N; i++)
+            for (int k = 0; k < in.K; k++) {
+                const size_t e = (size_t)i * in.K + k;
+                const int j = in.idx[e];
+                require(j >= 0 && j < N && j != i, TKA_E_WRONG_PARAMETER,
+                        "neighbour " + std::to_string(k) + " of object " + std::to_string(i) +
+                            " is out of range or the object itself");
+                require(std::isfinite(in.dist[e]) && in.dist[e] >= 0.0, TKA_E_WRONG_PARAMETER,
+                        "neighbour distances must be finite and non-negative");
+            }
+    } else if (in.kind == SimilarityInput::kJointP) {
+        require(in.row && in.col && in.val, TKA_E_WRONG_PARAMETER, "null CSR arrays");
+        require(in.row[0] == 0, TKA_E_WRONG_PARAMETER, "row_ptr[0] must be 0");
+        double sum = 0.0;
+        for (int i = 0; i < N; i++) {
+            require(in.row[i + 1] >= in.row[i], TKA_E_WRONG_PARAMETER, "row_ptr must be non-decreasing");
+            for (int e = in.row[i]; e < in.row[i + 1]; e++) {
+                require(in.col[e] >= 0 && in.col[e] < N, TKA_E_WRONG_PARAMETER, "P column index out of range");
+                require(std::isfinite(in.val[e]) && in.val[e] >= 0.0, TKA_E_WRONG_PARAMETER,
+                        "P values must be finite and non-negative");
+                sum += in.val[e];
+            }
+        }
+        require(sum > 0.0, TKA_E_WRONG_PARAMETER, "P sums to zero");
+    }
+}
+
 // Validation, normalisation, P and Y0: everything tsne.hpp:61-139 does
 // before the main loop.  The session's stream is created on o.device.
-tka_session* create_session(const double* Xh, int N, int D, int no_dims, double perplexity, double theta,
+tka_session* create_session(const SimilarityInput& in, int N, int no_dims, double perplexity, double theta,
                             const tka_tsne_options* opt_in)
 {
     tka_tsne_options o;
@@ -673,14 +721,20 @@ tka_session* create_session(const double* Xh, int N, int D, int no_dims, double 
                 "tka_tsne_options.struct_size mismatch");
         o = *opt_in;
     }
-    validate(N, D, no_dims, perplexity, theta);
-    require(Xh != nullptr, TKA_E_WRONG_PARAMETER, "null data pointer");
-    check_finite(Xh, (size_t)N * D, "input data");
+    const bool features = in.kind == SimilarityInput::kFeatures;
+    const int D = features ? in.D : 1;
+    validate(N, D, no_dims, features || in.kind == SimilarityInput::kKnn ? perplexity : 0.0, theta);
+    if (features) {
+        require(in.X != nullptr, TKA_E_WRONG_PARAMETER, "null data pointer");
+        check_finite(in.X, (size_t)N * D, "input data");
+    }
+    validate_graph(in, N);
     if (o.cancel && o.cancel()) fail(TKA_E_CANCELLED, "cancelled");  // methods.hpp:30-31
     const bool exact = (theta == 0.0);
-    const bool sparse = !exact || o.exact_p_mode == TKA_EXACT_P_KNN;
-    const int K = o.knn_k > 0 ? o.knn_k : (int)(3 * perplexity);
-    if (sparse) {
+    // a precomputed graph gives a sparse P also at theta = 0
+    const bool sparse = !exact || o.exact_p_mode == TKA_EXACT_P_KNN || !features;
+    const int K = in.kind == SimilarityInput::kKnn ? in.K : (o.knn_k > 0 ? o.knn_k : (int)(3 * perplexity));
+    if (sparse && in.kind != SimilarityInput::kJointP) {
         require(K >= 1, TKA_E_WRONG_PARAMETER, "perplexity must be at least 1/3 (K = 3*perplexity neighbours)");
         require(K <= N - 1, TKA_E_WRONG_PARAMETER, "K must be below the number of points");
         if (perplexity > K) log_message(TKA_LOG_WARNING, "Perplexity should be lower than K!");  // tsne.hpp:604-605
@@ -708,21 +762,44 @@ tka_session* create_session(const double* Xh, int N, int D, int no_dims, double 
     const double t0 = now_ms();
 
     DevBuf<double> X;  // normalise (tsne.hpp:88-89)
-    upload(X, Xh, (size_t)N * D, s, mb);
-    c.stream.sync();
-    st.ms_upload = now_ms() - t0;
-    normalize_device(X.get(), N, D, mb, s);
-    c.stream.sync();
+    if (features) {
+        upload(X, in.X, (size_t)N * D, s, mb);
+        c.stream.sync();
+        st.ms_upload = now_ms() - t0;
+        normalize_device(X.get(), N, D, mb, s);
+        c.stream.sync();
+    }
     const double t1 = now_ms();
     st.ms_normalize = t1 - t0;
 
     auto fill = [&](auto* S) {
         using TT = typename std::remove_reference<decltype(*S->L.val.get())>::type;
-        if (sparse) {
+        if (in.kind == SimilarityInput::kJointP) {
+            // a precomputed joint P, divided by its sum (tsne.hpp:120-124)
+            const int64_t nnz = in.row[N];
+            Csr P;
+            upload(P.row, in.row, (size_t)N + 1, s, mb);
+            upload(P.col, in.col, (size_t)std::max<int64_t>(nnz, 1), s, mb);
+            upload(P.val, in.val, (size_t)std::max<int64_t>(nnz, 1), s, mb);
+            P.nnz = nnz;
+            normalize_sum_device(P.val.get(), P.nnz, mb, s);
+            c.stream.sync();
+            st.ms_symmetrize = now_ms() - t1;
+            st.nnz = P.nnz;
+            S->L.row = std::move(P.row);
+            S->L.col = std::move(P.col);
+            S->L.val.alloc((size_t)std::max<int64_t>(P.nnz, 1), mb);
+            cast_device<double, TT>(P.val.get(), S->L.val.get(), P.nnz, s);
+        } else if (sparse) {
             DevBuf<int> idx((size_t)N * K, mb);
             DevBuf<double> dist((size_t)N * K, mb), val((size_t)N * K, mb);
             KnnStats ks;
-            knn_device(X.get(), N, D, K, o.dist_dtype, o.candidates, idx.get(), dist.get(), &ks, mb, s);
+            if (features) {
+                knn_device(X.get(), N, D, K, o.dist_dtype, o.candidates, idx.get(), dist.get(), &ks, mb, s);
+            } else {  // a precomputed neighbour graph
+                upload(idx, in.idx, (size_t)N * K, s, mb);
+                upload(dist, in.dist, (size_t)N * K, s, mb);
+            }
             c.stream.sync();
             const double t2 = now_ms();
             X.release();
@@ -773,7 +850,7 @@ tka_session* create_session(const double* Xh, int N, int D, int no_dims, double 
         check_finite(Y0.data(), Y0.size(), "y0");
     } else {
         if (o.y0_mode == TKA_Y0_SEEDED) std::srand(o.seed);
-        if (!exact) {
+        if (!exact && features) {  // the reference's VP-tree consumed these draws
             const long long burn = vptree_draws(N);
             for (long long i = 0; i < burn; i++) (void)std::rand();
         }
@@ -793,7 +870,10 @@ void tsne_run_impl(const double* Xh, int N, int D, double* Yh, int no_dims, doub
 {
     require(Yh != nullptr, TKA_E_WRONG_PARAMETER, "null output pointer");
     const double t0 = now_ms();
-    std::unique_ptr<tka_session> sess(create_session(Xh, N, D, no_dims, perplexity, theta, opt_in));
+    SimilarityInput in;
+    in.X = Xh;
+    in.D = D;
+    std::unique_ptr<tka_session> sess(create_session(in, N, no_dims, perplexity, theta, opt_in));
     const double t5 = now_ms();
     sess->step(sess->o.max_iter);                       // tsne.hpp:143-193
     sess->embedding(Yh);
@@ -943,7 +1023,41 @@ TKA_API tka_status tka_session_create(const double* X, int32_t N, int32_t D, int
     return guard([&] {
         require(out != nullptr, TKA_E_WRONG_PARAMETER, "null session pointer");
         *out = nullptr;
-        *out = create_session(X, N, D, no_dims, perplexity, theta, opt);
+        SimilarityInput in;
+        in.X = X;
+        in.D = D;
+        *out = create_session(in, N, no_dims, perplexity, theta, opt);
+    });
+}
+
+TKA_API tka_status tka_session_create_knn(const int32_t* idx, const double* dist, int32_t N, int32_t K, int32_t no_dims,
+                                          double perplexity, double theta, const tka_tsne_options* opt,
+                                          tka_session** out)
+{
+    return guard([&] {
+        require(out != nullptr, TKA_E_WRONG_PARAMETER, "null session pointer");
+        *out = nullptr;
+        SimilarityInput in;
+        in.kind = SimilarityInput::kKnn;
+        in.idx = idx;
+        in.dist = dist;
+        in.K = K;
+        *out = create_session(in, N, no_dims, perplexity, theta, opt);
+    });
+}
+
+TKA_API tka_status tka_session_create_p(const int32_t* row_ptr, const int32_t* col, const double* val, int32_t N,
+                                        int32_t no_dims, double theta, const tka_tsne_options* opt, tka_session** out)
+{
+    return guard([&] {
+        require(out != nullptr, TKA_E_WRONG_PARAMETER, "null session pointer");
+        *out = nullptr;
+        SimilarityInput in;
+        in.kind = SimilarityInput::kJointP;
+        in.row = row_ptr;
+        in.col = col;
+        in.val = val;
+        *out = create_session(in, N, no_dims, 0.0, theta, opt);
     });
 }
 

@@ -162,3 +162,17 @@ def test_neighbors_connected_fixed_graphs():
     assert not tapkee_amd.neighbors_connected(np.array([[1, 1], [0, 0], [3, 3], [2, 2]]))
     with pytest.raises(tapkee_amd.WrongParameterError):
         tapkee_amd.neighbors_connected(np.array([[1], [5], [0]]))
+
+
+def test_precomputed_inputs_are_checked_before_device_work():
+    row = np.array([0, 1, 2, 3], np.int32)
+    with pytest.raises(tapkee_amd.WrongParameterError, match="column index"):
+        tapkee_amd.Session.from_p(row, np.array([1, 2, 7]), np.array([0.2, 0.3, 0.5]))
+    with pytest.raises(tapkee_amd.WrongParameterError, match="non-negative"):
+        tapkee_amd.Session.from_p(row, np.array([1, 2, 0]), np.array([0.2, -0.3, 0.5]))
+    idx = np.array([[1], [2], [2], [0]])
+    with pytest.raises(tapkee_amd.WrongParameterError, match="object itself"):
+        tapkee_amd.Session.from_knn(idx, np.ones((4, 1)), perplexity=0.3, theta=0.0)
+    with pytest.raises(tapkee_amd.WrongParameterError, match="finite"):
+        tapkee_amd.Session.from_knn(np.array([[1], [2], [3], [0]]), np.array([[1.0], [np.nan], [1.0], [1.0]]),
+                                    perplexity=0.3, theta=0.0)

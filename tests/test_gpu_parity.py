@@ -400,3 +400,45 @@ def test_full_run_final_kl_within_oracle_spread(engine, oracle, theta):
         kl_o = _exact_kl_oracle(oracle, X, Y, 15.0, theta)
         assert abs(kl - kl_o) < tol * kl, f"{prec}: engine {kl!r} oracle {kl_o!r}"
         assert lo <= kl <= hi, (prec, kl, ref)
+
+
+def test_session_from_precomputed_knn_equals_features_path(engine):
+    # the neighbour graph the features path computes, handed over: the same
+    # P and the same Y0 give the same embedding bit for bit
+    X = mixture(3000, 16, k=5)
+    y0 = rng(9).normal(size=(3000, 2)) * 1e-4
+    idx, dist, _ = engine.knn(engine.normalize(X), 45, dist_dtype="fp32")
+    ref = engine.Session(X, perplexity=15.0, theta=0.5, y0=y0)
+    ref.step(300)
+    a = engine.Session.from_knn(idx, dist, perplexity=15.0, theta=0.5, y0=y0)
+    a.step(300)
+    np.testing.assert_array_equal(a.embedding(), ref.embedding())
+    assert a.stats()["kl"] == ref.stats()["kl"]
+    a.close()
+    ref.close()
+
+
+@pytest.mark.parametrize("theta", [0.5, 0.0])
+def test_session_from_precomputed_p(engine, oracle, theta):
+    # a joint P (here the engine's own, scaled by 3: the session divides by
+    # the sum) gives the features path's trajectory up to the rounding of
+    # that division; at theta=0 the sparse P runs with the exact repulsion
+    X = mixture(2000, 12, k=4)
+    y0 = rng(4).normal(size=(2000, 2)) * 1e-4
+    row, col, val = engine.build_p(X, 12.0, dist_dtype="fp32")
+    a = engine.Session.from_p(row, col, 3.0 * val, theta=theta, y0=y0, max_iter=200)
+    a.step(200)
+    Ya = a.embedding()
+    kla = a.stats()["kl"]
+    a.close()
+    kw = dict(exact_p="knn") if theta == 0.0 else {}
+    b = engine.Session(X, perplexity=12.0, theta=theta, y0=y0, max_iter=200, **kw)
+    b.step(200)
+    klb = b.stats()["kl"]
+    b.close()
+    np.testing.assert_allclose(kla, klb, rtol=1e-4)
+    # the gradient at the result against the oracle's with the same P
+    g, _ = engine.gradient(Ya, row, col, val, theta, precision="fp64")
+    r, _ = (oracle.gradient_bh(row, col, val, Ya, theta) if theta > 0
+            else oracle.gradient_exact_sparse(row, col, val, Ya))
+    assert rel_l2(g, r) < 1e-9
