@@ -1,4 +1,5 @@
-"""Compile the HIP engine into tapkee_amd/libtapkee_amd.so (gfx950).
+"""Compile the HIP engine into tapkee_amd/libtapkee_amd.so (gfx950) and the
+command-line front end into tapkee_amd/bin/tapkee.
 
 Usage: python -m tapkee_amd.build [--jobs N]
 
@@ -56,6 +57,24 @@ def _compile(src: str) -> str:
     return obj
 
 
+CLI_SRC = os.path.join(HERE, "cli", "tapkee_cli.cpp")
+CLI = os.path.join(HERE, "bin", "tapkee")
+
+
+def build_cli() -> str:
+    """The command-line front end (cli/tapkee_cli.cpp) against the library,
+    found next to it at run time (rpath $ORIGIN/..)."""
+    os.makedirs(os.path.dirname(CLI), exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"), CLI_SRC,
+           "-L", HERE, "-ltapkee_amd", "-Wl,-rpath,$ORIGIN/..", "-o", CLI + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"CLI build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(CLI + ".tmp", CLI)
+    return CLI
+
+
 def build(jobs: int | None = None) -> str:
     os.makedirs(BUILD, exist_ok=True)
     srcs = sources()
@@ -68,6 +87,7 @@ def build(jobs: int | None = None) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_cli()
     return LIB
 
 
