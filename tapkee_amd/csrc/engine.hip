@@ -772,6 +772,8 @@ tka_session* create_session(const SimilarityInput& in, int N, int no_dims, doubl
     const double t1 = now_ms();
     st.ms_normalize = t1 - t0;
 
+    DevBuf<uint32_t> order;  // the kNN's locality order of the points, if it made one
+    bool ordered = false;
     auto fill = [&](auto* S) {
         using TT = typename std::remove_reference<decltype(*S->L.val.get())>::type;
         if (in.kind == SimilarityInput::kJointP) {
@@ -795,7 +797,10 @@ tka_session* create_session(const SimilarityInput& in, int N, int no_dims, doubl
             DevBuf<double> dist((size_t)N * K, mb), val((size_t)N * K, mb);
             KnnStats ks;
             if (features) {
-                knn_device(X.get(), N, D, K, o.dist_dtype, o.candidates, idx.get(), dist.get(), &ks, mb, s);
+                if (!exact) order.alloc(N, mb);
+                knn_device(X.get(), N, D, K, o.dist_dtype, o.candidates, idx.get(), dist.get(), &ks, mb, s,
+                           exact ? nullptr : order.get());
+                ordered = ks.order_written;
             } else {  // a precomputed neighbour graph
                 upload(idx, in.idx, (size_t)N * K, s, mb);
                 upload(dist, in.dist, (size_t)N * K, s, mb);
@@ -856,10 +861,17 @@ tka_session* create_session(const SimilarityInput& in, int N, int no_dims, doubl
         }
         for (auto& v : Y0) v = gaussian_random() * .0001;
     }
-    if (o.loop_precision == TKA_FP64)
+    // Barnes-Hut loop: the points start in the kNN's locality order, in which
+    // each point's P neighbours have nearby numbers (the attraction's
+    // gathers stay local from the first iteration; the tree's Morton order
+    // takes over at the first periodic renumbering)
+    if (o.loop_precision == TKA_FP64) {
         static_cast<Session<double>*>(sess.get())->init(Y0);
-    else
+        if (ordered) static_cast<Session<double>*>(sess.get())->renumber(order.get());
+    } else {
         static_cast<Session<float>*>(sess.get())->init(Y0);
+        if (ordered) static_cast<Session<float>*>(sess.get())->renumber(order.get());
+    }
     c.stream.sync();
     log_took("Input similarities computation", (now_ms() - t0) * 1e-3, o.verbose != 0);  // tsne.hpp:87
     return sess.release();

@@ -1451,7 +1451,7 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, Tile
 }  // namespace
 
 void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, int* idx, double* dist, KnnStats* st,
-                MemBudget* mb, hipStream_t s)
+                MemBudget* mb, hipStream_t s, uint32_t* order_out)
 {
     require(N >= 2 && K >= 1 && K <= N - 1, TKA_E_WRONG_PARAMETER, "kNN needs 1 <= K <= N-1");
     require(K <= kMaxKprime, TKA_E_WRONG_PARAMETER,
@@ -1645,6 +1645,10 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         }
     }
     TKA_HIP(hipStreamSynchronize(s));  // rmax is on the host now
+    if (order_out && perm.get()) {
+        TKA_HIP(hipMemcpyAsync(order_out, perm.get(), (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+        if (st) st->order_written = true;
+    }
     RerankArgs ra{Xs, ns ? ns : n2.get(), N, D, K, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get(),
                   eps_a, eps_b, rmax, idx, dist, tr.fail.get(), perm.get()};
     hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(N, 4)), dim3(256), 0, s, ra);
