@@ -215,53 +215,6 @@ __global__ __launch_bounds__(256) void row_norms(const double* __restrict__ X, i
     }
 }
 
-__global__ __launch_bounds__(256) void prep_fp32(const double* __restrict__ X, size_t total,
-                                                 float* __restrict__ Xf)
-{
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < total; i += (size_t)gridDim.x * 256)
-        Xf[i] = (float)X[i];
-}
-
-// BF16 operand rows of width Kp (multiple of 16):
-//   candidates A = [c_hi | c_lo | c_hi | n_hi n_lo | 0...]
-//   queries    B = [-2 q_hi | -2 q_hi | -2 q_lo | 1 1 | 0...]
-// so that A.B = ||c||^2 - 2 q.c with the q_lo*c_lo term dropped (~2^-16
-// relative).  Scaling by -2 is exact in BF16.
-__global__ __launch_bounds__(256) void prep_bf16x3(const double* __restrict__ X, const double* __restrict__ n2,
-                                                   int N, int D, int Kp, __bf16* __restrict__ A,
-                                                   __bf16* __restrict__ B)
-{
-    const size_t total = (size_t)N * Kp;
-    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
-        const int n = (int)(e / Kp), k = (int)(e % Kp);
-        float a = 0.f, b = 0.f;
-        if (k < 3 * D) {
-            const int seg = k / D, d = k % D;
-            const double x = X[(size_t)n * D + d];
-            const __bf16 hi = (__bf16)x;
-            const __bf16 lo = (__bf16)(x - (double)(float)hi);
-            if (seg == 0) {
-                a = (float)hi;
-                b = -2.f * (float)hi;
-            } else if (seg == 1) {
-                a = (float)lo;
-                b = -2.f * (float)hi;
-            } else {
-                a = (float)hi;
-                b = -2.f * (float)lo;
-            }
-        } else if (k == 3 * D || k == 3 * D + 1) {
-            const double nn = n2[n];
-            const __bf16 hi = (__bf16)nn;
-            const __bf16 lo = (__bf16)(nn - (double)(float)hi);
-            a = (k == 3 * D) ? (float)hi : (float)lo;
-            b = 1.f;
-        }
-        A[e] = (__bf16)a;
-        B[e] = (__bf16)b;
-    }
-}
-
 // ---------------------------------------------------------------- tiles
 struct TileArgs {
     int N;
@@ -474,139 +427,6 @@ TKA_DEV void compact_if_needed(int w, const TileArgs& a, int slot0, int* s_cnt, 
     }
 }
 
-// FP32 VALU tiles.  Xf: N x D FP32, nrm: ||c||^2 in FP32.
-__global__ __launch_bounds__(256) void knn_tiles_fp32(const float* __restrict__ Xf, const float* __restrict__ nrm,
-                                                      int D, TileArgs a)
-{
-    constexpr int KC = 32;
-    __shared__ float sC[KC][kCB + 4];
-    __shared__ float sQ[KC][kQB + 4];
-    __shared__ float sN[kCB];
-    __shared__ int s_cnt[kQB];
-    __shared__ float s_tau[kQB];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
-    const int slot0 = blockIdx.x * kQB;
-    const int sl = 32 * w + (lane & 31);
-    const int q = query_row(a, slot0 + sl);
-    if (tid < kQB) {
-        s_cnt[tid] = 0;
-        s_tau[tid] = HUGE_VALF;
-    }
-    uint64_t* qbuf = a.buf + (size_t)(slot0 + sl) * kCap;
-    for (int cb = 0; cb < a.N; cb += kCB) {
-        float acc[4][16];
-#pragma unroll
-        for (int t = 0; t < 4; t++)
-#pragma unroll
-            for (int i = 0; i < 16; i++) acc[t][i] = 0.f;
-        for (int kc = 0; kc < D; kc += KC) {
-            const int kn = (D - kc) < KC ? (D - kc) : KC;
-            __syncthreads();
-            for (int e = tid; e < kCB * KC; e += 256) {
-                const int m = e / KC, k = e % KC;
-                const int row = cb + m;
-                sC[k][m] = (row < a.N && k < kn) ? Xf[(size_t)row * D + kc + k] : 0.f;
-                const int qs = m;  // also stage queries (kQB == kCB)
-                const int qr = query_row(a, slot0 + qs);
-                sQ[k][qs] = (qr >= 0 && k < kn) ? Xf[(size_t)qr * D + kc + k] : 0.f;
-            }
-            if (kc == 0 && tid < kCB) sN[tid] = (cb + tid < a.N) ? nrm[cb + tid] : 0.f;
-            __syncthreads();
-            for (int k = 0; k < kn; k++) {
-                const float qv = sQ[k][sl];
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-#pragma unroll
-                    for (int g = 0; g < 4; g++) {
-                        const float4 c4 = *reinterpret_cast<const float4*>(&sC[k][32 * t + 8 * g + 4 * half]);
-                        acc[t][4 * g + 0] = fmaf(qv, c4.x, acc[t][4 * g + 0]);
-                        acc[t][4 * g + 1] = fmaf(qv, c4.y, acc[t][4 * g + 1]);
-                        acc[t][4 * g + 2] = fmaf(qv, c4.z, acc[t][4 * g + 2]);
-                        acc[t][4 * g + 3] = fmaf(qv, c4.w, acc[t][4 * g + 3]);
-                    }
-                }
-            }
-        }
-        const float tau = s_tau[sl];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            float v[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const int m = 32 * t + 8 * (i >> 2) + 4 * half + (i & 3);
-                v[i] = fmaf(-2.f, acc[t][i], sN[m]);
-            }
-            filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
-        }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau);
-    }
-    if (lane < 32 && slot0 + sl < a.nq) {
-        a.cnt_out[slot0 + sl] = s_cnt[sl];
-        a.tau_out[slot0 + sl] = s_tau[sl];
-    }
-}
-
-// BF16 MFMA tiles.  A/B: N x Kp operand rows (see prep_bf16x3).
-__global__ __launch_bounds__(256) void knn_tiles_bf16(const __bf16* __restrict__ A, const __bf16* __restrict__ B,
-                                                      int Kp, TileArgs a)
-{
-    constexpr int KC = 160;                 // bf16 per LDS chunk (multiple of 16)
-    constexpr int KV = KC / 8;              // bf16x8 vectors per row in a chunk
-    __shared__ bf16x8 sA[kCB][KV + 1];      // +1 vector of padding per row
-    __shared__ int s_cnt[kQB];
-    __shared__ float s_tau[kQB];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
-    const int slot0 = blockIdx.x * kQB;
-    const int sl = 32 * w + (lane & 31);
-    const int q = query_row(a, slot0 + sl);
-    if (tid < kQB) {
-        s_cnt[tid] = 0;
-        s_tau[tid] = HUGE_VALF;
-    }
-    uint64_t* qbuf = a.buf + (size_t)(slot0 + sl) * kCap;
-    const bf16x8 zero8 = {};
-    for (int cb = 0; cb < a.N; cb += kCB) {
-        f32x16 acc[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) acc[t] = f32x16{};
-        for (int kc = 0; kc < Kp; kc += KC) {
-            const int kn = (Kp - kc) < KC ? (Kp - kc) : KC;  // multiple of 16
-            const int nv = kn / 8;
-            __syncthreads();
-            for (int e = tid; e < kCB * KV; e += 256) {
-                const int m = e / KV, v = e % KV;
-                const int row = cb + m;
-                sA[m][v] = (row < a.N && v < nv)
-                               ? *reinterpret_cast<const bf16x8*>(A + (size_t)row * Kp + kc + 8 * v)
-                               : zero8;
-            }
-            __syncthreads();
-            for (int ks = 0; ks < kn / 16; ks++) {
-                const bf16x8 b = (q >= 0) ? *reinterpret_cast<const bf16x8*>(B + (size_t)q * Kp + kc + 16 * ks + 8 * half)
-                                          : zero8;
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const bf16x8 av = sA[32 * t + (lane & 31)][2 * ks + half];
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b, acc[t], 0, 0, 0);
-                }
-            }
-        }
-        const float tau = s_tau[sl];
-#pragma unroll
-        for (int t = 0; t < 4; t++) {
-            float v[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) v[i] = acc[t][i];
-            filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
-        }
-        compact_if_needed(w, a, slot0, s_cnt, s_tau);
-    }
-    if (lane < 32 && slot0 + sl < a.nq) {
-        a.cnt_out[slot0 + sl] = s_cnt[sl];
-        a.tau_out[slot0 + sl] = s_tau[sl];
-    }
-}
-
 // FP8 (e4m3) MFMA tiles with per-row absmax scaling: A8 rows hold
 // fp8(x / s_x), s_x = absmax(x) / 224, so q.c ~= s_q s_c sum(q8 c8).
 // The squared norms come from FP32 copies of the FP64 norms.
@@ -770,6 +590,15 @@ struct ChunkOrder {
     }
 };
 
+// The first chunk of a block's stream: the chunk holding its first query row
+// (rows are in locality order, so the rows near a query come early).
+TKA_DEV int seed_chunk(const TileArgs& a, int slot0, int nch)
+{
+    if (!a.rotate) return -1;
+    const int r = query_row(a, slot0);
+    return (r >= 0 ? r / kCB : 0) % nch;
+}
+
 typedef __attribute__((address_space(3))) void lds_void;
 
 // Copies `bytes` (a multiple of 1024) from global src to LDS dst, 16 bytes
@@ -878,8 +707,7 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
     }
     const unsigned char* src = reinterpret_cast<const unsigned char*>(A);
     const int nch = (a.N + kCB - 1) / kCB;
-    // the chunk holding the block's first query rows (rows are in locality order)
-    const ChunkOrder order(a.rotate ? (int)(((size_t)blockIdx.x * QB / kCB) % nch) : -1, nch);
+    const ChunkOrder order(seed_chunk(a, slot0, nch), nch);
     if constexpr (kSplit)
         dma_stage_to_lds<V, NV, NV1, 0, WAVES>(A + (size_t)order.at(0) * kCB * RS, dyn_lds);
     else
@@ -958,9 +786,193 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
     }
 }
 
-// Padded half-major BF16 operands (hi/lo splits as prep_bf16x3): vector
-// (h, g) of a row holds k = 16 g + 8 h + 0..7; the pad vector and the rows
-// past N are zero.  A and B share the layout (B: the query-side factors).
+// ---------------------------------------------------------------- wide rows
+// Rows too wide to keep a query operand in registers (BF16 beyond 28 vectors
+// per half row, FP32 beyond 20: D above about 146 and 77): both operands are
+// streamed through LDS in k-stages of KS vectors per half row.  The operands
+// are stage-major (hm_to_stages): stage st of row r is the 2 KS vectors
+// (h, st KS + g), h = 0, 1, g < KS, at ((st Npad) + r) 2 KS, so a stage of a
+// chunk is one contiguous block.  A stage image holds those vectors for the
+// kCB candidate rows of a chunk and then for the block's kWideQB query rows,
+// with a pad slot per row (RSS = 2 KS + 1 vectors, an odd stride for the LDS
+// banks), copied by direct LDS loads.  Every wave issues the same number of
+// them per stage (the last one is repeated where the image does not divide
+// evenly).
+constexpr int kWideQB = 128;    // query rows per block: four 32-row tiles
+constexpr int kWideWaves = 8;   // two waves per query tile, two candidate tiles each
+
+template <int KS>
+struct WideStage {
+    static constexpr int RSS = 2 * KS + 1;
+    static constexpr int IMG = (kCB + kWideQB) * RSS;            // vectors per stage image
+    static constexpr int TL = IMG / 64;                           // direct loads per image
+    static constexpr int L = (TL + kWideWaves - 1) / kWideWaves;  // per wave
+    static_assert(IMG % 64 == 0 && kCB * RSS % 64 == 0, "whole direct loads, rows of one kind per load");
+};
+
+// Stage i of the stream (chunk order.at(i / nst), stage i % nst) into image
+// buffer i % 2: the lane's direct loads at their precomputed offsets.
+template <int KS>
+TKA_DEV void wide_copy(const uint4* __restrict__ A, const uint4* __restrict__ B, int Npad, int nst,
+                       const ChunkOrder& order, int i, const int* off, const bool* isq, unsigned char* lds, int w)
+{
+    typedef WideStage<KS> S;
+    const size_t so = (size_t)(i % nst) * Npad * (2 * KS);
+    const uint4* ca = A + so + (size_t)order.at(i / nst) * kCB * (2 * KS);
+    const uint4* qb = B + so;
+    unsigned char* dst = lds + (size_t)(i & 1) * S::IMG * 16;
+#pragma unroll
+    for (int j = 0; j < S::L; j++) {
+        const int e = min(w + j * kWideWaves, S::TL - 1);
+        __builtin_amdgcn_global_load_lds((isq[j] ? qb : ca) + off[j], (lds_void*)(dst + (size_t)e * 1024), 16, 0, 0);
+    }
+}
+
+// All rows against the block's query rows with both operands streamed
+// through two stage images (the next one is copied while this one is
+// multiplied).  Wave w multiplies query tile w % 4 by candidate tiles
+// 2 (w / 4) and 2 (w / 4) + 1 of the chunk, so two waves on each SIMD
+// overlap one's copies and sums with the other's MFMAs.  Each stage is
+// accumulated from zero and added to the chunk's running sums (no value
+// carries more than 16 KS (BF16) or 8 KS (FP32) MFMA roundings plus one
+// addition per stage).  After a chunk's last stage its values are filtered
+// exactly as in knn_tiles_reg (same seeds, order, appends -- by LDS atomics,
+// so the two waves of a query tile share its rows -- and compaction), so the
+// candidate buffers have the same meaning.
+template <class Op, int KS>
+__global__ __launch_bounds__(64 * kWideWaves) void knn_tiles_wide(const typename Op::V* __restrict__ A,
+                                                                  const typename Op::V* __restrict__ B, int NV,
+                                                                  int Npad, TileArgs a)
+{
+    typedef typename Op::V V;
+    typedef WideStage<KS> S;
+    static_assert(sizeof(V) == 16, "one vector per lane per direct load");
+    constexpr int RSS = S::RSS;
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    const V* lds = reinterpret_cast<const V*>(dyn_lds);  // two stage images
+    __shared__ int s_cnt[kWideQB];
+    __shared__ float s_tau[kWideQB];
+    __shared__ int s_qrow[kWideQB];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
+    const int qt = w & 3, ch = w >> 2;  // query tile, candidate half
+    const int slot0 = blockIdx.x * kWideQB;
+    for (int i = tid; i < kWideQB; i += 64 * kWideWaves) {
+        s_cnt[i] = 0;
+        s_tau[i] = HUGE_VALF;
+        s_qrow[i] = max(query_row(a, slot0 + i), 0);  // slots past the queries copy row 0
+    }
+    const int sl = 32 * qt + (lane & 31);
+    const int q = query_row(a, slot0 + sl);
+    uint64_t* qbuf = a.buf + (size_t)(slot0 + sl) * kCap;
+    const int nch = (a.N + kCB - 1) / kCB;
+    const int nst = NV / KS;
+    const int T = nch * nst;  // stages of the whole stream
+    const ChunkOrder order(seed_chunk(a, slot0, nch), nch);
+    __syncthreads();  // s_qrow
+    // this lane's direct loads: source offset within a stage (candidate rows
+    // relative to the chunk) and whether it is a query row
+    int off[S::L];
+    bool isq[S::L];
+#pragma unroll
+    for (int j = 0; j < S::L; j++) {
+        const int e = min(w + j * kWideWaves, S::TL - 1);
+        const int v = e * 64 + lane, r = v / RSS, k = v - r * RSS;
+        const int kk = k < 2 * KS ? k : 0;  // the pad slot: any vector of the row (never read)
+        isq[j] = r >= kCB;
+        off[j] = (isq[j] ? s_qrow[r - kCB] : r) * (2 * KS) + kk;
+    }
+    const uint4* A4 = reinterpret_cast<const uint4*>(A);
+    const uint4* B4 = reinterpret_cast<const uint4*>(B);
+    unsigned char* ring = dyn_lds;
+    wide_copy<KS>(A4, B4, Npad, nst, order, 0, off, isq, ring, w);
+    int i = 0;  // stage of the stream
+    for (int c = 0; c < nch; c++) {
+        const int cb = order.at(c) * kCB;
+        f32x16 tot[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) tot[t] = f32x16{};
+        for (int st = 0; st < nst; st++, i++) {
+            __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of stage i have landed
+            __syncthreads();                // every wave's have; the other image is free
+            if (i + 1 < T) wide_copy<KS>(A4, B4, Npad, nst, order, i + 1, off, isq, ring, w);
+            const V* sC = lds + (i & 1) * S::IMG;
+            const V* crow = sC + (64 * ch + (lane & 31)) * RSS + half * KS;
+            const V* qrow = sC + kCB * RSS + sl * RSS + half * KS;
+            // every fragment of the stage first, then the MFMAs back to back
+            V fa[KS][2], fb[KS];
+#pragma unroll
+            for (int g = 0; g < KS; g++) {
+                fb[g] = qrow[g];
+#pragma unroll
+                for (int t = 0; t < 2; t++) fa[g][t] = crow[32 * t * RSS + g];
+            }
+            f32x16 acc[2];
+#pragma unroll
+            for (int g = 0; g < KS; g++)
+#pragma unroll
+                for (int t = 0; t < 2; t++) acc[t] = Op::mma(fa[g][t], fb[g], g == 0 ? f32x16{} : acc[t]);
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) tot[t][e] += acc[t][e];
+            if (st == nst - 1) {  // the chunk's values (norms folded into the operands)
+                const bool seeding = c < order.nseed;
+                const float tau = s_tau[sl];
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    float v[16];
+#pragma unroll
+                    for (int e = 0; e < 16; e++) v[e] = tot[t][e];
+                    if (seeding) {
+                        if (q >= 0) seed_store(v, 2 * ch + t, cb, q, c, a, qbuf);
+                    } else {
+                        filter_tile(v, 2 * ch + t, cb, q, sl, tau, a, s_cnt, qbuf);
+                    }
+                }
+                __syncthreads();  // both candidate halves of the chunk are in the rows' buffers
+                if (ch == 0) {
+                    if (c == order.nseed - 1 && lane < 32 && q >= 0) s_cnt[sl] = order.nseed * kCB;
+                    if (c >= order.nseed - 1) compact_if_needed<32>(qt, a, slot0, s_cnt, s_tau);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (ch == 0 && lane < 32 && slot0 + sl < a.nq) {
+        a.cnt_out[slot0 + sl] = s_cnt[sl];
+        a.tau_out[slot0 + sl] = s_tau[sl];
+    }
+}
+
+// Padded half-major rows (RS = 2 NV + 1 vectors, Npad rows) to the
+// stage-major layout of knn_tiles_wide (stages of KS vectors per half).
+template <int KS>
+__global__ __launch_bounds__(256) void hm_to_stages(const uint4* __restrict__ hm, int Npad, int NV,
+                                                    uint4* __restrict__ W)
+{
+    const int RS = 2 * NV + 1;
+    const size_t total = (size_t)(NV / KS) * Npad * 2 * KS;
+    for (size_t o = (size_t)blockIdx.x * 256 + threadIdx.x; o < total; o += (size_t)gridDim.x * 256) {
+        const int k = (int)(o % (2 * KS));
+        const size_t sr = o / (2 * KS);
+        const int r = (int)(sr % Npad), st = (int)(sr / Npad);
+        const int h = k / KS, g = st * KS + (k - h * KS);
+        W[o] = hm[(size_t)r * RS + h * NV + g];
+    }
+}
+
+constexpr int kWideKS = 8;  // vectors per half row per k-stage of knn_tiles_wide
+
+// Operand width (vectors per half row) of the wide tiles: a whole number of stages.
+inline int wide_nv(int NV) { return (NV + kWideKS - 1) / kWideKS * kWideKS; }
+
+// BF16 operands over the k-values
+//   candidates A = [c_hi | c_lo | c_hi | n_hi n_lo | 0...]
+//   queries    B = [-2 q_hi | -2 q_hi | -2 q_lo | 1 1 | 0...]
+// so that A.B = ||c||^2 - 2 q.c with the q_lo*c_lo term dropped (~2^-16
+// relative); scaling by -2 is exact in BF16.  Padded half-major rows:
+// vector (h, g) of a row holds k = 16 g + 8 h + 0..7; the pad vector and
+// the rows past N are zero.  A and B share the layout.
 __global__ __launch_bounds__(256) void prep_bf16x3_hm(const double* __restrict__ X, const double* __restrict__ n2,
                                                       int N, int Npad, int D, int NV, __bf16* __restrict__ A,
                                                       __bf16* __restrict__ B)
@@ -1224,11 +1236,10 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     }
 }
 
-__global__ void collect_failures(const int* __restrict__ fail, int n, const uint32_t* __restrict__ perm,
-                                 int* __restrict__ list, int* __restrict__ count)
+__global__ void collect_failures(const int* __restrict__ fail, int n, int* __restrict__ list, int* __restrict__ count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && fail[i]) list[atomicAdd(count, 1)] = perm ? (int)perm[i] : i;
+    if (i < n && fail[i]) list[atomicAdd(count, 1)] = i;
 }
 
 // ---------------------------------------------------------------- locality order
@@ -1381,24 +1392,17 @@ struct TileRun {
     DevBuf<int> fail;
 };
 
-void run_tiles(int dtype, const void* op_a, const void* op_b, const float* nrm, int N, int D, int Kp, int kprime,
-               const int* qlist, int nq, TileRun& tr, MemBudget* mb, hipStream_t s)
+// FP8 rows too wide for the register tiles: LDS-staged tiles.
+void run_tiles_fp8(const uint8_t* A8, const float* scale, const float* nrm, int N, int Kp, int kprime, TileRun& tr,
+                   MemBudget* mb, hipStream_t s)
 {
-    const int nslots = (int)ceil_div(nq, kQB) * kQB;
+    const int nslots = (int)ceil_div(N, kQB) * kQB;
     tr.buf.alloc((size_t)nslots * kCap, mb);
     tr.cnt.alloc(nslots, mb);
     tr.tau.alloc(nslots, mb);
     tr.fail.alloc(nslots, mb);
-    TileArgs ta{N, kprime, 0, qlist, nq, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
-    const dim3 grid(ceil_div(nq, kQB));
-    if (dtype == TKA_DIST_BF16) {
-        hipLaunchKernelGGL(knn_tiles_bf16, grid, dim3(256), 0, s, (const __bf16*)op_a, (const __bf16*)op_b, Kp, ta);
-    } else if (dtype == TKA_DIST_FP8) {
-        hipLaunchKernelGGL(knn_tiles_fp8, grid, dim3(256), 0, s, (const uint8_t*)op_a, (const float*)op_b, nrm, Kp,
-                           ta);
-    } else {
-        hipLaunchKernelGGL(knn_tiles_fp32, grid, dim3(256), 0, s, (const float*)op_a, nrm, D, ta);
-    }
+    TileArgs ta{N, kprime, 0, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
+    hipLaunchKernelGGL(knn_tiles_fp8, dim3(ceil_div(N, kQB)), dim3(256), 0, s, A8, scale, nrm, Kp, ta);
     TKA_LAUNCH_CHECK();
 }
 
@@ -1417,21 +1421,22 @@ void launch_tiles_reg(const void* A, const void* B, dim3 grid, size_t lds, const
 }
 
 template <class Op, int kMode, int QT, int WAVES>
-void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, TileRun& tr, MemBudget* mb, hipStream_t s)
+void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, const int* qlist, int nq, TileRun& tr,
+                   MemBudget* mb, hipStream_t s)
 {
     typedef typename Op::V V;
     constexpr bool kSplits = std::is_same<Op, OpBF16>::value;  // two-stage rows: BF16 only
     require(NV >= 1 && NV <= (kSplits ? kNVSplitLimit : kNVLimit), TKA_E_INTERNAL,
             "register tiles: bad operand width");
     constexpr int QB = 32 * WAVES * QT;
-    const int nslots = (int)ceil_div(N, QB) * QB;
+    const int nslots = (int)ceil_div(nq, QB) * QB;
     tr.buf.alloc((size_t)nslots * kCap, mb);
     tr.cnt.alloc(nslots, mb);
     tr.tau.alloc(nslots, mb);
     tr.fail.alloc(nslots, mb);
-    TileArgs ta{N, kprime, 1, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
+    TileArgs ta{N, kprime, 1, qlist, nq, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
     const size_t lds = 2 * (size_t)kCB * (2 * split_nv1(NV) + 1) * sizeof(V);
-    const dim3 grid(ceil_div(N, QB));
+    const dim3 grid(ceil_div(nq, QB));
 #define TKA_W(W) \
     case W: launch_tiles_reg<Op, kMode, W, QT, WAVES>(A, B, grid, lds, ta, s); break
     switch (NV) {
@@ -1446,6 +1451,44 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, Tile
     }
 #undef TKA_W
     TKA_LAUNCH_CHECK();
+}
+
+// Query rows (qlist[0, nq), or all rows) against all rows with both operands
+// streamed in k-stages (knn_tiles_wide).  A, B: padded half-major rows of NV
+// vectors per half (NV a multiple of kWideKS) for ceil(N / kCB) * kCB rows.
+template <class Op>
+void launch_tiles_wide(const void* A, const void* B, int NV, int Npad, int nq, const TileArgs& ta, hipStream_t s)
+{
+    typedef typename Op::V V;
+    const size_t lds = 2 * (size_t)WideStage<kWideKS>::IMG * sizeof(V);
+    hipLaunchKernelGGL((knn_tiles_wide<Op, kWideKS>), dim3(ceil_div(nq, kWideQB)), dim3(64 * kWideWaves), lds, s,
+                       (const V*)A, (const V*)B, NV, Npad, ta);
+}
+
+// A, B: padded half-major rows (NV vectors per half, a multiple of kWideKS)
+// for Npad = ceil(N / kCB) * kCB rows; copied to the stage-major layout here.
+template <class Op>
+void run_tiles_wide(const void* A, const void* B, int NV, int N, int kprime, const int* qlist, int nq, TileRun& tr,
+                    MemBudget* mb, hipStream_t s)
+{
+    require(NV >= kWideKS && NV % kWideKS == 0, TKA_E_INTERNAL, "wide tiles: bad operand width");
+    const int Npad = (int)ceil_div(N, kCB) * kCB;
+    const size_t nvec = (size_t)Npad * 2 * NV;
+    DevBuf<uint4> Aw(nvec, mb), Bw(nvec, mb);
+    const unsigned g = (unsigned)std::min<size_t>(8192, ceil_div(nvec, 256));
+    hipLaunchKernelGGL(hm_to_stages<kWideKS>, dim3(g), dim3(256), 0, s, (const uint4*)A, Npad, NV, Aw.get());
+    TKA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(hm_to_stages<kWideKS>, dim3(g), dim3(256), 0, s, (const uint4*)B, Npad, NV, Bw.get());
+    TKA_LAUNCH_CHECK();
+    const int nslots = (int)ceil_div(nq, kWideQB) * kWideQB;
+    tr.buf.alloc((size_t)nslots * kCap, mb);
+    tr.cnt.alloc(nslots, mb);
+    tr.tau.alloc(nslots, mb);
+    tr.fail.alloc(nslots, mb);
+    TileArgs ta{N, kprime, 1, qlist, nq, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
+    launch_tiles_wide<Op>(Aw.get(), Bw.get(), NV, Npad, nq, ta, s);
+    TKA_LAUNCH_CHECK();
+    TKA_HIP(hipStreamSynchronize(s));  // Aw, Bw are released on return
 }
 
 }  // namespace
@@ -1475,28 +1518,12 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     TKA_HIP(hipMemcpyAsync(&rmax, rmaxd.get(), sizeof(double), hipMemcpyDeviceToHost, s));
 
     // Operands.
-    DevBuf<float> Xf, nrm;
     DevBuf<__bf16> A, B;
     DevBuf<uint8_t> A8;
     DevBuf<float> scale8, nrm8;
     int Kp = 0;
-    auto make_fp32 = [&]() {
-        if (Xf.get()) return;
-        Xf.alloc((size_t)N * D, mb);
-        nrm.alloc(N, mb);
-        const size_t total = (size_t)N * D;
-        hipLaunchKernelGGL(prep_fp32, dim3(std::min<size_t>(4096, ceil_div(total, 256))), dim3(256), 0, s, Xn, total,
-                           Xf.get());
-        TKA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(prep_fp32, dim3(ceil_div(N, 256)), dim3(256), 0, s, n2.get(), (size_t)N, nrm.get());
-        TKA_LAUNCH_CHECK();
-    };
     const double u32 = std::ldexp(1.0, -24);
     double eps_a = 0.0, eps_b = 0.0;
-    auto fp32_bound = [&]() {
-        eps_a = 2.0 * (2.0 * (D + 3) * u32);
-        eps_b = 2.0 * (2.0 * u32);
-    };
     TileRun tr;
     const unsigned gprep = (unsigned)std::min<size_t>(8192, ceil_div((size_t)N * 64, 256));
     const int Npad = (int)ceil_div(N, kCB) * kCB;
@@ -1575,9 +1602,34 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         Xs = Xp.get();
         ns = n2p.get();
     };
+    // FP32 MFMA tiles over the locality-ordered rows for the query rows
+    // ql[0, nq) (all rows when ql is null), candidate lists of kp; sets the
+    // error bound of the values they produce.  The norm is one more term of
+    // the FP32 sum: its rounding is bounded like the products' (D+1 terms
+    // plus the two conversions).
+    auto fp32_tiles = [&](const int* ql, int nq, int kp, TileRun& t) {
+        const int NV = ((D + 2) / 2 + 3) / 4;  // float4 vectors of k-steps (2 k per step) per half row
+        const bool reg = reg_tiles_fit(NV, sizeof(float4));
+        const int NVo = reg ? NV : wide_nv(NV);
+        DevBuf<float> Ah((size_t)Npad * 4 * (2 * NVo + 1), mb), Bh((size_t)Npad * 4 * (2 * NVo + 1), mb);
+        hipLaunchKernelGGL(prep_fp32n_hm, dim3(gprep), dim3(256), 0, s, Xs, ns ? ns : n2.get(), N, Npad, D, NVo,
+                           Ah.get(), Bh.get());
+        TKA_LAUNCH_CHECK();
+        if (reg) {
+            run_tiles_reg<OpF32, 0, 1, 8>(Ah.get(), Bh.get(), NV, N, kp, ql, nq, t, mb, s);
+            eps_a = 2.0 * (2.0 * (D + 3) * u32);
+        } else {
+            // stages of 8 kWideKS products summed from zero, one addition per stage
+            run_tiles_wide<OpF32>(Ah.get(), Bh.get(), NVo, N, kp, ql, nq, t, mb, s);
+            eps_a = 2.0 * (2.0 * (8 * kWideKS + NVo / kWideKS + 2) * u32);
+        }
+        eps_b = eps_a;
+        TKA_HIP(hipStreamSynchronize(s));  // Ah, Bh are released on return
+    };
     if (dtype == TKA_DIST_BF16) {
         Kp = ((3 * D + 2) + 15) / 16 * 16;
         const int NV = Kp / 16;
+        int nsum = Kp;  // roundings in one value's FP32 sum
         if (reg_tiles_fit(NV, sizeof(bf16x8), true)) {
             locality();
             const size_t rows = (size_t)Npad * 8 * (2 * NV + 1);
@@ -1585,16 +1637,26 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
             B.alloc(rows, mb);
             hipLaunchKernelGGL(prep_bf16x3_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, A.get(), B.get());
             TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpBF16, 0, 1, 8>(A.get(), B.get(), NV, N, kprime, tr, mb, s);
+            run_tiles_reg<OpBF16, 0, 1, 8>(A.get(), B.get(), NV, N, kprime, nullptr, N, tr, mb, s);
         } else {
-            A.alloc((size_t)N * Kp, mb);
-            B.alloc((size_t)N * Kp, mb);
-            hipLaunchKernelGGL(prep_bf16x3, dim3(gprep), dim3(256), 0, s, Xn, n2.get(), N, D, Kp, A.get(), B.get());
+            // both operands streamed in k-stages (the same values, NV padded
+            // with zero vectors to whole stages)
+            locality();
+            const int NVw = wide_nv(NV);
+            const size_t rows = (size_t)Npad * 8 * (2 * NVw + 1);
+            A.alloc(rows, mb);
+            B.alloc(rows, mb);
+            hipLaunchKernelGGL(prep_bf16x3_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NVw, A.get(),
+                               B.get());
             TKA_LAUNCH_CHECK();
-            run_tiles(dtype, A.get(), B.get(), nullptr, N, D, Kp, kprime, nullptr, N, tr, mb, s);
+            run_tiles_wide<OpBF16>(A.get(), B.get(), NVw, N, kprime, nullptr, N, tr, mb, s);
+            // each stage's 16 kWideKS products are summed from zero, then
+            // added to the running sum: at most that many roundings plus one
+            // per stage
+            nsum = 16 * kWideKS + NVw / kWideKS;
         }
         const double ub2 = std::ldexp(1.0, -16);
-        const double gam = Kp * u32;
+        const double gam = nsum * u32;
         eps_a = 2.0 * (6.2 * ub2 + 2.1 * gam);
         eps_b = 2.0 * (1.1 * ub2 + 1.1 * gam);
     } else if (dtype == TKA_DIST_FP8) {
@@ -1613,10 +1675,10 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
             hipLaunchKernelGGL(fp8_to_hm, dim3(gprep), dim3(256), 0, s, A8.get(), nrm8.get(), scale8.get(), N, Npad,
                                Kp / 16, NV, A8h.get());
             TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpFP8, 2, 1, 4>(A8h.get(), A8h.get(), NV, N, kprime, tr, mb, s);
+            run_tiles_reg<OpFP8, 2, 1, 4>(A8h.get(), A8h.get(), NV, N, kprime, nullptr, N, tr, mb, s);
             TKA_HIP(hipStreamSynchronize(s));  // A8h is released on return
         } else {
-            run_tiles(dtype, A8.get(), scale8.get(), nrm8.get(), N, D, Kp, kprime, nullptr, N, tr, mb, s);
+            run_tiles_fp8(A8.get(), scale8.get(), nrm8.get(), N, Kp, kprime, tr, mb, s);
         }
         // e4m3 keeps 4 significant bits (unit roundoff 2^-4): the bound is
         // loose at any useful k'; FP8 rows are not re-solved (approximate mode)
@@ -1624,25 +1686,8 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         eps_a = 2.0 * (2.0 * (2 * u8 + u8 * u8) + 2.0 * D * u32);
         eps_b = 2.0 * (2.0 * u32);
     } else {
-        // float4 vectors of k-steps (2 k per step) per half row, D+1 k-values
-        const int NV = ((D + 2) / 2 + 3) / 4;
-        if (reg_tiles_fit(NV, sizeof(float4))) {
-            locality();
-            DevBuf<float> Ah((size_t)Npad * 4 * (2 * NV + 1), mb), Bh((size_t)Npad * 4 * (2 * NV + 1), mb);
-            hipLaunchKernelGGL(prep_fp32n_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, Ah.get(),
-                               Bh.get());
-            TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpF32, 0, 1, 8>(Ah.get(), Bh.get(), NV, N, kprime, tr, mb, s);
-            TKA_HIP(hipStreamSynchronize(s));  // Ah, Bh are released on return
-            // the norm is one more term of the FP32 chain: its rounding is
-            // bounded like the products' (D+1 terms plus the conversions)
-            eps_a = 2.0 * (2.0 * (D + 3) * u32);
-            eps_b = eps_a;
-        } else {
-            make_fp32();
-            run_tiles(dtype, Xf.get(), nullptr, nrm.get(), N, D, 0, kprime, nullptr, N, tr, mb, s);
-            fp32_bound();
-        }
+        locality();
+        fp32_tiles(nullptr, N, kprime, tr);
     }
     TKA_HIP(hipStreamSynchronize(s));  // rmax is on the host now
     if (order_out && perm.get()) {
@@ -1654,10 +1699,11 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(N, 4)), dim3(256), 0, s, ra);
     TKA_LAUNCH_CHECK();
 
-    // Rows whose bound failed: FP32 tiles with the widest candidate list.
+    // Rows whose bound failed: FP32 tiles with the widest candidate list
+    // (rows in the tiles' order; the re-rank maps them to the caller's).
     DevBuf<int> list(N, mb), nfail(1, mb);
     TKA_HIP(hipMemsetAsync(nfail.get(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), N, perm.get(), list.get(),
+    hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), N, list.get(),
                        nfail.get());
     TKA_LAUNCH_CHECK();
     int hf = 0;
@@ -1668,18 +1714,17 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     if (hf > 0 && dtype != TKA_DIST_FP8) {
         // Deterministic order of the re-solved rows does not matter: each row
         // is solved independently and written to its own output slot.
-        make_fp32();
-        fp32_bound();
         TileRun tr2;
         const int kp2 = std::min(kMaxKprime, N - 1);
-        run_tiles(TKA_DIST_FP32, Xf.get(), nullptr, nrm.get(), N, D, 0, kp2, list.get(), hf, tr2, mb, s);
-        RerankArgs rb{Xn, n2.get(), N, D, K, list.get(), hf, tr2.buf.get(), tr2.cnt.get(), tr2.tau.get(),
-                      eps_a, eps_b, rmax, idx, dist, tr2.fail.get(), nullptr};
+        fp32_tiles(list.get(), hf, kp2, tr2);
+        RerankArgs rb{Xs, ns ? ns : n2.get(), N, D, K, list.get(), hf, tr2.buf.get(), tr2.cnt.get(), tr2.tau.get(),
+                      eps_a, eps_b, rmax, idx, dist, tr2.fail.get(), perm.get()};
         hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(hf, 4)), dim3(256), 0, s, rb);
         TKA_LAUNCH_CHECK();
+        DevBuf<int> list2(hf, mb);
         TKA_HIP(hipMemsetAsync(nfail.get(), 0, sizeof(int), s));
-        hipLaunchKernelGGL(collect_failures, dim3(ceil_div(hf, 256)), dim3(256), 0, s, tr2.fail.get(), hf, nullptr, list.get(),
-                           nfail.get());
+        hipLaunchKernelGGL(collect_failures, dim3(ceil_div(hf, 256)), dim3(256), 0, s, tr2.fail.get(), hf,
+                           list2.get(), nfail.get());
         TKA_LAUNCH_CHECK();
         int hf2 = 0;
         TKA_HIP(hipMemcpyAsync(&hf2, nfail.get(), sizeof(int), hipMemcpyDeviceToHost, s));

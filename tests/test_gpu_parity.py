@@ -39,15 +39,32 @@ def test_normalize_bit_exact(engine, oracle, shape):
 # d = 128: FP32 operand rows of 17 vectors (D+1 k-values), BF16 rows of 25
 # vectors (3D+2 k-values) streamed in two k-stages; d = 93: BF16 rows of 18
 # vectors; d = 107 and 148: the narrowest (21) and widest (28) two-stage BF16
-# rows; d = 784: the LDS-staged kernels
+# rows; d = 150 (BF16 29 vectors, padded to 32), 333 (BF16 63 -> 64, FP32
+# 42 -> 44) and 784: rows streamed through LDS in k-stages of 4 vectors
 @pytest.mark.parametrize("n,d,K", [(700, 20, 30), (1500, 50, 90), (300, 784, 20), (129, 3, 128), (600, 128, 40),
-                                   (500, 93, 30), (400, 107, 25), (350, 148, 20)])
+                                   (500, 93, 30), (400, 107, 25), (350, 148, 20), (450, 150, 20), (260, 333, 30)])
 def test_knn_bit_exact(engine, oracle, dtype, n, d, K):
     X = oracle.normalize(mixture(n, d))
     idx, dist, fails = engine.knn(X, K, dist_dtype=dtype)
     ridx, rdist = oracle.knn_brute(X, K)
     np.testing.assert_array_equal(idx, ridx)
     np.testing.assert_array_equal(dist, rdist)
+
+
+@pytest.mark.parametrize("dtype", ["fp32", "bf16"])
+def test_knn_wide_rows_many_blocks(engine, oracle, dtype):
+    # N = 33,000 rows of d = 160 (BF16 31 vectors per half row, FP32 21):
+    # enough query blocks for the four-wave launch of the k-staged tiles;
+    # every row's K = 30 neighbours are exact (checked on a sample of rows
+    # against brute force), and both precisions give the same graph
+    n, d, K = 33_000, 160, 30
+    X = oracle.normalize(mixture(n, d, k=40, seed=7))
+    idx, dist, fails = engine.knn(X, K, dist_dtype=dtype)
+    rows = np.unique(np.concatenate([np.arange(0, n, 997), [n - 1]]))
+    ridx, rdist = oracle.knn_brute_rows(X, K, rows)
+    np.testing.assert_array_equal(idx[rows], ridx)
+    np.testing.assert_array_equal(dist[rows], rdist)
+    assert np.all(np.diff(dist, axis=1) >= 0)
 
 
 def test_knn_fp8_is_approximate_but_close(engine, oracle):
