@@ -20,6 +20,9 @@ HBM; `value` is iterations per second.  The same JSON line also reports:
                 mixture) with FP32, BF16 and FP8 e4m3 distance tiles
   exact_theta0  workload 1 (theta=0, N=2000): exact-loop iterations/s with the
                 dense and the kNN P, all-pairs force kernel time (and at N=10k)
+  wide_rows_d784  workload 2 (N=20,000, D=784): kNN + P with FP32 and BF16
+                distance tiles streamed in k-stages, MFMA rate over the kNN
+                stage, recall and max |dP| of BF16 against FP32
   parametric    N=10,000, D=784: a 784-256-256-2 MLP trained by Adam on the
                 engine's KL autograd function vs non-parametric t-SNE
   roofline      of the dominant kernel (HIP events on the engine's stream)
@@ -191,6 +194,43 @@ def precision_sweep(tapkee_amd, datasets, perp=30.0):
     return out
 
 
+def wide_leg(tapkee_amd, datasets, perp=30.0):
+    """Workload 2 (N=20000, D=784 sparse image prototypes, perplexity 30,
+    K=90): kNN + P with FP32 and BF16 distance tiles, the rows streamed
+    through LDS in k-stages of 8 vectors: device time of each stage (from a
+    session's setup), the MFMA work of the distance tiles (FP32: D+1 terms
+    per pair, BF16: 3D+2 split terms) over the kNN stage time -- a lower
+    bound on the tiles' own rate, which also pays the locality order and the
+    FP64 re-rank -- and recall / max |dP| of BF16 against FP32."""
+    X2, _ = datasets.image_prototypes()
+    X2 = X2.astype("float64")
+    N, D = X2.shape
+    K = int(3 * perp)
+    out = {"workload": f"image prototypes N={N} D={D} perplexity={perp} K={K} (workload 2)"}
+    ref_idx = ref_p = None
+    for dt in ("fp32", "bf16"):
+        s = tapkee_amd.Session(X2, perplexity=perp, theta=0.5, dist_dtype=dt, seed=1, log_every=0)
+        st = s.stats()
+        s.close()
+        idx, _, fails = tapkee_amd.knn(X2, K, dist_dtype=dt)
+        P = tapkee_amd.build_p(X2, perp, dist_dtype=dt)
+        if dt == "fp32":
+            ref_idx, ref_p = idx, P
+        r, exact = recall(idx, ref_idx)
+        terms = D + 1 if dt == "fp32" else 3 * D + 2
+        mfma_tf = 2.0 * N * N * terms / (st["ms_knn"] * 1e-3) / 1e12
+        peak = 157.3 if dt == "fp32" else 2500.0
+        out[dt] = {"knn_ms": round(st["ms_knn"], 3), "p_ms": round(st["ms_perplexity"] + st["ms_symmetrize"], 3),
+                   "knn_p_ms": round(st["ms_knn"] + st["ms_perplexity"] + st["ms_symmetrize"], 3),
+                   "mfma_tflops_over_knn_stage": round(mfma_tf, 2), "mfma_peak_tflops": peak,
+                   "mfma_frac_over_knn_stage": round(mfma_tf / peak, 4),
+                   "recall_at_k": r, "rows_exact": exact, "certificate_failures": fails,
+                   "max_abs_dP_vs_fp32": max_dp(P, ref_p)}
+        log(f"[bench] wide {dt}: {out[dt]}")
+    out["bf16_speedup_knn_p"] = round(out["fp32"]["knn_p_ms"] / out["bf16"]["knn_p_ms"], 3)
+    return out
+
+
 def exact_leg(tapkee_amd, datasets, perp=30.0, steps=500):
     """Workload 1 (theta=0, N=2000, D=50): iterations/s of the exact loop with
     the reference's dense P and with the kNN P (K=90), and the all-pairs
@@ -261,6 +301,7 @@ def main():
     ap.add_argument("--no-accuracy", action="store_true", help="skip the error-vs-FP64 measurements")
     ap.add_argument("--no-parametric", action="store_true", help="skip the parametric (MLP) t-SNE leg")
     ap.add_argument("--no-exact", action="store_true", help="skip the theta=0 (workload 1) leg")
+    ap.add_argument("--no-wide", action="store_true", help="skip the D=784 kNN + P leg (workload 2)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run 3 plain device-to-device copies of MIB MiB (a known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -415,6 +456,9 @@ def main():
     exact = None
     if not a.no_exact:
         exact = exact_leg(tapkee_amd, datasets, perp)
+    wide = None
+    if not a.no_wide:
+        wide = wide_leg(tapkee_amd, datasets, perp)
     param = None
     if not a.no_parametric:
         param = parametric_leg(tapkee_amd, datasets, perp)
@@ -479,6 +523,7 @@ def main():
         "pipeline": pipeline,
         "precision_sweep": sweep,
         "exact_theta0": exact,
+        "wide_rows_d784": wide,
         "parametric": param,
     }
     if cpu:
