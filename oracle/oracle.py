@@ -84,6 +84,12 @@ def lib():
     L.or_symmetrize.restype = C.c_longlong
     L.or_gradient_bh.argtypes = [_ip, _ip, _dp, _dp, C.c_int, C.c_double, _dp]
     L.or_gradient_bh.restype = C.c_double
+    L.or_gradient_bh3.argtypes = [_ip, _ip, _dp, _dp, C.c_int, C.c_double, _dp]
+    L.or_gradient_bh3.restype = C.c_double
+    L.or_kl_bh3.argtypes = [_ip, _ip, _dp, _dp, C.c_int, C.c_double]
+    L.or_kl_bh3.restype = C.c_double
+    L.or_octree_stats.argtypes = [_dp, C.c_int, C.c_int, _ip]
+    L.or_octree_stats.restype = C.c_int
     L.or_quadtree_export.argtypes = [_dp, C.c_int, C.c_int, _ip, _ip, _ip, _ip, _dp, _dp, _ip, _dp]
     L.or_quadtree_export.restype = C.c_int
     L.or_gradient_exact_dense.argtypes = [_dp, _dp, C.c_int, C.c_int, _dp, C.c_int]
@@ -233,9 +239,13 @@ def build_p(X, perplexity, K=None, knn_mode=1):
 
 
 def gradient_bh(row, col, val, Y, theta):
+    """Barnes-Hut gradient (no factor 4) and Z: the reference's quadtree for
+    N x 2 embeddings, its octree generalisation for N x 3."""
     Y = _f64(Y)
     dC = np.empty_like(Y)
-    z = lib().or_gradient_bh(_i32(row), _i32(col), _f64(val), Y, Y.shape[0], theta, dC)
+    fn = lib().or_gradient_bh3 if Y.shape[1] == 3 else lib().or_gradient_bh
+    assert Y.shape[1] in (2, 3)
+    z = fn(_i32(row), _i32(col), _f64(val), Y, Y.shape[0], theta, dC)
     return dC, z
 
 
@@ -260,7 +270,17 @@ def kl_exact_dense(P, Y, compat=False):
 
 def kl_bh(row, col, val, Y, theta):
     Y = _f64(Y)
-    return lib().or_kl_bh(_i32(row), _i32(col), _f64(val), Y, Y.shape[0], theta)
+    assert Y.shape[1] in (2, 3)
+    fn = lib().or_kl_bh3 if Y.shape[1] == 3 else lib().or_kl_bh
+    return fn(_i32(row), _i32(col), _f64(val), Y, Y.shape[0], theta)
+
+
+def octree_stats(Y, cap=64):
+    """Octree of N x 3 points: (non-empty node count, non-empty cells per level)."""
+    Y = _f64(Y)
+    per = np.empty(cap, np.int32)
+    n = lib().or_octree_stats(Y, Y.shape[0], cap, per)
+    return n, per
 
 
 def kl_exact_sparse(row, col, val, Y):

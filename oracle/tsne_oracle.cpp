@@ -625,6 +625,179 @@ OR_API double or_gradient_bh(const int* row_P, const int* col_P, const double* v
     return sum_Q;
 }
 
+// ---------------------------------------------------------------------------
+// Pointer octree: the quadtree above (quadtree.hpp:44-384) with a third axis.
+// The reference's tree is 2-D only (QT_NO_DIMS = 2, quadtree.hpp:71); this
+// is its direct generalisation, the definition the 3-D Barnes-Hut engine is
+// checked against.  Root box centred on the mean with half-widths
+// max(max-mean, mean-min)+1e-5 per axis; capacity-1 leaves; online centre of
+// mass; duplicates counted but not stored; child c = 4 zbit + 2 ybit + xbit
+// (bit 1 = upper half), tried in order 0..7 with closed-interval containment
+// (a point on a split plane goes to the lower half); a cell is a summary when
+// it is a leaf or max(half-widths)/sqrt(D) < theta.
+// ---------------------------------------------------------------------------
+namespace {
+struct ONode {
+    double c[3], h[3];
+    bool is_leaf = true;
+    int size = 0;
+    int cum_size = 0;
+    double com[3] = {0.0, 0.0, 0.0};
+    int index = -1;
+    ONode* child[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int level = 0;
+
+    bool contains(const double* p) const
+    {
+        for (int d = 0; d < 3; d++) {
+            if (c[d] - h[d] > p[d]) return false;
+            if (c[d] + h[d] < p[d]) return false;
+        }
+        return true;
+    }
+};
+
+struct OTree {
+    const double* Y;
+    ONode* root = nullptr;
+
+    static ONode* make(const double* c, const double* h, int level)
+    {
+        ONode* q = new ONode();
+        for (int d = 0; d < 3; d++) {
+            q->c[d] = c[d];
+            q->h[d] = h[d];
+        }
+        q->level = level;
+        return q;
+    }
+
+    void subdivide(ONode* q)
+    {
+        double h[3];
+        for (int d = 0; d < 3; d++) h[d] = .5 * q->h[d];
+        for (int k = 0; k < 8; k++) {
+            double c[3];
+            for (int d = 0; d < 3; d++) c[d] = ((k >> d) & 1) ? q->c[d] + h[d] : q->c[d] - h[d];
+            q->child[k] = make(c, h, q->level + 1);
+        }
+        for (int i = 0; i < q->size; i++) {
+            bool ok = false;
+            for (int k = 0; k < 8 && !ok; k++) ok = insert(q->child[k], q->index);
+            q->index = -1;
+        }
+        q->size = 0;
+        q->is_leaf = false;
+    }
+
+    bool insert(ONode* q, int idx)
+    {
+        const double* p = Y + (size_t)idx * 3;
+        if (!q->contains(p)) return false;
+        q->cum_size++;
+        const double mult1 = (double)(q->cum_size - 1) / (double)q->cum_size;
+        const double mult2 = 1.0 / (double)q->cum_size;
+        for (int d = 0; d < 3; d++) q->com[d] *= mult1;
+        for (int d = 0; d < 3; d++) q->com[d] += mult2 * p[d];
+        if (q->is_leaf && q->size < 1) {
+            q->index = idx;
+            q->size++;
+            return true;
+        }
+        if (q->size == 1) {
+            const double* o = Y + (size_t)q->index * 3;
+            if (o[0] == p[0] && o[1] == p[1] && o[2] == p[2]) return true;  // duplicate
+        }
+        if (q->is_leaf) subdivide(q);
+        for (int k = 0; k < 8; k++)
+            if (insert(q->child[k], idx)) return true;
+        return false;
+    }
+
+    void build(const double* Yin, int N)
+    {
+        Y = Yin;
+        double mean[3] = {0, 0, 0}, mn[3] = {DBL_MAX, DBL_MAX, DBL_MAX}, mx[3] = {-DBL_MAX, -DBL_MAX, -DBL_MAX};
+        for (int n = 0; n < N; n++)
+            for (int d = 0; d < 3; d++) {
+                const double v = Y[(size_t)n * 3 + d];
+                mean[d] += v;
+                if (v < mn[d]) mn[d] = v;
+                if (v > mx[d]) mx[d] = v;
+            }
+        double h[3];
+        for (int d = 0; d < 3; d++) {
+            mean[d] /= (double)N;
+            h[d] = std::max(mx[d] - mean[d], mean[d] - mn[d]) + 1e-5;
+        }
+        root = make(mean, h, 0);
+        for (int i = 0; i < N; i++) insert(root, i);
+    }
+
+    static void destroy(ONode* q)
+    {
+        if (!q) return;
+        for (int k = 0; k < 8; k++) destroy(q->child[k]);
+        delete q;
+    }
+
+    void non_edge(const ONode* q, int pi, double theta, double* neg_f, double* sum_Q) const
+    {
+        if (q->cum_size == 0 || (q->is_leaf && q->size == 1 && q->index == pi)) return;
+        double buff[3];
+        double D = 0.0;
+        for (int d = 0; d < 3; d++) buff[d] = Y[(size_t)pi * 3 + d] - q->com[d];
+        for (int d = 0; d < 3; d++) D += buff[d] * buff[d];
+        if (q->is_leaf || std::max(std::max(q->h[0], q->h[1]), q->h[2]) / std::sqrt(D) < theta) {
+            const double Q = 1.0 / (1.0 + D);
+            *sum_Q += q->cum_size * Q;
+            const double mult = q->cum_size * Q * Q;
+            for (int d = 0; d < 3; d++) neg_f[d] += mult * buff[d];
+        } else {
+            for (int k = 0; k < 8; k++) non_edge(q->child[k], pi, theta, neg_f, sum_Q);
+        }
+    }
+};
+}  // namespace
+
+// Barnes-Hut gradient on the octree (3-D counterpart of or_gradient_bh; no
+// factor 4: dC = pos_f - neg_f/sum_Q).  Y is N x 3.
+OR_API double or_gradient_bh3(const int* row_P, const int* col_P, const double* val_P, const double* Y, int N,
+                              double theta, double* dC)
+{
+    OTree t;
+    t.build(Y, N);
+    std::vector<double> pos(3 * (size_t)N, 0.0), neg(3 * (size_t)N, 0.0);
+    double sum_Q = 0.0;
+    edge_forces(row_P, col_P, val_P, Y, N, 3, pos.data());
+    for (int n = 0; n < N; n++) t.non_edge(t.root, n, theta, neg.data() + 3 * (size_t)n, &sum_Q);
+    for (size_t i = 0; i < 3 * (size_t)N; i++) dC[i] = pos[i] - (neg[i] / sum_Q);
+    OTree::destroy(t.root);
+    return sum_Q;
+}
+
+// Octree node count, depth and the number of non-empty cells per level (up
+// to cap levels), for structural checks.
+OR_API int or_octree_stats(const double* Y, int N, int cap, int* per_level)
+{
+    OTree t;
+    t.build(Y, N);
+    for (int l = 0; l < cap; l++) per_level[l] = 0;
+    int nodes = 0;
+    std::vector<const ONode*> st{t.root};
+    while (!st.empty()) {
+        const ONode* q = st.back();
+        st.pop_back();
+        if (q->cum_size == 0) continue;
+        nodes++;
+        if (q->level < cap) per_level[q->level]++;
+        for (int k = 0; k < 8; k++)
+            if (q->child[k]) st.push_back(q->child[k]);
+    }
+    OTree::destroy(t.root);
+    return nodes;
+}
+
 // Flattened export of the quadtree for structural comparison.  For every node
 // (pre-order, children NW,NE,SW,SE): level, cum_size, is_leaf, stored index,
 // com x/y, number of non-empty children.  Returns the node count; writes at
@@ -759,6 +932,29 @@ OR_API double or_kl_bh(const int* row_P, const int* col_P, const double* val_P, 
             double Q = 0.0;
             for (int d = 0; d < 2; d++) {
                 double b = Y[(size_t)n * 2 + d] - Y[(size_t)col_P[i] * 2 + d];
+                Q += b * b;
+            }
+            Q = (1.0 / (1.0 + Q)) / sum_Q;
+            C += val_P[i] * std::log((val_P[i] + FLT_MIN) / (Q + FLT_MIN));
+        }
+    return C;
+}
+
+// or_kl_bh with the octree's estimate of Z (Y is N x 3).
+OR_API double or_kl_bh3(const int* row_P, const int* col_P, const double* val_P, const double* Y, int N, double theta)
+{
+    OTree t;
+    t.build(Y, N);
+    double buff[3] = {0, 0, 0};
+    double sum_Q = 0.0;
+    for (int n = 0; n < N; n++) t.non_edge(t.root, n, theta, buff, &sum_Q);
+    OTree::destroy(t.root);
+    double C = 0.0;
+    for (int n = 0; n < N; n++)
+        for (int i = row_P[n]; i < row_P[n + 1]; i++) {
+            double Q = 0.0;
+            for (int d = 0; d < 3; d++) {
+                double b = Y[(size_t)n * 3 + d] - Y[(size_t)col_P[i] * 3 + d];
                 Q += b * b;
             }
             Q = (1.0 / (1.0 + Q)) / sum_Q;
@@ -914,6 +1110,8 @@ OR_API int or_tsne_run(double* X, int N, int D, double* Y, int no_dims, double p
             or_gradient_exact_dense(P.data(), Y, N, no_dims, dY.data(), opt->dd_compat);
         else if (exact)
             or_gradient_exact_sparse(row_P.data(), col_P.data(), val_P.data(), Y, N, no_dims, dY.data());
+        else if (no_dims == 3)
+            or_gradient_bh3(row_P.data(), col_P.data(), val_P.data(), Y, N, theta, dY.data());
         else
             or_gradient_bh(row_P.data(), col_P.data(), val_P.data(), Y, N, theta, dY.data());
         or_update(dY.data(), uY.data(), gains.data(), Y, N, no_dims, momentum, eta);
@@ -930,6 +1128,8 @@ OR_API int or_tsne_run(double* X, int N, int D, double* Y, int no_dims, double p
                 C = or_kl_exact_dense(P.data(), Y, N, 2, opt->dd_compat);
             else if (exact)
                 C = or_kl_exact_sparse(row_P.data(), col_P.data(), val_P.data(), Y, N, no_dims);
+            else if (no_dims == 3)
+                C = or_kl_bh3(row_P.data(), col_P.data(), val_P.data(), Y, N, theta);
             else
                 C = or_kl_bh(row_P.data(), col_P.data(), val_P.data(), Y, N, theta);
             if (opt->kl_trace && opt->kl_count < opt->kl_cap) {

@@ -245,6 +245,72 @@ def test_bh_error_shrinks_with_theta(oracle):
     assert errs[1] / np.linalg.norm(exact) < 0.05
 
 
+# ---------------------------------------------------------------- octree (3-D)
+# The octree restatement has no counterpart in the reference (its tree is
+# 2-D).  It is pinned by reducing to the reference's quadtree: points on a
+# coordinate plane through the mean sit on every split plane of the third
+# axis, go to its lower half at every level, and see the quadtree's cells of
+# the other two axes (same child order, same centres of mass, max of the
+# half-widths unchanged by the third axis' 1e-5): the 3-D gradient and Z
+# equal the 2-D ones bit for bit, for each of the three planes.  Its theta=0
+# limit is the exact gradient.
+@pytest.mark.parametrize("plane", [(0, 1), (0, 2), (1, 2)], ids=["xy", "xz", "yz"])
+def test_octree_on_a_plane_is_the_quadtree(oracle, plane):
+    row, col, val, Y2, _ = _csr_problem(oracle, n=300)
+    Y2 = Y2 - Y2.mean(0)  # the mean of the planar coordinates is exactly 0 on the third axis anyway
+    Y3 = np.zeros((len(Y2), 3))
+    Y3[:, plane[0]] = Y2[:, 0]
+    Y3[:, plane[1]] = Y2[:, 1]
+    for theta in (0.3, 0.5, 1.0):
+        g2, z2 = oracle.gradient_bh(row, col, val, Y2, theta)
+        g3, z3 = oracle.gradient_bh(row, col, val, Y3, theta)
+        assert z3 == z2
+        np.testing.assert_array_equal(g3[:, plane[0]], g2[:, 0])
+        np.testing.assert_array_equal(g3[:, plane[1]], g2[:, 1])
+        assert np.all(g3[:, 3 - plane[0] - plane[1]] == 0.0)
+        assert oracle.kl_bh(row, col, val, Y3, theta) == oracle.kl_bh(row, col, val, Y2, theta)
+
+
+def test_octree_theta0_equals_exact(oracle):
+    row, col, val, _, _ = _csr_problem(oracle)
+    Y = rng(4).normal(size=(len(row) - 1, 3)) * 2
+    a, za = oracle.gradient_bh(row, col, val, Y, 0.0)
+    b, zb = oracle.gradient_exact_sparse(row, col, val, Y)
+    np.testing.assert_allclose(a, b, rtol=1e-10, atol=1e-14)
+    assert abs(za - zb) < 1e-10 * zb
+
+
+def test_octree_error_shrinks_with_theta(oracle):
+    row, col, val, _, _ = _csr_problem(oracle, n=400)
+    Y = rng(5).normal(size=(400, 3)) * 3
+    exact, _ = oracle.gradient_exact_sparse(row, col, val, Y)
+    errs = [np.linalg.norm(oracle.gradient_bh(row, col, val, Y, t)[0] - exact) for t in (0.1, 0.4, 0.8)]
+    assert errs[0] < errs[1] < errs[2]
+    assert errs[1] / np.linalg.norm(exact) < 0.05
+
+
+def test_octree_structure_by_hand(oracle):
+    # two points in opposite corners: the root splits once into two leaves;
+    # eight points, one per octant: one root and eight leaves
+    n, per = oracle.octree_stats(np.array([[-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]]))
+    assert n == 3 and per[0] == 1 and per[1] == 2
+    corners = np.array([[x, y, z] for z in (-1.0, 1.0) for y in (-1.0, 1.0) for x in (-1.0, 1.0)])
+    n, per = oracle.octree_stats(corners)
+    assert n == 9 and per[1] == 8 and per[2:].sum() == 0
+    # x of the mean is 0.85/3 and hw = 1 + 0.85/3 + 1e-5: the level-1 cell of
+    # the pair spans x in [0.2833, 1.5667] and splits at 0.925, between them
+    P = np.array([[0.9, 0.9, 0.9], [0.95, 0.9, 0.9], [-1.0, -1.0, -1.0]])
+    n, per = oracle.octree_stats(P)
+    assert n == 5 and list(per[:3]) == [1, 2, 2] and per[3:].sum() == 0
+    # with (-1, 1, 1) added the x-range of the pair's cells is [-0.0475,
+    # 0.91001] at level 1, halved towards the pair: splits at 0.43125,
+    # 0.6706, 0.7903, 0.8502, 0.8801, 0.8951 keep 0.9 and 0.91 together, the
+    # level-7 split at 0.90256 parts them into two level-8 leaves
+    P = np.array([[0.9, 0.9, 0.9], [0.91, 0.9, 0.9], [-1.0, -1.0, -1.0], [-1.0, 1.0, 1.0]])
+    n, per = oracle.octree_stats(P)
+    assert list(per[:10]) == [1, 3, 1, 1, 1, 1, 1, 1, 2, 0] and n == 12
+
+
 # ---------------------------------------------------------------- Barnes-Hut acceptance
 # Eight points on the integer grid.  Their mean (-1.375, 4.75) is exact, so
 # the root box is exact up to the +1e-5 of quadtree.hpp:128-129: it is taller
