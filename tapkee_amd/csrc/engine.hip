@@ -19,6 +19,7 @@
 
 #include "affinity.hpp"
 #include "bh.hpp"
+#include "bh3.hpp"
 #include "common.hpp"
 #include "forces.hpp"
 #include "knn.hpp"
@@ -301,10 +302,12 @@ void validate(int N, int D, int no_dims, double perplexity, double theta)
                                         std::to_string(perplexity));
     if (!(theta >= 0.0)) fail(TKA_E_WRONG_PARAMETER, "Non-negativity check failed for SNE theta");
     if (D <= 0) fail(TKA_E_WRONG_PARAMETER, "feature dimension must be positive");
-    // Engine limits (documented deviations): the Barnes-Hut tree is 2-D, the
-    // exact path supports 1..3 dims, and K = (int)(3*perplexity) must be >= 1.
-    if (theta > 0.0 && no_dims != 2)
-        fail(TKA_E_WRONG_PARAMETER, "Barnes-Hut t-SNE (theta > 0) needs target dimension 2");
+    // Engine limits (documented deviations): Barnes-Hut runs on a quadtree
+    // (2-D, the reference's) or an octree (3-D; the reference's quadtree
+    // silently assumes 2-D, quadtree.hpp:71), the exact path supports 1..3
+    // dims, and K = (int)(3*perplexity) must be >= 1.
+    if (theta > 0.0 && no_dims != 2 && no_dims != 3)
+        fail(TKA_E_WRONG_PARAMETER, "Barnes-Hut t-SNE (theta > 0) needs target dimension 2 or 3");
     if (theta > 0.0 && N > kMaxTreePoints)
         fail(TKA_E_UNSUPPORTED, "Barnes-Hut t-SNE: at most 2^26 - 1 points (tree node counts are 26-bit)");
     if (no_dims > 3) fail(TKA_E_WRONG_PARAMETER, "target dimension above 3 is not supported");
@@ -409,7 +412,8 @@ void log_kl(tka_session& c, int iter, double C)
 template <class T>
 struct Session : tka_session {
     LoopState<T> L;
-    BhTree<T> tree;
+    BhTree<T> tree;    // 2-D
+    BhTree3<T> tree3;  // 3-D
     T* Ycur = nullptr;
     T* Ynext = nullptr;
     int nbox = 0, nz = 0;
@@ -502,7 +506,12 @@ struct Session : tka_session {
         if (exact) L.work.alloc(exact_work_elems(N, dims, L.dense), m);
         // tsne.hpp:75-80: uY = 0, gains = 1; Y0 is not centred before iteration 0
         bbox_partials_device<T>(L.Ya.get(), N, L.box.get(), s);
-        if (!exact) tree.alloc(N, m);
+        if (!exact) {
+            if (dims == 3)
+                tree3.alloc(N, m);
+            else
+                tree.alloc(N, m);
+        }
         nbox = nbox0;
         Ycur = L.Ya.get();
         Ynext = L.Yb.get();
@@ -526,6 +535,16 @@ struct Session : tka_session {
                 exact_forces_device<T>(dims, Ycur, N, dd_mode, L.dense ? L.val.get() : nullptr, exag, L.neg.get(),
                                        L.dense ? L.pos.get() : nullptr, L.zpart.get(), L.work.get(), s);
                 if (profile) timers[kTimExact].end(s);
+            } else if (!tree_ready && dims == 3) {
+                // octree: its own box partials, then the attraction alone (on
+                // the Y the build has centred)
+                if (profile) timers[kTimBuild].begin(s);
+                bh3_build<T>(Ycur, N, tree3, s);
+                if (profile) timers[kTimBuild].end(s);
+                attraction_alone();
+                if (profile) timers[kTimRepulsion].begin(s);
+                bh3_repulsion<T>(tree3, theta, L.neg.get(), L.zpart.get(), profile ? interactions.get() : nullptr, s);
+                if (profile) timers[kTimRepulsion].end(s);
             } else if (!tree_ready) {
                 if (iter > 0 && iter % kRenumberEvery == 0 && tree.sorted_perm != nullptr)
                     renumber(tree.sorted_perm);  // order of the previous iteration's tree
@@ -588,8 +607,13 @@ struct Session : tka_session {
                 // tsne.hpp:454-486: Barnes-Hut estimate of Z from a tree on the
                 // updated, zero-mean Y, then the sum over P's pattern.  The
                 // next iteration reuses this tree and these forces.
-                bh_build<T>(Ycur, N, L.box.get(), nbox, tree, s);
-                bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(), nullptr, s);
+                if (dims == 3) {
+                    bh3_build<T>(Ycur, N, tree3, s);
+                    bh3_repulsion<T>(tree3, theta, L.neg.get(), L.zpart.get(), nullptr, s);
+                } else {
+                    bh_build<T>(Ycur, N, L.box.get(), nbox, tree, s);
+                    bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(), nullptr, s);
+                }
                 tree_ready = true;
                 centred = true;
                 if (o.kl_mode == TKA_KL_EXACT_Z) {
@@ -621,11 +645,14 @@ struct Session : tka_session {
     {
         if (!exact && tree.N > 0 && bh_sort_failed(tree, s()))
             fail(TKA_E_INTERNAL, "radix sort look-back gave up; the tree of some iteration was wrong");
+        if (!exact && tree3.N > 0 && bh3_sort_failed(tree3, s()))
+            fail(TKA_E_INTERNAL, "radix sort look-back gave up; the tree of some iteration was wrong");
     }
 
     void inject_sort_fault() override
     {
         if (!exact && tree.N > 0) bh_inject_sort_fault(tree, s());
+        if (!exact && tree3.N > 0) bh3_inject_sort_fault(tree3, s());
     }
 
     void embedding(double* Y) override
@@ -797,9 +824,10 @@ tka_session* create_session(const SimilarityInput& in, int N, int no_dims, doubl
             DevBuf<double> dist((size_t)N * K, mb), val((size_t)N * K, mb);
             KnnStats ks;
             if (features) {
-                if (!exact) order.alloc(N, mb);
+                const bool want_order = !exact && no_dims == 2;  // the quadtree loop renumbers points
+                if (want_order) order.alloc(N, mb);
                 knn_device(X.get(), N, D, K, o.dist_dtype, o.candidates, idx.get(), dist.get(), &ks, mb, s,
-                           exact ? nullptr : order.get());
+                           want_order ? order.get() : nullptr);
                 ordered = ks.order_written;
             } else {  // a precomputed neighbour graph
                 upload(idx, in.idx, (size_t)N * K, s, mb);
@@ -897,6 +925,33 @@ void tsne_run_impl(const double* Xh, int N, int D, double* Yh, int no_dims, doub
     if (st) *st = sess->st;
 }
 
+// Barnes-Hut repulsion of y (N x dims, dims 2 or 3; centred in place by the
+// tree build) on a tree built for this call: neg and the Z partials; returns
+// the number of partials.  Fails if the tree's Morton sort gave up.
+template <class T>
+int bh_forces_once(int dims, T* y, int N, double theta, T* neg, DevBuf<double>& zpart, MemBudget* mb, hipStream_t s)
+{
+    require(dims == 2 || dims == 3, TKA_E_WRONG_PARAMETER, "Barnes-Hut needs 2 or 3 dimensions");
+    const int nz = bh_zblocks(N);
+    zpart.alloc(nz, mb);
+    if (dims == 3) {
+        BhTree3<T> tree;
+        tree.alloc(N, mb);
+        bh3_build<T>(y, N, tree, s);
+        bh3_repulsion<T>(tree, theta, neg, zpart.get(), nullptr, s);
+        require(!bh3_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
+    } else {
+        DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N), mb);
+        bbox_partials_device<T>(y, N, box.get(), s);
+        BhTree<T> tree;
+        tree.alloc(N, mb);
+        bh_build<T>(y, N, box.get(), bbox_blocks(N), tree, s);
+        bh_repulsion<T>(tree, y, theta, neg, zpart.get(), nullptr, s);
+        require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
+    }
+    return nz;
+}
+
 template <class T>
 void gradient_impl(const double* Yh, int N, int dims, const int* rowh, const int* colh, const double* valh,
                    double theta, double* dYh, double* sum_q)
@@ -919,16 +974,7 @@ void gradient_impl(const double* Yh, int N, int dims, const int* rowh, const int
     int nz;
     DevBuf<double> zpart;
     if (theta > 0.0) {
-        require(dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut gradient needs 2 dimensions");
-        nz = bh_zblocks(N);
-        zpart.alloc(nz);
-        DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
-        bbox_partials_device<T>(Y.get(), N, box.get(), s);
-        BhTree<T> tree;
-        tree.alloc(N, &mb);
-        bh_build<T>(Y.get(), N, box.get(), bbox_blocks(N), tree, s);
-        bh_repulsion<T>(tree, Y.get(), theta, neg.get(), zpart.get(), nullptr, s);
-        require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
+        nz = bh_forces_once<T>(dims, Y.get(), N, theta, neg.get(), zpart, &mb, s);
         gradient_combine_device<T>(dims, Y.get(), neg.get(), nullptr, row.get(), col.get(), val.get(), 1.0,
                                    zpart.get(), nz, N, dC.get(), s);
     } else {
@@ -1340,15 +1386,8 @@ TKA_API tka_status tka_kl(const double* Y, int32_t N, int32_t no_dims, const int
         upload(colv, col, (size_t)row_ptr[N], s);
         upload(vd, val, (size_t)row_ptr[N], s);
         if (kl_mode == TKA_KL_AS_REFERENCE && theta > 0.0) {
-            require(no_dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut KL needs 2 dimensions");
-            const int nz = bh_zblocks(N);
-            DevBuf<double> zpart(nz), neg(nd), box((size_t)kBoxStride * bbox_blocks(N));
-            bbox_partials_device<double>(Yd.get(), N, box.get(), s);
-            BhTree<double> tree;
-            tree.alloc(N, &mb);
-            bh_build<double>(Yd.get(), N, box.get(), bbox_blocks(N), tree, s);
-            bh_repulsion<double>(tree, Yd.get(), theta, neg.get(), zpart.get(), nullptr, s);
-            require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
+            DevBuf<double> zpart, neg(nd);
+            const int nz = bh_forces_once<double>(no_dims, Yd.get(), N, theta, neg.get(), zpart, &mb, s);
             const double Z = sum_device(zpart.get(), nz, &mb, s);
             *kl = kl_edges_reference_device<double>(no_dims, Yd.get(), row.get(), colv.get(), vd.get(), 1.0, Z, N, &mb, s);
         } else {
@@ -1501,19 +1540,10 @@ TKA_API tka_status tka_gradient_dev(const float* Y, int32_t N, int32_t no_dims, 
         int nz;
         DevBuf<double> zpart;
         if (theta > 0.0) {
-            require(no_dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut gradient needs 2 dimensions");
             // the tree build centres its input: work on a copy
             DevBuf<float> y(nd);
             TKA_HIP(hipMemcpyAsync(y.get(), Y, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
-            nz = bh_zblocks(N);
-            zpart.alloc(nz);
-            DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
-            bbox_partials_device<float>(y.get(), N, box.get(), s);
-            BhTree<float> tree;
-            tree.alloc(N, &mb);
-            bh_build<float>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
-            bh_repulsion<float>(tree, y.get(), theta, neg.get(), zpart.get(), nullptr, s);
-            require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
+            nz = bh_forces_once<float>(no_dims, y.get(), N, theta, neg.get(), zpart, &mb, s);
             gradient_combine_device<float>(no_dims, y.get(), neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(),
                                            nz, N, dY, s);
             const double z = sum_device(zpart.get(), nz, &mb, s);
@@ -1544,18 +1574,11 @@ TKA_API tka_status tka_kl_dev(const float* Y, int32_t N, int32_t no_dims, const 
         AllocStream as(s);  // temporaries ordered on the caller's stream, which may be non-blocking
         double Z;
         if (theta > 0.0) {
-            require(no_dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut KL needs 2 dimensions");
-            const size_t nd = (size_t)N * 2;
+            const size_t nd = (size_t)N * no_dims;
             DevBuf<float> y(nd), neg(nd);
             TKA_HIP(hipMemcpyAsync(y.get(), Y, nd * sizeof(float), hipMemcpyDeviceToDevice, s));
-            const int nz = bh_zblocks(N);
-            DevBuf<double> zpart(nz), box((size_t)kBoxStride * bbox_blocks(N));
-            bbox_partials_device<float>(y.get(), N, box.get(), s);
-            BhTree<float> tree;
-            tree.alloc(N, &mb);
-            bh_build<float>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
-            bh_repulsion<float>(tree, y.get(), theta, neg.get(), zpart.get(), nullptr, s);
-            require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
+            DevBuf<double> zpart;
+            const int nz = bh_forces_once<float>(no_dims, y.get(), N, theta, neg.get(), zpart, &mb, s);
             Z = sum_device(zpart.get(), nz, &mb, s);
         } else {
             Z = exact_z_device<float>(no_dims, Y, N, &mb, s);
@@ -1581,19 +1604,11 @@ void kl_grad_impl(const T* Y, int N, int dims, const int* row_ptr, const int* co
     DevBuf<double> zpart;
     double Z;
     if (theta > 0.0) {
-        require(dims == 2, TKA_E_WRONG_PARAMETER, "Barnes-Hut KL needs 2 dimensions");
         DevBuf<T> y(nd);  // the tree build centres its input
         TKA_HIP(hipMemcpyAsync(y.get(), Y, nd * sizeof(T), hipMemcpyDeviceToDevice, s));
-        const int nz = bh_zblocks(N);
-        zpart.alloc(nz);
-        DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
-        bbox_partials_device<T>(y.get(), N, box.get(), s);
-        BhTree<T> tree;
-        tree.alloc(N, &mb);
-        bh_build<T>(y.get(), N, box.get(), bbox_blocks(N), tree, s);
-        bh_repulsion<T>(tree, y.get(), theta, neg.get(), zpart.get(), nullptr, s);
-        require(!bh_sort_failed(tree, s), TKA_E_INTERNAL, "radix sort look-back gave up; the tree is not valid");
-        gradient_combine_device<T>(2, y.get(), neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(), nz, N, dY, s);
+        const int nz = bh_forces_once<T>(dims, y.get(), N, theta, neg.get(), zpart, &mb, s);
+        gradient_combine_device<T>(dims, y.get(), neg.get(), nullptr, row_ptr, col, val, 1.0, zpart.get(), nz, N, dY,
+                                   s);
         Z = sum_device(zpart.get(), nz, &mb, s);
     } else {
         const int nz = exact_zblocks(N);

@@ -93,9 +93,10 @@ int main(int argc, char** argv)
     EXPECT_THROW_T(with((method = tDistributedStochasticNeighborEmbedding, sne_perplexity = 10.0,
                          target_dimension = 100)).embedUsing(X),
                    wrong_parameter_error);
-    // documented deviation: Barnes-Hut needs 2 output dimensions
+    // documented deviation: Barnes-Hut runs on a quadtree or an octree, so
+    // it needs 2 or 3 output dimensions
     EXPECT_THROW_T(with((method = tDistributedStochasticNeighborEmbedding, sne_perplexity = 10.0,
-                         target_dimension = 3)).embedUsing(X),
+                         target_dimension = 4)).embedUsing(X),
                    wrong_parameter_error);
     // interface.cpp:181-196 cancelled_exception
     EXPECT_THROW_T(with((method = tDistributedStochasticNeighborEmbedding, sne_perplexity = 10.0,

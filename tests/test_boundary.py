@@ -54,7 +54,7 @@ def X100():
     (dict(sne_perplexity=10.0, sne_theta=-1.0), tapkee_amd.WrongParameterError),  # :21
     (dict(sne_perplexity=10.0, target_dimension=0), tapkee_amd.WrongParameterError),  # base.hpp:43
     (dict(sne_perplexity=10.0, target_dimension=100), tapkee_amd.WrongParameterError),
-    (dict(sne_perplexity=10.0, target_dimension=3), tapkee_amd.WrongParameterError),  # BH is 2-D
+    (dict(sne_perplexity=10.0, target_dimension=4), tapkee_amd.WrongParameterError),  # BH trees: 2-D or 3-D
     (dict(sne_perplexity=0.1, sne_theta=0.5), tapkee_amd.WrongParameterError),  # K = 0
 ])
 def test_embed_validation(kwargs, exc):
