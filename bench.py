@@ -20,6 +20,8 @@ HBM; `value` is iterations per second.  The same JSON line also reports:
                 mixture) with FP32, BF16 and FP8 e4m3 distance tiles
   exact_theta0  workload 1 (theta=0, N=2000): exact-loop iterations/s with the
                 dense and the kNN P, all-pairs force kernel time (and at N=10k)
+  octree_3d     the same swiss roll embedded in 3-D (octree Barnes-Hut):
+                loop iterations/s, tree build and traversal times
   wide_rows_d784  workload 2 (N=20,000, D=784): kNN + P with FP32 and BF16
                 distance tiles streamed in k-stages, MFMA rate over the kNN
                 stage, recall and max |dP| of BF16 against FP32
@@ -231,6 +233,30 @@ def wide_leg(tapkee_amd, datasets, perp=30.0):
     return out
 
 
+def octree_leg(tapkee_amd, X, perp=30.0, steps=200):
+    """Barnes-Hut t-SNE with a 3-D output (octree) on the headline data:
+    iterations/s of the loop after 50 warm-up iterations (early
+    exaggeration), and the average tree build and traversal times (HIP
+    events) over 50 profiled iterations."""
+    s = tapkee_amd.Session(X, no_dims=3, perplexity=perp, theta=0.5, seed=1, log_every=0)
+    s.step(50)
+    s.synchronize()
+    t = time.perf_counter()
+    s.step(steps)
+    s.synchronize()
+    dt = time.perf_counter() - t
+    s.profile(True)
+    s.step(50)
+    kt = s.kernel_timing()
+    s.close()
+    out = {"workload": f"bh-tsne swiss-roll N={X.shape[0]} D={X.shape[1]} perplexity={perp} theta=0.5 3-D",
+           "iters_per_s": round(steps / dt, 2), "ms_per_iter": round(dt * 1e3 / steps, 4),
+           "build_ms": round(kt["ms_build"] / max(kt["launches_build"], 1), 4),
+           "traverse_ms": round(kt["ms_repulsion"] / max(kt["launches_repulsion"], 1), 4)}
+    log(f"[bench] octree: {out}")
+    return out
+
+
 def exact_leg(tapkee_amd, datasets, perp=30.0, steps=500):
     """Workload 1 (theta=0, N=2000, D=50): iterations/s of the exact loop with
     the reference's dense P and with the kNN P (K=90), and the all-pairs
@@ -302,6 +328,7 @@ def main():
     ap.add_argument("--no-parametric", action="store_true", help="skip the parametric (MLP) t-SNE leg")
     ap.add_argument("--no-exact", action="store_true", help="skip the theta=0 (workload 1) leg")
     ap.add_argument("--no-wide", action="store_true", help="skip the D=784 kNN + P leg (workload 2)")
+    ap.add_argument("--no-octree", action="store_true", help="skip the 3-D (octree) Barnes-Hut leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run 3 plain device-to-device copies of MIB MiB (a known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -459,6 +486,9 @@ def main():
     wide = None
     if not a.no_wide:
         wide = wide_leg(tapkee_amd, datasets, perp)
+    octree = None
+    if not a.no_octree:
+        octree = octree_leg(tapkee_amd, X, perp)
     param = None
     if not a.no_parametric:
         param = parametric_leg(tapkee_amd, datasets, perp)
@@ -524,6 +554,7 @@ def main():
         "precision_sweep": sweep,
         "exact_theta0": exact,
         "wide_rows_d784": wide,
+        "octree_3d": octree,
         "parametric": param,
     }
     if cpu:
