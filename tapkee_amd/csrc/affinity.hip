@@ -86,6 +86,74 @@ __global__ __launch_bounds__(256) void perplexity_knn(const double* __restrict__
     }
 }
 
+// Rows of K <= 16 R values, four per wave: 16 lanes per row, R values per
+// lane.  Each evaluation's sums are 16-lane butterflies, so every lane of a
+// row holds the same bits and steps the bisection identically; a row whose
+// bisection has ended keeps its values while the wave's other rows go on.
+// Four rows share each exp instruction sequence and each reduction step,
+// where one row per wave left a 90-value row's second sequence on 26 lanes
+// and paid a 64-lane reduction per row.
+template <int R>
+__global__ __launch_bounds__(256) void perplexity_knn_packed(const double* __restrict__ dist, int N, int K,
+                                                             double logp, double* __restrict__ val,
+                                                             double* __restrict__ beta_out,
+                                                             int* __restrict__ iters_out, int* __restrict__ unconv)
+{
+    const int lane = threadIdx.x & 63, sub = lane & 15;
+    const int row = blockIdx.x * 16 + (threadIdx.x >> 4);
+    const bool live = row < N;
+    double d[R], p[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int m = r * 16 + sub;
+        d[r] = (live && m < K) ? dist[(size_t)row * K + m] : 0.0;
+        p[r] = 0.0;
+    }
+    Bisect b;
+    bool found = !live;
+    int iter = 0;
+    double sum_P = DBL_MIN, last_beta = b.beta;
+    while (__ballot(!found && iter < kBisectMaxIter) != 0ull) {
+        const bool go = !found && iter < kBisectMaxIter;
+        double s = 0.0, h = 0.0;
+        const double beta = b.beta;
+        if (go) {
+            last_beta = beta;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int m = r * 16 + sub;
+                if (m < K) {
+                    p[r] = exp(-beta * d[r]);
+                    s += p[r];
+                    h += beta * (d[r] * p[r]);
+                }
+            }
+        }
+#pragma unroll
+        for (int o = 8; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o, 16);
+            h += __shfl_xor(h, o, 16);
+        }
+        if (go) {
+            sum_P = DBL_MIN + s;
+            const double H = (h / sum_P) + log(sum_P);
+            found = b.step(H, logp);
+            iter++;
+        }
+    }
+    if (!live) return;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int m = r * 16 + sub;
+        if (m < K) val[(size_t)row * K + m] = p[r] / sum_P;
+    }
+    if (sub == 0) {
+        if (beta_out) beta_out[row] = last_beta;
+        if (iters_out) iters_out[row] = iter;
+        if (!found) atomicAdd(unconv, 1);
+    }
+}
+
 // DD[n][m] for the dense theta = 0 path: textbook sum of squared differences
 // (the order of the reference's euclidean_distance), or the reference's
 // -2<x_n, x_m> (tsne.hpp:948-950) in compat mode.
@@ -235,8 +303,20 @@ int perplexity_knn_device(const double* dist, int N, int K, double perplexity, d
     require(K <= 512, TKA_E_WRONG_PARAMETER, "perplexity calibration supports K <= 512");
     DevBuf<int> unconv(1, mb);
     TKA_HIP(hipMemsetAsync(unconv.get(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(perplexity_knn, dim3(ceil_div(N, 4)), dim3(256), 0, s, dist, N, K, std::log(perplexity), val,
-                       beta, iters, unconv.get());
+    const double logp = std::log(perplexity);
+    if (K <= 128) {
+        const dim3 g(ceil_div(N, 16));
+#define TKA_PK(R) \
+    case R: hipLaunchKernelGGL(perplexity_knn_packed<R>, g, dim3(256), 0, s, dist, N, K, logp, val, beta, iters, \
+                               unconv.get()); break
+        switch ((K + 15) / 16) {
+            TKA_PK(1); TKA_PK(2); TKA_PK(3); TKA_PK(4); TKA_PK(5); TKA_PK(6); TKA_PK(7); TKA_PK(8);
+        }
+#undef TKA_PK
+    } else {
+        hipLaunchKernelGGL(perplexity_knn, dim3(ceil_div(N, 4)), dim3(256), 0, s, dist, N, K, logp, val, beta, iters,
+                           unconv.get());
+    }
     TKA_LAUNCH_CHECK();
     int h = 0;
     TKA_HIP(hipMemcpyAsync(&h, unconv.get(), sizeof(int), hipMemcpyDeviceToHost, s));
