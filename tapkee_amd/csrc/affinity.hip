@@ -154,6 +154,71 @@ __global__ __launch_bounds__(256) void perplexity_knn_packed(const double* __res
     }
 }
 
+// Rows of K > 512 values (up to 32 per thread, K <= 8192): one workgroup per
+// row.  Each sum is a 64-lane butterfly per wave, then the four wave sums
+// added in wave order by every thread, so all threads hold the same bits
+// and step the bisection identically.
+__global__ __launch_bounds__(256) void perplexity_knn_block(const double* __restrict__ dist, int N, int K, double logp,
+                                                            double* __restrict__ val, double* __restrict__ beta_out,
+                                                            int* __restrict__ iters_out, int* __restrict__ unconv)
+{
+    constexpr int R = 32;
+    __shared__ double shs[4], shh[4];
+    const int row = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double d[R], p[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int m = r * 256 + threadIdx.x;
+        d[r] = m < K ? dist[(size_t)row * K + m] : 0.0;
+        p[r] = 0.0;
+    }
+    Bisect b;
+    bool found = false;
+    int iter = 0;
+    double sum_P = DBL_MIN, last_beta = b.beta;
+    while (!found && iter < kBisectMaxIter) {
+        const double beta = b.beta;
+        last_beta = beta;
+        double s = 0.0, h = 0.0;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int m = r * 256 + threadIdx.x;
+            if (m < K) {
+                p[r] = exp(-beta * d[r]);
+                s += p[r];
+                h += beta * (d[r] * p[r]);
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o, 64);
+            h += __shfl_xor(h, o, 64);
+        }
+        __syncthreads();  // the previous evaluation's sums have been read
+        if (lane == 0) {
+            shs[w] = s;
+            shh[w] = h;
+        }
+        __syncthreads();
+        s = ((shs[0] + shs[1]) + shs[2]) + shs[3];
+        h = ((shh[0] + shh[1]) + shh[2]) + shh[3];
+        sum_P = DBL_MIN + s;
+        const double H = (h / sum_P) + log(sum_P);
+        found = b.step(H, logp);
+        iter++;
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int m = r * 256 + threadIdx.x;
+        if (m < K) val[(size_t)row * K + m] = p[r] / sum_P;
+    }
+    if (threadIdx.x == 0) {
+        if (beta_out) beta_out[row] = last_beta;
+        if (iters_out) iters_out[row] = iter;
+        if (!found) atomicAdd(unconv, 1);
+    }
+}
+
 // DD[n][m] for the dense theta = 0 path: textbook sum of squared differences
 // (the order of the reference's euclidean_distance), or the reference's
 // -2<x_n, x_m> (tsne.hpp:948-950) in compat mode.
@@ -300,7 +365,7 @@ __global__ __launch_bounds__(256) void sym_write(const uint64_t* __restrict__ ke
 int perplexity_knn_device(const double* dist, int N, int K, double perplexity, double* val, double* beta, int* iters,
                           MemBudget* mb, hipStream_t s)
 {
-    require(K <= 512, TKA_E_WRONG_PARAMETER, "perplexity calibration supports K <= 512");
+    require(K <= 8192, TKA_E_WRONG_PARAMETER, "perplexity calibration supports K <= 8192");
     DevBuf<int> unconv(1, mb);
     TKA_HIP(hipMemsetAsync(unconv.get(), 0, sizeof(int), s));
     const double logp = std::log(perplexity);
@@ -313,8 +378,11 @@ int perplexity_knn_device(const double* dist, int N, int K, double perplexity, d
             TKA_PK(1); TKA_PK(2); TKA_PK(3); TKA_PK(4); TKA_PK(5); TKA_PK(6); TKA_PK(7); TKA_PK(8);
         }
 #undef TKA_PK
-    } else {
+    } else if (K <= 512) {
         hipLaunchKernelGGL(perplexity_knn, dim3(ceil_div(N, 4)), dim3(256), 0, s, dist, N, K, logp, val, beta, iters,
+                           unconv.get());
+    } else {
+        hipLaunchKernelGGL(perplexity_knn_block, dim3(N), dim3(256), 0, s, dist, N, K, logp, val, beta, iters,
                            unconv.get());
     }
     TKA_LAUNCH_CHECK();

@@ -21,6 +21,7 @@
 #include "knn.hpp"
 #include "radix_sort.hpp"
 
+#include <climits>
 #include <cmath>
 #include <type_traits>
 #include <vector>
@@ -1143,6 +1144,196 @@ TKA_DEV void sort_prefix(uint64_t (&d)[8], uint32_t (&ix)[8], int R)
         wave_sort_pairs<8>(d, ix);
 }
 
+// ---------------------------------------------------------------- large K
+// K beyond the candidate buffers (kMaxKprime) up to kLargeKMax: every
+// distance in FP64 with the re-rank's arithmetic, then an exact selection
+// per row, so no certificate is needed.
+constexpr int kLargeKMax = 8192;
+constexpr int kPairTile = 64;   // query and candidate rows per workgroup of pair_sqdist_rows
+constexpr int kPairDims = 32;   // dimensions per LDS stage
+
+// out[(q - q0) N + c] = squared distance of query row q in [q0, q0 + B) to
+// row c, summed in dimension order without contraction (exact_sqdist's
+// arithmetic); the row itself is +inf.  64 x 64 pairs per workgroup, 4 x 4
+// per thread, coordinates staged through LDS 32 dimensions at a time.
+__global__ __launch_bounds__(256) void pair_sqdist_rows(const double* __restrict__ X, int N, int D, int q0, int B,
+                                                        double* __restrict__ out)
+{
+    __shared__ double sq[kPairTile][kPairDims + 1], sc[kPairTile][kPairDims + 1];
+    const int qb = blockIdx.y * kPairTile, cb = blockIdx.x * kPairTile;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = 0.0;
+    for (int d0 = 0; d0 < D; d0 += kPairDims) {
+        const int dn = min(kPairDims, D - d0);
+        for (int e = threadIdx.x; e < kPairTile * kPairDims; e += 256) {
+            const int r = e / kPairDims, k = e % kPairDims;
+            const bool kin = k < dn;
+            sq[r][k] = (kin && qb + r < B) ? X[(size_t)(q0 + qb + r) * D + d0 + k] : 0.0;
+            sc[r][k] = (kin && cb + r < N) ? X[(size_t)(cb + r) * D + d0 + k] : 0.0;
+        }
+        __syncthreads();
+        for (int k = 0; k < dn; k++) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                a[i] = sq[4 * ty + i][k];
+                b[i] = sc[4 * tx + i][k];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const double t = dadd_sep(a[i], -b[j]);
+                    acc[i][j] = dadd_sep(acc[i][j], dmul_sep(t, t));
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int q = qb + 4 * ty + i;
+        if (q >= B) continue;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int c = cb + 4 * tx + j;
+            if (c < N) out[(size_t)q * N + c] = (q0 + q == c) ? HUGE_VAL : acc[i][j];
+        }
+    }
+}
+
+// Block-wide exclusive prefix of one count per thread (256 threads), in
+// thread order; the block total in *total.
+TKA_DEV int block_excl_scan(int v, int* sh, int* total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(x, o, 64);
+        if (lane >= o) x += t;
+    }
+    __syncthreads();
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; k++) base += sh[k];
+    *total = sh[0] + sh[1] + sh[2] + sh[3];
+    return base + x - v;
+}
+
+// The K smallest (distance, index) of each of B rows of `dist` (N values,
+// non-negative or +inf), sorted: a radix select of the K-th distance's bits
+// (six rounds of 11, 11, 11, 11, 11 and 9 bits), the rows below it and the
+// equal ones of smallest index gathered into LDS, then a bitonic sort of
+// (bits, index).  One workgroup per row; K <= kLargeKMax.
+__global__ __launch_bounds__(256) void select_k_rows(const double* __restrict__ dist, int N, int B, int K, int q0,
+                                                     int* __restrict__ idx, double* __restrict__ dout)
+{
+    __shared__ uint64_t skey[kLargeKMax];
+    __shared__ int sidx[kLargeKMax];
+    __shared__ int hist[2048];
+    __shared__ int sh[4];
+    __shared__ int s_cnt, s_digit, s_below;
+    const int r = blockIdx.x;
+    if (r >= B) return;
+    const uint64_t* row = reinterpret_cast<const uint64_t*>(dist + (size_t)r * N);
+    uint64_t prefix = 0ull, pmask = 0ull;
+    int k = K;  // rank (1-based) of the wanted key among the keys that match the prefix
+    for (int round = 0; round < 6; round++) {
+        const int w = round < 5 ? 11 : 9;
+        const int sft = 64 - 11 * (round + 1) < 0 ? 0 : 64 - 11 * (round + 1);
+        const int nb = 1 << w;
+        for (int b = threadIdx.x; b < nb; b += 256) hist[b] = 0;
+        __syncthreads();
+        for (int c = threadIdx.x; c < N; c += 256) {
+            const uint64_t key = row[c];
+            if ((key & pmask) == prefix) atomicAdd(&hist[(int)((key >> sft) & (uint64_t)(nb - 1))], 1);
+        }
+        __syncthreads();
+        // the bin holding the k-th key: per-thread partial sums of 8 bins, a block scan, then the thread whose range
+        // crosses k
+        const int per = nb / 256;
+        int local = 0;
+        for (int b = 0; b < per; b++) local += hist[threadIdx.x * per + b];
+        int tot;
+        const int before = block_excl_scan(local, sh, &tot);
+        if (before < k && k <= before + local) {
+            int acc = before;
+            for (int b = 0; b < per; b++) {
+                const int h = hist[threadIdx.x * per + b];
+                if (acc + h >= k) {
+                    s_digit = threadIdx.x * per + b;
+                    s_below = acc;
+                    break;
+                }
+                acc += h;
+            }
+        }
+        __syncthreads();
+        prefix |= (uint64_t)s_digit << sft;
+        pmask |= (uint64_t)(nb - 1) << sft;
+        k -= s_below;
+        __syncthreads();
+    }
+    // prefix is the K-th key; K - k keys lie below it; take the k equal keys of smallest index
+    const uint64_t T = prefix;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    int eq_taken = 0;
+    for (int c0 = 0; c0 < N; c0 += 256) {
+        const int c = c0 + threadIdx.x;
+        const uint64_t key = c < N ? row[c] : ~0ull;
+        if (key < T) {
+            const int pos = atomicAdd(&s_cnt, 1);
+            skey[pos] = key;
+            sidx[pos] = c;
+        }
+        const int eq = (key == T) ? 1 : 0;
+        int tot;
+        const int rank = block_excl_scan(eq, sh, &tot);
+        if (eq && eq_taken + rank < k) {
+            skey[K - k + eq_taken + rank] = key;
+            sidx[K - k + eq_taken + rank] = c;
+        }
+        eq_taken += tot;
+    }
+    __syncthreads();
+    // bitonic sort of (key, index) over the next power of two >= K, padded with +inf
+    int P2 = 1;
+    while (P2 < K) P2 <<= 1;
+    for (int i = K + threadIdx.x; i < P2; i += 256) {
+        skey[i] = ~0ull;
+        sidx[i] = INT_MAX;
+    }
+    __syncthreads();
+    for (int size = 2; size <= P2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < P2 / 2; t += 256) {
+                const int i = 2 * stride * (t / stride) + (t % stride), j = i + stride;
+                const bool up = (i & size) == 0;
+                const uint64_t ki = skey[i], kj = skey[j];
+                const int ii = sidx[i], ij = sidx[j];
+                const bool gt = ki > kj || (ki == kj && ii > ij);
+                if (gt == up) {
+                    skey[i] = kj;
+                    skey[j] = ki;
+                    sidx[i] = ij;
+                    sidx[j] = ii;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int j = threadIdx.x; j < K; j += 256) {
+        idx[(size_t)(q0 + r) * K + j] = sidx[j];
+        dout[(size_t)(q0 + r) * K + j] = __longlong_as_double((long long)skey[j]);
+    }
+}
+
 // One wave per row.  The kept candidates are sorted by their approximate
 // values v_j (|v_j - e_j| <= eps, e_j = ||c_j||^2 - 2 q.c_j); with v_(K) the
 // K-th smallest, any candidate with v_j > v_(K) + 2 eps has e_j > e_K and
@@ -1493,15 +1684,38 @@ void run_tiles_wide(const void* A, const void* B, int NV, int N, int kprime, con
 
 }  // namespace
 
+// K beyond the candidate buffers: every distance in FP64, selected exactly
+// per row, query rows in batches of at most 512 MB of distances.
+static void knn_large_k(const double* Xn, int N, int D, int K, int* idx, double* dist, MemBudget* mb, hipStream_t s)
+{
+    require(K <= kLargeKMax, TKA_E_WRONG_PARAMETER,
+            "K = " + std::to_string(K) + " exceeds the engine's limit of " + std::to_string(kLargeKMax) +
+                " neighbours (perplexity < " + std::to_string(kLargeKMax / 3 + 1) + ")");
+    const size_t budget = (size_t)512 << 20;
+    int B = (int)std::min<size_t>((size_t)N, std::max<size_t>(kPairTile, budget / ((size_t)N * sizeof(double))));
+    if (B < N) B = B / kPairTile * kPairTile;
+    DevBuf<double> buf((size_t)B * N, mb);
+    for (int q0 = 0; q0 < N; q0 += B) {
+        const int b = std::min(B, N - q0);
+        hipLaunchKernelGGL(pair_sqdist_rows, dim3(ceil_div(N, kPairTile), ceil_div(b, kPairTile)), dim3(256), 0, s,
+                           Xn, N, D, q0, b, buf.get());
+        TKA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(select_k_rows, dim3(b), dim3(256), 0, s, (const double*)buf.get(), N, b, K, q0, idx, dist);
+        TKA_LAUNCH_CHECK();
+    }
+    TKA_HIP(hipStreamSynchronize(s));  // buf is released on return
+}
+
 void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, int* idx, double* dist, KnnStats* st,
                 MemBudget* mb, hipStream_t s, uint32_t* order_out)
 {
     require(N >= 2 && K >= 1 && K <= N - 1, TKA_E_WRONG_PARAMETER, "kNN needs 1 <= K <= N-1");
-    require(K <= kMaxKprime, TKA_E_WRONG_PARAMETER,
-            "K = " + std::to_string(K) + " exceeds the engine's limit of " + std::to_string(kMaxKprime) +
-                " neighbours (perplexity <= 128)");
     require(dtype == TKA_DIST_FP32 || dtype == TKA_DIST_BF16 || dtype == TKA_DIST_FP8, TKA_E_UNSUPPORTED,
             "distance dtype " + std::to_string(dtype) + " is not available");
+    if (K > kMaxKprime) {  // all distances in FP64 (whatever the tile dtype): exact without a certificate
+        knn_large_k(Xn, N, D, K, idx, dist, mb, s);
+        return;
+    }
     if (kprime <= 0) kprime = 2 * K;
     kprime = std::max(kprime, K);
     kprime = std::min(kprime, kMaxKprime);

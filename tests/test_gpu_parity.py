@@ -67,6 +67,31 @@ def test_knn_wide_rows_many_blocks(engine, oracle, dtype):
     assert np.all(np.diff(dist, axis=1) >= 0)
 
 
+@pytest.mark.parametrize("dtype", ["fp32", "bf16"])
+@pytest.mark.parametrize("n,d,K", [(900, 20, 450), (1300, 7, 1299), (600, 130, 385)])
+def test_knn_large_k_bit_exact(engine, oracle, dtype, n, d, K):
+    # K beyond the candidate buffers (384): every distance in FP64 and an
+    # exact selection per row, whatever the tile dtype; K = N - 1 takes
+    # every other row
+    X = oracle.normalize(mixture(n, d))
+    idx, dist, fails = engine.knn(X, K, dist_dtype=dtype)
+    ridx, rdist = oracle.knn_brute(X, K)
+    np.testing.assert_array_equal(idx, ridx)
+    np.testing.assert_array_equal(dist, rdist)
+    assert fails == 0
+
+
+def test_knn_large_k_ties(engine, oracle):
+    # a lattice with duplicated points: many equal distances around the K-th
+    g = np.stack(np.meshgrid(np.arange(20.0), np.arange(20.0)), -1).reshape(-1, 2)
+    X = np.concatenate([g, g[:100]])
+    for K in (400, 499):
+        idx, dist, _ = engine.knn(X, K, dist_dtype="fp32")
+        ridx, rdist = oracle.knn_brute(X, K)
+        np.testing.assert_array_equal(idx, ridx)
+        np.testing.assert_array_equal(dist, rdist)
+
+
 def test_knn_fp8_is_approximate_but_close(engine, oracle):
     # FP8 e4m3 tiles keep 4 significant bits; the FP64 re-rank of 4K
     # candidates recovers most true neighbours, and reported distances are
@@ -93,11 +118,13 @@ def test_knn_ties_broken_by_index(engine, oracle):
 
 
 # ---------------------------------------------------------------- perplexity
-def test_perplexity_rows(engine, oracle):
-    X = oracle.normalize(mixture(800, 20))
-    _, dist = oracle.knn_brute(X, 90)
-    val, beta, iters, unconv = engine.perplexity(dist, 30.0)
-    rval, rbeta, riters, runconv = oracle.perplexity_sparse(dist, 30.0)
+# K = 90: four rows per wave; 300: one wave per row; 900: one workgroup per row
+@pytest.mark.parametrize("perp", [30.0, 100.0, 300.0])
+def test_perplexity_rows(engine, oracle, perp):
+    X = oracle.normalize(mixture(1000, 20))
+    _, dist = oracle.knn_brute(X, int(3 * perp))
+    val, beta, iters, unconv = engine.perplexity(dist, perp)
+    rval, rbeta, riters, runconv = oracle.perplexity_sparse(dist, perp)
     # same bisection path: identical iteration counts and betas (dyadic updates)
     np.testing.assert_array_equal(iters, riters)
     np.testing.assert_array_equal(beta, rbeta)
@@ -159,6 +186,28 @@ def test_build_p_matches_oracle(engine, oracle, dtype):
         np.testing.assert_allclose(val[a], rval[a][o], rtol=1e-13, atol=0)
 
 
+def test_build_p_large_perplexity_matches_oracle(engine, oracle):
+    """Perplexity 150 (K = 450, beyond the tiles' candidate buffers): the
+    exact kNN path, the workgroup-per-row bisection and the symmetrisation
+    of 2NK = 630k edges; identical pattern, values to 1e-13."""
+    X = mixture(700, 12)
+    row, col, val = engine.build_p(X, 150.0, dist_dtype="bf16")
+    rrow, rcol, rval, _ = oracle.build_p(X, 150.0, knn_mode=1)
+    np.testing.assert_array_equal(row, rrow)
+    for i in range(len(row) - 1):
+        a = slice(row[i], row[i + 1])
+        o = np.argsort(rcol[a], kind="stable")
+        np.testing.assert_array_equal(col[a], rcol[a][o])
+        np.testing.assert_allclose(val[a], rval[a][o], rtol=1e-13, atol=0)
+
+
+def test_session_large_perplexity_runs(engine):
+    X = mixture(1500, 10, k=4)
+    Y, st = engine.tsne(X, perplexity=200.0, theta=0.5, seed=2, max_iter=300)
+    assert np.all(np.isfinite(Y)) and st["knn_k"] == 600
+    assert st["kl_final_exact"] > 0.0
+
+
 # ---------------------------------------------------------------- gradient
 @pytest.fixture(scope="module")
 def small_problem(oracle):
@@ -210,6 +259,44 @@ def test_gradient_bh_theta0_is_exact(engine, oracle, small_problem):
     rdY, rz = oracle.gradient_exact_sparse(row, col, val, Y)
     assert rel_l2(dY_tree, rdY) < 1e-12
     assert abs(z_tree - rz) / rz < 1e-12
+
+
+@pytest.mark.parametrize("dims", [2, 3])
+def test_duplicate_embedding_points(engine, oracle, small_problem, dims):
+    """Exact duplicates in Y (documented deviation): every point is its own
+    leaf and skips only itself, so each duplicate sees the others at D = 0
+    (q = 1), as the exact gradient has it -- the limit of distinct points
+    that coincide.  The reference instead stores one point per leaf and only
+    counts the others (quadtree.hpp:218-234), and loses that count when the
+    leaf later splits.  Checked: through the tree kernel with theta -> 0 the
+    gradient equals the exact one; at theta = 0.5 the forces and Z equal the
+    oracle's for the same points with each duplicate moved by eps (distinct
+    points, which the reference's tree handles), to O(eps)."""
+    row, col, val, Y = small_problem
+    Y = rng(17).normal(size=(len(Y), dims)) * 3.0
+    Y[1::7] = Y[0::7][: len(Y[1::7])]  # every 7th point duplicated into the next
+    Y[5] = Y[0]                         # a triple
+    Y = Y - Y.mean(0)
+    g0, z0 = engine.gradient(Y, row, col, val, 1e-300, precision="fp64")
+    r0, rz0 = oracle.gradient_exact_sparse(row, col, val, Y)
+    assert rel_l2(g0, r0) < 1e-12
+    assert abs(z0 - rz0) / rz0 < 1e-12
+    # displaced copies: separate Morton keys (above 2^-32 of the box in 2-D,
+    # 2^-21 in 3-D), so the oracle's tree separates them too
+    eps = 1e-6 if dims == 2 else 1e-4
+    Yd = Y.copy()
+    seen = {}
+    for i in range(len(Y)):
+        key = Y[i].tobytes()
+        k = seen.get(key, 0)
+        Yd[i] += eps * k * np.arange(1, dims + 1)
+        seen[key] = k + 1
+    assert sum(v > 1 for v in seen.values()) > 50
+    g, z = engine.gradient(Y, row, col, val, 0.5, precision="fp64")
+    r, rz = oracle.gradient_bh(row, col, val, Yd, 0.5)
+    tol = 1e-5 if dims == 2 else 1e-3
+    assert rel_l2(g, r) < tol
+    assert abs(z - rz) / rz < tol
 
 
 @pytest.mark.parametrize("compat", [False, True])
