@@ -2,22 +2,26 @@
 // search, vptree.hpp:126-293, used at tsne.hpp:628-646).
 //
 // Pipeline per call:
-//   1. prep:   low-precision copies of the normalised rows (FP32, or BF16
-//              hi/lo splits laid out for MFMA) plus FP64 row norms.
+//   1. prep:   rows put in a locality order (Morton order of a projection on
+//              their estimated top-3 principal subspace), then low-precision
+//              operands (FP32, BF16 hi/lo splits or FP8) laid out for MFMA,
+//              plus FP64 row norms.
 //   2. tiles:  each wave owns 32 query rows; the block streams all candidate
 //              rows through LDS in chunks of 128 and computes 32x32 distance
-//              tiles (MFMA v_mfma_f32_32x32x16_bf16, or FP32 FMAs laid out the
-//              same way).  Values below the row's running threshold are
-//              appended to a per-row buffer of kCap entries in HBM; when a
-//              buffer could overflow the wave sorts it (register bitonic) and
-//              keeps the k' smallest, lowering the threshold.
+//              tiles on the matrix cores (register-resident query operands,
+//              or both operands streamed in k-stages for wide rows).  Values
+//              below the row's running threshold are appended to a per-row
+//              buffer of kCap entries in HBM; when a buffer could overflow the
+//              wave keeps the k' smallest (a selection by ballot bisection),
+//              lowering the threshold.
 //   3. rerank: one wave per row recomputes the kept candidates' squared
 //              distances in FP64 (same summation as the reference's
 //              euclidean_distance, vptree.hpp:95-101), sorts by
 //              (distance, index) and checks an a-priori error bound: the K-th
 //              exact distance must lie below (threshold - bound), which proves
 //              no dropped candidate can belong to the top K.
-//   4. rows failing the bound are re-run with FP32 tiles and k' = 384.
+//   4. rows failing the bound get every distance in FP64 and an exact
+//      selection (knn_exact_rows), as do all rows when K exceeds the buffers.
 #include "knn.hpp"
 #include "radix_sort.hpp"
 
@@ -1152,11 +1156,13 @@ constexpr int kLargeKMax = 8192;
 constexpr int kPairTile = 64;   // query and candidate rows per workgroup of pair_sqdist_rows
 constexpr int kPairDims = 32;   // dimensions per LDS stage
 
-// out[(q - q0) N + c] = squared distance of query row q in [q0, q0 + B) to
-// row c, summed in dimension order without contraction (exact_sqdist's
-// arithmetic); the row itself is +inf.  64 x 64 pairs per workgroup, 4 x 4
-// per thread, coordinates staged through LDS 32 dimensions at a time.
-__global__ __launch_bounds__(256) void pair_sqdist_rows(const double* __restrict__ X, int N, int D, int q0, int B,
+// out[r N + c] = squared distance of query row q(r) to row c, for the B
+// query rows q(r) = qlist[q0 + r] (or q0 + r without a list), summed in
+// dimension order without contraction (exact_sqdist's arithmetic); the row
+// itself is +inf.  64 x 64 pairs per workgroup, 4 x 4 per thread,
+// coordinates staged through LDS 32 dimensions at a time.
+__global__ __launch_bounds__(256) void pair_sqdist_rows(const double* __restrict__ X, int N, int D,
+                                                        const int* __restrict__ qlist, int q0, int B,
                                                         double* __restrict__ out)
 {
     __shared__ double sq[kPairTile][kPairDims + 1], sc[kPairTile][kPairDims + 1];
@@ -1172,7 +1178,8 @@ __global__ __launch_bounds__(256) void pair_sqdist_rows(const double* __restrict
         for (int e = threadIdx.x; e < kPairTile * kPairDims; e += 256) {
             const int r = e / kPairDims, k = e % kPairDims;
             const bool kin = k < dn;
-            sq[r][k] = (kin && qb + r < B) ? X[(size_t)(q0 + qb + r) * D + d0 + k] : 0.0;
+            const int qr = qb + r < B ? (qlist ? qlist[q0 + qb + r] : q0 + qb + r) : 0;
+            sq[r][k] = (kin && qb + r < B) ? X[(size_t)qr * D + d0 + k] : 0.0;
             sc[r][k] = (kin && cb + r < N) ? X[(size_t)(cb + r) * D + d0 + k] : 0.0;
         }
         __syncthreads();
@@ -1197,10 +1204,11 @@ __global__ __launch_bounds__(256) void pair_sqdist_rows(const double* __restrict
     for (int i = 0; i < 4; i++) {
         const int q = qb + 4 * ty + i;
         if (q >= B) continue;
+        const int qr = qlist ? qlist[q0 + q] : q0 + q;
 #pragma unroll
         for (int j = 0; j < 4; j++) {
             const int c = cb + 4 * tx + j;
-            if (c < N) out[(size_t)q * N + c] = (q0 + q == c) ? HUGE_VAL : acc[i][j];
+            if (c < N) out[(size_t)q * N + c] = (qr == c) ? HUGE_VAL : acc[i][j];
         }
     }
 }
@@ -1229,9 +1237,11 @@ TKA_DEV int block_excl_scan(int v, int* sh, int* total)
 // non-negative or +inf), sorted: a radix select of the K-th distance's bits
 // (six rounds of 11, 11, 11, 11, 11 and 9 bits), the rows below it and the
 // equal ones of smallest index gathered into LDS, then a bitonic sort of
-// (bits, index).  One workgroup per row; K <= kLargeKMax.
-__global__ __launch_bounds__(256) void select_k_rows(const double* __restrict__ dist, int N, int B, int K, int q0,
-                                                     int* __restrict__ idx, double* __restrict__ dout)
+// (bits, index).  One workgroup per row; K <= kLargeKMax.  Ties are broken
+// by row index (X in the caller's order).
+__global__ __launch_bounds__(256) void select_k_rows(const double* __restrict__ dist, int N, int B, int K,
+                                                     const int* __restrict__ qlist, int q0, int* __restrict__ idx,
+                                                     double* __restrict__ dout)
 {
     __shared__ uint64_t skey[kLargeKMax];
     __shared__ int sidx[kLargeKMax];
@@ -1328,9 +1338,10 @@ __global__ __launch_bounds__(256) void select_k_rows(const double* __restrict__ 
             __syncthreads();
         }
     }
+    const int qr = qlist ? qlist[q0 + r] : q0 + r;
     for (int j = threadIdx.x; j < K; j += 256) {
-        idx[(size_t)(q0 + r) * K + j] = sidx[j];
-        dout[(size_t)(q0 + r) * K + j] = __longlong_as_double((long long)skey[j]);
+        idx[(size_t)qr * K + j] = sidx[j];
+        dout[(size_t)qr * K + j] = __longlong_as_double((long long)skey[j]);
     }
 }
 
@@ -1427,10 +1438,11 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     }
 }
 
-__global__ void collect_failures(const int* __restrict__ fail, int n, int* __restrict__ list, int* __restrict__ count)
+__global__ void collect_failures(const int* __restrict__ fail, int n, const uint32_t* __restrict__ perm,
+                                 int* __restrict__ list, int* __restrict__ count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && fail[i]) list[atomicAdd(count, 1)] = i;
+    if (i < n && fail[i]) list[atomicAdd(count, 1)] = perm ? (int)perm[i] : i;
 }
 
 // ---------------------------------------------------------------- locality order
@@ -1684,23 +1696,27 @@ void run_tiles_wide(const void* A, const void* B, int NV, int N, int kprime, con
 
 }  // namespace
 
-// K beyond the candidate buffers: every distance in FP64, selected exactly
-// per row, query rows in batches of at most 512 MB of distances.
-static void knn_large_k(const double* Xn, int N, int D, int K, int* idx, double* dist, MemBudget* mb, hipStream_t s)
+// Every distance in FP64, selected exactly per row (X in the caller's
+// order, so ties go by the caller's index): for K beyond the candidate
+// buffers (all rows), and to re-solve the rows whose certificate failed
+// (qlist, nq rows).  Query rows in batches of at most 512 MB of distances.
+static void knn_exact_rows(const double* Xn, int N, int D, int K, const int* qlist, int nq, int* idx, double* dist,
+                           MemBudget* mb, hipStream_t s)
 {
     require(K <= kLargeKMax, TKA_E_WRONG_PARAMETER,
             "K = " + std::to_string(K) + " exceeds the engine's limit of " + std::to_string(kLargeKMax) +
                 " neighbours (perplexity < " + std::to_string(kLargeKMax / 3 + 1) + ")");
     const size_t budget = (size_t)512 << 20;
-    int B = (int)std::min<size_t>((size_t)N, std::max<size_t>(kPairTile, budget / ((size_t)N * sizeof(double))));
-    if (B < N) B = B / kPairTile * kPairTile;
+    int B = (int)std::min<size_t>((size_t)nq, std::max<size_t>(kPairTile, budget / ((size_t)N * sizeof(double))));
+    if (B < nq) B = B / kPairTile * kPairTile;
     DevBuf<double> buf((size_t)B * N, mb);
-    for (int q0 = 0; q0 < N; q0 += B) {
-        const int b = std::min(B, N - q0);
+    for (int q0 = 0; q0 < nq; q0 += B) {
+        const int b = std::min(B, nq - q0);
         hipLaunchKernelGGL(pair_sqdist_rows, dim3(ceil_div(N, kPairTile), ceil_div(b, kPairTile)), dim3(256), 0, s,
-                           Xn, N, D, q0, b, buf.get());
+                           Xn, N, D, qlist, q0, b, buf.get());
         TKA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(select_k_rows, dim3(b), dim3(256), 0, s, (const double*)buf.get(), N, b, K, q0, idx, dist);
+        hipLaunchKernelGGL(select_k_rows, dim3(b), dim3(256), 0, s, (const double*)buf.get(), N, b, K, qlist, q0, idx,
+                           dist);
         TKA_LAUNCH_CHECK();
     }
     TKA_HIP(hipStreamSynchronize(s));  // buf is released on return
@@ -1713,10 +1729,12 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     require(dtype == TKA_DIST_FP32 || dtype == TKA_DIST_BF16 || dtype == TKA_DIST_FP8, TKA_E_UNSUPPORTED,
             "distance dtype " + std::to_string(dtype) + " is not available");
     if (K > kMaxKprime) {  // all distances in FP64 (whatever the tile dtype): exact without a certificate
-        knn_large_k(Xn, N, D, K, idx, dist, mb, s);
+        knn_exact_rows(Xn, N, D, K, nullptr, N, idx, dist, mb, s);
         return;
     }
-    if (kprime <= 0) kprime = 2 * K;
+    // default candidates per row: 1.5 K for the certified tiles (rows the
+    // bound cannot certify are re-solved exactly), 2 K for FP8 (not re-solved)
+    if (kprime <= 0) kprime = dtype == TKA_DIST_FP8 ? 2 * K : std::max(K + (K + 1) / 2, K + 16);
     kprime = std::max(kprime, K);
     kprime = std::min(kprime, kMaxKprime);
     kprime = std::min(kprime, N - 1);
@@ -1913,38 +1931,21 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(N, 4)), dim3(256), 0, s, ra);
     TKA_LAUNCH_CHECK();
 
-    // Rows whose bound failed: FP32 tiles with the widest candidate list
-    // (rows in the tiles' order; the re-rank maps them to the caller's).
+    // Rows whose bound failed: every distance of those rows in FP64 (exact,
+    // no certificate needed), with X in the caller's order so that ties go
+    // by the caller's index.  Their cost is the rows' share of the N^2 D
+    // work, where a tile pass costs a whole stream of the candidates.
     DevBuf<int> list(N, mb), nfail(1, mb);
     TKA_HIP(hipMemsetAsync(nfail.get(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), N, list.get(),
-                       nfail.get());
+    hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), N, perm.get(),
+                       list.get(), nfail.get());
     TKA_LAUNCH_CHECK();
     int hf = 0;
     TKA_HIP(hipMemcpyAsync(&hf, nfail.get(), sizeof(int), hipMemcpyDeviceToHost, s));
     TKA_HIP(hipStreamSynchronize(s));
     if (st) st->certificate_failures = hf;
-    if (st) st->unresolved = hf;
-    if (hf > 0 && dtype != TKA_DIST_FP8) {
-        // Deterministic order of the re-solved rows does not matter: each row
-        // is solved independently and written to its own output slot.
-        TileRun tr2;
-        const int kp2 = std::min(kMaxKprime, N - 1);
-        fp32_tiles(list.get(), hf, kp2, tr2);
-        RerankArgs rb{Xs, ns ? ns : n2.get(), N, D, K, list.get(), hf, tr2.buf.get(), tr2.cnt.get(), tr2.tau.get(),
-                      eps_a, eps_b, rmax, idx, dist, tr2.fail.get(), perm.get()};
-        hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(hf, 4)), dim3(256), 0, s, rb);
-        TKA_LAUNCH_CHECK();
-        DevBuf<int> list2(hf, mb);
-        TKA_HIP(hipMemsetAsync(nfail.get(), 0, sizeof(int), s));
-        hipLaunchKernelGGL(collect_failures, dim3(ceil_div(hf, 256)), dim3(256), 0, s, tr2.fail.get(), hf,
-                           list2.get(), nfail.get());
-        TKA_LAUNCH_CHECK();
-        int hf2 = 0;
-        TKA_HIP(hipMemcpyAsync(&hf2, nfail.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-        TKA_HIP(hipStreamSynchronize(s));
-        if (st) st->unresolved = hf2;
-    }
+    if (st) st->unresolved = dtype == TKA_DIST_FP8 ? hf : 0;
+    if (hf > 0 && dtype != TKA_DIST_FP8) knn_exact_rows(Xn, N, D, K, list.get(), hf, idx, dist, mb, s);
 }
 
 }  // namespace tka

@@ -81,6 +81,20 @@ def test_knn_large_k_bit_exact(engine, oracle, dtype, n, d, K):
     assert fails == 0
 
 
+@pytest.mark.parametrize("dtype", ["fp32", "bf16"])
+def test_knn_uncertified_rows_are_resolved_exactly(engine, oracle, dtype):
+    # k' = K leaves little margin for the error bound: rows fail their
+    # certificate and are re-solved with every distance in FP64; the result
+    # is still the exact neighbour list (the tiles' rows are in locality
+    # order, the re-solve works in the caller's: ties by the caller's index)
+    X = oracle.normalize(mixture(1500, 50))
+    idx, dist, fails = engine.knn(X, 60, dist_dtype=dtype, candidates=60)
+    assert fails > 0  # (the compaction keeps somewhat more than k', so most rows still certify)
+    ridx, rdist = oracle.knn_brute(X, 60)
+    np.testing.assert_array_equal(idx, ridx)
+    np.testing.assert_array_equal(dist, rdist)
+
+
 def test_knn_large_k_ties(engine, oracle):
     # a lattice with duplicated points: many equal distances around the K-th
     g = np.stack(np.meshgrid(np.arange(20.0), np.arange(20.0)), -1).reshape(-1, 2)
