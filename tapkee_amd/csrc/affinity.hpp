@@ -23,16 +23,18 @@ void symmetrize_dense_device(double* P, int N, MemBudget* mb, hipStream_t s);
 
 // CSR symmetrisation (tsne.hpp:207-322): (P + P^T)/2 over the union
 // pattern, columns ascending in each row.  Output buffers are allocated here.
+// order (optional, N entries): a locality order of the points in which to
+// visit the rows; the result does not depend on it.
 struct Csr {
     DevBuf<int> row;     // N+1
     DevBuf<int> col;     // nnz
     DevBuf<double> val;  // nnz
     int64_t nnz = 0;
 };
-void symmetrize_knn_device(const int* idx, const double* val, int N, int K, Csr& out, MemBudget* mb, hipStream_t s);
+void symmetrize_knn_device(const int* idx, const double* val, int N, int K, Csr& out, MemBudget* mb, hipStream_t s,
+                           const uint32_t* order = nullptr);
 
 // Divides a device array by the FP64 sum of its entries (tsne.hpp:120-124).
-// Returns the sum.
-double normalize_sum_device(double* v, int64_t n, MemBudget* mb, hipStream_t s);
+void normalize_sum_device(double* v, int64_t n, MemBudget* mb, hipStream_t s);
 
 }  // namespace tka

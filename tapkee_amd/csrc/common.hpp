@@ -190,6 +190,11 @@ class DevBuf {
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Copies `bytes` (at most 256) of device memory to the host after the work
+// queued on s, through a pinned per-thread buffer (a pageable destination
+// makes the runtime stage the copy), and waits for it.
+void read_to_host(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 // Buffers carved one after another (256-byte aligned) out of one region that
 // the caller owns.  With a null base nothing is carved and `used` counts the
 // bytes a region would need.

@@ -36,6 +36,9 @@ std::mutex g_cache_mu;
 std::map<std::pair<int, size_t>, std::vector<CachedBlock>> g_cache;  // (device, block bytes) -> free blocks
 size_t g_cache_bytes = 0;
 constexpr size_t kCacheLimit = (size_t)32 << 30;  // cached bytes beyond this are returned with hipFree
+// Events of reused blocks, kept for the next frees on the same device
+// (creating an event per free costs more host time than recording one).
+std::map<int, std::vector<hipEvent_t>> g_free_events;
 
 // Size classes: powers of two up to 1 MiB, whole 2 MiB pages above.
 size_t size_class(size_t b)
@@ -128,7 +131,7 @@ void* dev_alloc(size_t bytes, size_t* block_bytes, int* device)
             for (size_t i = 0; i < v.size(); i++) {
                 if (hipEventQuery(v[i].ready) != hipSuccess) continue;  // still in use by queued work
                 void* p = v[i].p;
-                (void)hipEventDestroy(v[i].ready);
+                g_free_events[dev].push_back(v[i].ready);
                 v.erase(v.begin() + (long)i);
                 g_cache_bytes -= bb;
                 *block_bytes = bb;
@@ -150,10 +153,29 @@ void* dev_alloc(size_t bytes, size_t* block_bytes, int* device)
     return p;
 }
 
+void read_to_host(void* dst, const void* src, size_t bytes, hipStream_t s)
+{
+    static thread_local void* pinned = nullptr;  // held until the process ends
+    require(bytes <= 256, TKA_E_INTERNAL, "read_to_host: at most 256 bytes");
+    if (!pinned) TKA_HIP(hipHostMalloc(&pinned, 256, hipHostMallocDefault));
+    TKA_HIP(hipMemcpyAsync(pinned, src, bytes, hipMemcpyDeviceToHost, s));
+    TKA_HIP(hipStreamSynchronize(s));
+    std::memcpy(dst, pinned, bytes);
+}
+
 void dev_free(void* p, size_t bb, int dev, hipStream_t stream) noexcept
 {
     hipEvent_t ev = nullptr;
-    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess && hipEventRecord(ev, stream) == hipSuccess) {
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto& pool = g_free_events[dev];
+        if (!pool.empty()) {
+            ev = pool.back();
+            pool.pop_back();
+        }
+    }
+    if (!ev && hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+    if (ev && hipEventRecord(ev, stream) == hipSuccess) {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         if (g_cache_bytes + bb <= kCacheLimit) {
             g_cache[{dev, bb}].push_back(CachedBlock{p, ev});
@@ -719,6 +741,16 @@ void validate_graph(const SimilarityInput& in, int N)
                 require(std::isfinite(in.dist[e]) && in.dist[e] >= 0.0, TKA_E_WRONG_PARAMETER,
                         "neighbour distances must be finite and non-negative");
             }
+        // a neighbour listed twice in one row has no meaning in the union pattern E u E^T
+        std::vector<int> row(in.K);
+        for (int i = 0; i < N; i++) {
+            std::copy(in.idx + (size_t)i * in.K, in.idx + (size_t)(i + 1) * in.K, row.begin());
+            std::sort(row.begin(), row.end());
+            const auto dup = std::adjacent_find(row.begin(), row.end());
+            if (dup != row.end())
+                fail(TKA_E_WRONG_PARAMETER, "object " + std::to_string(*dup) +
+                                                " is listed twice among the neighbours of object " + std::to_string(i));
+        }
     } else if (in.kind == SimilarityInput::kJointP) {
         require(in.row && in.col && in.val, TKA_E_WRONG_PARAMETER, "null CSR arrays");
         require(in.row[0] == 0, TKA_E_WRONG_PARAMETER, "row_ptr[0] must be 0");
@@ -784,6 +816,10 @@ tka_session* create_session(const SimilarityInput& in, int N, int no_dims, doubl
     c.exact = exact;
     c.dd_mode = o.dd_mode;
     hipStream_t s = c.s();
+    // every buffer of the build, and the session's, is used on s only: their
+    // frees record the reuse event there (not on the null stream, which
+    // would first wait for every other stream)
+    AllocStream alloc_on_s(s);
     MemBudget* mb = &c.mb;
     tka_tsne_stats& st = c.st;
     const double t0 = now_ms();
@@ -840,7 +876,7 @@ tka_session* create_session(const SimilarityInput& in, int N, int no_dims, doubl
             c.stream.sync();
             const double t3 = now_ms();
             Csr P;
-            symmetrize_knn_device(idx.get(), val.get(), N, K, P, mb, s);
+            symmetrize_knn_device(idx.get(), val.get(), N, K, P, mb, s, ordered ? order.get() : nullptr);
             normalize_sum_device(P.val.get(), P.nnz, mb, s);  // tsne.hpp:120-124
             c.stream.sync();
             const double t4 = now_ms();

@@ -775,8 +775,7 @@ double sum_device(const double* v, int n, MemBudget* mb, hipStream_t s)
     hipLaunchKernelGGL(sum_fixed, dim3(1), dim3(256), 0, s, v, n, out.get());
     TKA_LAUNCH_CHECK();
     double h = 0.0;
-    TKA_HIP(hipMemcpyAsync(&h, out.get(), sizeof(double), hipMemcpyDeviceToHost, s));
-    TKA_HIP(hipStreamSynchronize(s));
+    read_to_host(&h, out.get(), sizeof h, s);
     return h;
 }
 

@@ -109,23 +109,19 @@ TKA_DEV void wave_sort_pairs(uint64_t (&d)[8], uint32_t (&ix)[8])
 // ---------------------------------------------------------------- normalise
 // Column means with the reference's summation order: row 0, 1, 2, ... into a
 // zero-initialised FP64 accumulator, no contraction (tsne.hpp:496-506).  Each
-// column's sum is one chain of N dependent adds, so one wave takes one column:
-// its lanes keep kColGroups groups of 64 consecutive rows in flight (strided
-// loads of that column; the columns' waves share the cache lines), and the
-// chain takes the values of a group lane by lane through readlane.
+// column's sum is one chain of N dependent adds, so its time is N times the
+// latency of one FP64 add.  One wave takes one column: its lanes load
+// kColGroups groups of 64 consecutive rows one batch ahead (strided loads of
+// that column; the columns' waves share the cache lines) and store the batch
+// to LDS in row order; every lane then runs the same chain over the batch
+// from broadcast LDS reads (no cross-lane moves inside the chain).
 constexpr int kColGroups = 8;
-
-TKA_DEV double readlane_d(double v, int j)
-{
-    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, j);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), j);
-    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
-}
+constexpr int kColBatch = 64 * kColGroups;
 
 __global__ __launch_bounds__(64) void col_mean_seq(const double* __restrict__ X, int N, int D,
                                                    double* __restrict__ mean)
 {
+    __shared__ __attribute__((aligned(16))) double s_rows[2][kColBatch];
     const int d = blockIdx.x, lane = threadIdx.x;
     double buf[kColGroups];
 #pragma unroll
@@ -134,19 +130,42 @@ __global__ __launch_bounds__(64) void col_mean_seq(const double* __restrict__ X,
         buf[g] = row < N ? X[(size_t)row * D + d] : 0.0;
     }
     double acc = 0.0;
-    for (int r0 = 0; r0 < N; r0 += 64 * kColGroups) {
+    int b = 0;
+    for (int r0 = 0; r0 < N; r0 += kColBatch, b ^= 1) {
+        double* sb = s_rows[b];
 #pragma unroll
-        for (int g = 0; g < kColGroups; g++) {
-            const int base = r0 + 64 * g;
-            const double v = buf[g];
-            const int next = base + 64 * kColGroups + lane;  // refill with the group kColGroups ahead
+        for (int g = 0; g < kColGroups; g++) sb[g * 64 + lane] = buf[g];
+#pragma unroll
+        for (int g = 0; g < kColGroups; g++) {  // the next batch, in flight under this one's chain
+            const int next = r0 + kColBatch + g * 64 + lane;
             buf[g] = next < N ? X[(size_t)next * D + d] : 0.0;
-            if (base + 64 <= N) {
+        }
+        __syncthreads();  // one wave: orders the stores before the broadcast reads
+        const int cnt = N - r0 < kColBatch ? N - r0 : kColBatch;
+        if (cnt == kColBatch) {
+            // software-pipelined: the broadcast reads of the next 32 values
+            // are issued before the chain over the current 32
+            const double2* s2 = reinterpret_cast<const double2*>(sb);
+            constexpr int kP = 16;
+            double2 cur[kP], nxt[kP];
 #pragma unroll
-                for (int j = 0; j < 64; j++) acc = dadd_sep(acc, readlane_d(v, j));
-            } else if (base < N) {
-                for (int j = 0; j < N - base; j++) acc = dadd_sep(acc, __shfl(v, j, 64));
+            for (int k = 0; k < kP; k++) cur[k] = s2[k];
+#pragma unroll
+            for (int t0 = 0; t0 < kColBatch / 2; t0 += kP) {
+                if (t0 + kP < kColBatch / 2) {
+#pragma unroll
+                    for (int k = 0; k < kP; k++) nxt[k] = s2[t0 + kP + k];
+                }
+#pragma unroll
+                for (int k = 0; k < kP; k++) {
+                    acc = dadd_sep(acc, cur[k].x);
+                    acc = dadd_sep(acc, cur[k].y);
+                }
+#pragma unroll
+                for (int k = 0; k < kP; k++) cur[k] = nxt[k];
             }
+        } else {
+            for (int t = 0; t < cnt; t++) acc = dadd_sep(acc, sb[t]);
         }
     }
     if (lane == 0) mean[d] = acc / (double)N;
@@ -1084,12 +1103,13 @@ struct RerankArgs {
     const float* tau;
     double eps_a;         // bound = eps_a * ||q|| * Rmax + eps_b * Rmax^2
     double eps_b;
-    double rmax;
+    const double* rmax;   // device word: the largest row norm
     int* idx;             // N x K (indexed by row)
     double* dist;
     int* fail;            // per slot: 1 if the bound failed
     const uint32_t* perm; // optional: X rows are the caller's rows perm[0..N) (outputs and ties use those)
 };
+
 
 // Exact FP64 squared distance, summed in index order without contraction
 // (the reference's euclidean_distance, vptree.hpp:95-101).
@@ -1363,7 +1383,8 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     const uint64_t* qbuf = a.buf + (size_t)slot * kCap;
     const double* xq = a.X + (size_t)q * a.D;
     const double qn2 = a.n2[q];
-    const double eps = a.eps_a * sqrt(qn2) * a.rmax + a.eps_b * a.rmax * a.rmax + 1e-12 * (qn2 + a.rmax * a.rmax);
+    const double rmax = *a.rmax;
+    const double eps = a.eps_a * sqrt(qn2) * rmax + a.eps_b * rmax * rmax + 1e-12 * (qn2 + rmax * rmax);
     const int K = a.K;
 
     __shared__ uint64_t s_list[4][kCap];
@@ -1491,63 +1512,152 @@ __global__ __launch_bounds__(256) void proj3(const double* __restrict__ X, int N
     }
 }
 
-// Per-workgroup partials of X^T P (D x 3) over kGramRows rows: thread d
-// accumulates column d in row order (deterministic).
+// Per-workgroup partials of X^T P (D x 3) over kGramRows rows: thread t
+// takes column t % 64 (+ 64 j) of every fourth row (coalesced row reads), the
+// four row phases are added in LDS (fixed order: deterministic).
 constexpr int kGramRows = 256;
 __global__ __launch_bounds__(256) void gram3(const double* __restrict__ X, int N, int D, const float* __restrict__ P,
                                              float* __restrict__ part)
 {
+    __shared__ float sh[4][64][3];
     const int r0 = blockIdx.x * kGramRows, r1 = min(N, r0 + kGramRows);
-    for (int d = threadIdx.x; d < D; d += 256) {
+    const int dl = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    for (int d0 = 0; d0 < D; d0 += 64) {
+        const int d = d0 + dl;
         float g0 = 0.f, g1 = 0.f, g2 = 0.f;
-        for (int r = r0; r < r1; r++) {
-            const float x = (float)X[(size_t)r * D + d];
-            g0 = fmaf(x, P[3 * (size_t)r], g0);
-            g1 = fmaf(x, P[3 * (size_t)r + 1], g1);
-            g2 = fmaf(x, P[3 * (size_t)r + 2], g2);
+        if (d < D) {
+            for (int r = r0 + ph; r < r1; r += 4) {
+                const float x = (float)X[(size_t)r * D + d];
+                g0 = fmaf(x, P[3 * (size_t)r], g0);
+                g1 = fmaf(x, P[3 * (size_t)r + 1], g1);
+                g2 = fmaf(x, P[3 * (size_t)r + 2], g2);
+            }
         }
-        float* o = part + ((size_t)blockIdx.x * D + d) * 3;
-        o[0] = g0;
-        o[1] = g1;
-        o[2] = g2;
+        sh[ph][dl][0] = g0;
+        sh[ph][dl][1] = g1;
+        sh[ph][dl][2] = g2;
+        __syncthreads();
+        if (ph == 0 && d < D) {
+            float* o = part + ((size_t)blockIdx.x * D + d) * 3;
+#pragma unroll
+            for (int k = 0; k < 3; k++) o[k] = ((sh[0][dl][k] + sh[1][dl][k]) + sh[2][dl][k]) + sh[3][dl][k];
+        }
+        __syncthreads();
     }
 }
 
-TKA_DEV uint64_t spread21(uint64_t v)
+// Bounding box of the projections (min x, y, z, max x, y, z) from proj3's
+// per-workgroup partials.  One workgroup.
+__global__ __launch_bounds__(256) void reduce_lim(const float* __restrict__ part, int nb, float* __restrict__ lim)
 {
-    v &= 0x1fffffull;
-    v = (v | (v << 32)) & 0x1f00000000ffffull;
-    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
-    v = (v | (v << 8)) & 0x100f00f00f00f00full;
-    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
-    v = (v | (v << 2)) & 0x1249249249249249ull;
+    __shared__ float sh[6][4];
+    float v[6] = {HUGE_VALF, HUGE_VALF, HUGE_VALF, -HUGE_VALF, -HUGE_VALF, -HUGE_VALF};
+    for (int b = threadIdx.x; b < nb; b += 256) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            v[k] = fminf(v[k], part[6 * b + k]);
+            v[k + 3] = fmaxf(v[k + 3], part[6 * b + k + 3]);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        for (int o = 32; o > 0; o >>= 1) {
+            v[k] = fminf(v[k], __shfl_xor(v[k], o, 64));
+            v[k + 3] = fmaxf(v[k + 3], __shfl_xor(v[k + 3], o, 64));
+        }
+    }
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 6; k++) sh[k][threadIdx.x >> 6] = v[k];
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        float r = sh[threadIdx.x][0];
+        for (int w = 1; w < 4; w++) r = threadIdx.x < 3 ? fminf(r, sh[threadIdx.x][w]) : fmaxf(r, sh[threadIdx.x][w]);
+        lim[threadIdx.x] = r;
+    }
+}
+
+// The fixed starting projection: 3 x D values uniform in (-1, 1) from a
+// 64-bit LCG (any fixed full-rank projection works).
+__global__ void proj_init(float* __restrict__ R, int n)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    for (int e = 0; e < n; e++) {
+        st = st * 6364136223846793005ull + 1442695040888963407ull;
+        R[e] = (float)((double)(st >> 11) * 0x1.0p-53 * 2.0 - 1.0);
+    }
+}
+
+// G = X^T P (D x 3) from gram3's ngr partials; R <- orth(G): Gram-Schmidt on
+// the three columns in FP64 (one workgroup); a column that vanishes (data of
+// rank < 3) keeps the previous direction.
+__global__ __launch_bounds__(256) void gram_sum(const float* __restrict__ gpart, int ngr, int D, double* __restrict__ G)
+{
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * 4 + (threadIdx.x >> 6);  // a wave per entry
+    if (e >= 3 * D) return;
+    double g = 0.0;
+    for (int b = lane; b < ngr; b += 64) g += gpart[(size_t)b * D * 3 + e];
+    g = wave_sum(g);
+    if (lane == 0) G[e] = g;
+}
+
+__global__ __launch_bounds__(256) void orth3(int D, double* __restrict__ G, float* __restrict__ R)
+{
+    __shared__ double sh[4];
+    for (int k = 0; k < 3; k++) {
+        for (int j = 0; j < k; j++) {
+            double dot = 0.0;
+            for (int d = threadIdx.x; d < D; d += 256) dot += G[3 * d + k] * G[3 * d + j];
+            dot = block_sum(dot, sh);
+            for (int d = threadIdx.x; d < D; d += 256) G[3 * d + k] -= dot * G[3 * d + j];
+            __syncthreads();
+        }
+        double nrm = 0.0;
+        for (int d = threadIdx.x; d < D; d += 256) nrm += G[3 * d + k] * G[3 * d + k];
+        nrm = sqrt(block_sum(nrm, sh));
+        for (int d = threadIdx.x; d < D; d += 256) {
+            if (nrm > 1e-30) {
+                G[3 * d + k] /= nrm;
+                R[3 * d + k] = (float)G[3 * d + k];
+            } else {
+                G[3 * d + k] = 0.0;  // later columns ignore it
+            }
+        }
+        __syncthreads();
+    }
+}
+
+TKA_DEV uint32_t spread10(uint32_t v)
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
     return v;
 }
 
-__global__ __launch_bounds__(256) void morton3(const float* __restrict__ P, int N, const float* __restrict__ part,
-                                               int nb, uint64_t* __restrict__ keys, uint32_t* __restrict__ vals)
+// 30-bit Morton keys (10 bits per projected axis) over the projections'
+// bounding box.
+__global__ __launch_bounds__(256) void morton3(const float* __restrict__ P, int N, const float* __restrict__ lim,
+                                               uint32_t* __restrict__ keys, uint32_t* __restrict__ vals)
 {
-    __shared__ float lim[6];
-    if (threadIdx.x < 6) {
-        float r = part[threadIdx.x];
-        for (int b = 1; b < nb; b++)
-            r = threadIdx.x < 3 ? fminf(r, part[6 * b + threadIdx.x]) : fmaxf(r, part[6 * b + threadIdx.x]);
-        lim[threadIdx.x] = r;
-    }
-    __syncthreads();
     const int n = blockIdx.x * 256 + threadIdx.x;
     if (n >= N) return;
-    uint64_t key = 0;
+    uint32_t key = 0;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const float span = lim[k + 3] - lim[k];
         float t = span > 0.f ? (P[3 * (size_t)n + k] - lim[k]) / span : 0.f;
         t = fminf(fmaxf(t, 0.f), 1.f);
-        key |= spread21((uint64_t)(t * 2097151.f)) << k;
+        key |= spread10((uint32_t)(t * 1023.f)) << k;
     }
     keys[n] = key;
     vals[n] = (uint32_t)n;
 }
+
+__global__ void copy_word(const uint32_t* __restrict__ src, int* __restrict__ dst) { *dst = (int)*src; }
 
 __global__ __launch_bounds__(256) void gather_rows(const double* __restrict__ X, const double* __restrict__ n2,
                                                    const uint32_t* __restrict__ perm, int N, int D,
@@ -1556,7 +1666,8 @@ __global__ __launch_bounds__(256) void gather_rows(const double* __restrict__ X,
     const size_t total = (size_t)N * D;
     for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
         const size_t i = e / D, d = e % D;
-        const uint32_t src = perm[i];
+        const uint32_t p = perm[i];
+        const uint32_t src = p < (uint32_t)N ? p : 0u;  // in bounds whatever the sort left
         Xp[e] = X[(size_t)src * D + d];
         if (d == 0) n2p[i] = n2[src];
     }
@@ -1577,8 +1688,7 @@ void normalize_device(double* X, int N, int D, MemBudget* mb, hipStream_t s)
     hipLaunchKernelGGL(reduce_max, dim3(1), dim3(256), 0, s, bmax.get(), nb, mx.get());
     TKA_LAUNCH_CHECK();
     double h = 0.0;
-    TKA_HIP(hipMemcpyAsync(&h, mx.get(), sizeof(double), hipMemcpyDeviceToHost, s));
-    TKA_HIP(hipStreamSynchronize(s));
+    read_to_host(&h, mx.get(), sizeof h, s);
     require(h > 0.0 && std::isfinite(h), TKA_E_WRONG_PARAMETER,
             "input has no spread: the largest centred coefficient is " + std::to_string(h) +
                 " (the reference would divide by it and return NaN)");
@@ -1691,7 +1801,6 @@ void run_tiles_wide(const void* A, const void* B, int NV, int N, int kprime, con
     TileArgs ta{N, kprime, 1, qlist, nq, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
     launch_tiles_wide<Op>(Aw.get(), Bw.get(), NV, Npad, nq, ta, s);
     TKA_LAUNCH_CHECK();
-    TKA_HIP(hipStreamSynchronize(s));  // Aw, Bw are released on return
 }
 
 }  // namespace
@@ -1719,7 +1828,6 @@ static void knn_exact_rows(const double* Xn, int N, int D, int K, const int* qli
                            dist);
         TKA_LAUNCH_CHECK();
     }
-    TKA_HIP(hipStreamSynchronize(s));  // buf is released on return
 }
 
 void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, int* idx, double* dist, KnnStats* st,
@@ -1746,8 +1854,10 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(reduce_max, dim3(1), dim3(256), 0, s, rpart.get(), nbr, rmaxd.get());
     TKA_LAUNCH_CHECK();
-    double rmax = 0.0;
-    TKA_HIP(hipMemcpyAsync(&rmax, rmaxd.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+    // status words read once after the re-rank: [0] rows that failed the
+    // bound, [1] the locality sort's fault word
+    DevBuf<int> status(2, mb);
+    TKA_HIP(hipMemsetAsync(status.get(), 0, 2 * sizeof(int), s));
 
     // Operands.
     DevBuf<__bf16> A, B;
@@ -1769,68 +1879,42 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     const double* Xs = Xn;
     const double* ns = nullptr;
     auto locality = [&]() {
-        std::vector<float> R(3 * (size_t)D);
-        uint64_t st64 = 0x9E3779B97F4A7C15ull;
-        for (auto& r : R) {  // uniform in (-1, 1): any fixed full-rank projection works
-            st64 = st64 * 6364136223846793005ull + 1442695040888963407ull;
-            r = (float)((double)(st64 >> 11) * 0x1.0p-53 * 2.0 - 1.0);
-        }
-        DevBuf<float> Rd(R.size(), mb), P((size_t)N * 3, mb), part((size_t)6 * nbr, mb);
+        DevBuf<float> Rd(3 * (size_t)D, mb), P((size_t)N * 3, mb), part((size_t)6 * nbr, mb);
         const int ngr = (int)ceil_div(N, kGramRows);
         DevBuf<float> gpart((size_t)ngr * D * 3, mb);
-        std::vector<float> gh((size_t)ngr * D * 3);
-        TKA_HIP(hipMemcpyAsync(Rd.get(), R.data(), R.size() * sizeof(float), hipMemcpyHostToDevice, s));
+        DevBuf<double> G(3 * (size_t)D, mb);
+        hipLaunchKernelGGL(proj_init, dim3(1), dim3(64), 0, s, Rd.get(), 3 * D);
+        TKA_LAUNCH_CHECK();
         for (int it = 0; it < 2; it++) {  // R <- orth(X^T X R)
             hipLaunchKernelGGL(proj3, dim3(nbr), dim3(256), 0, s, Xn, N, D, Rd.get(), P.get(), part.get());
             TKA_LAUNCH_CHECK();
             hipLaunchKernelGGL(gram3, dim3(ngr), dim3(256), 0, s, Xn, N, D, P.get(), gpart.get());
             TKA_LAUNCH_CHECK();
-            TKA_HIP(hipMemcpyAsync(gh.data(), gpart.get(), gh.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-            TKA_HIP(hipStreamSynchronize(s));
-            std::vector<double> G(3 * (size_t)D, 0.0);
-            for (int b = 0; b < ngr; b++)
-                for (size_t e = 0; e < G.size(); e++) G[e] += gh[(size_t)b * D * 3 + e];
-            // Gram-Schmidt on the three columns; a column that vanishes keeps
-            // the previous direction (data of rank < 3)
-            for (int k = 0; k < 3; k++) {
-                for (int j = 0; j < k; j++) {
-                    double dot = 0.0;
-                    for (int d = 0; d < D; d++) dot += G[3 * d + k] * G[3 * d + j];
-                    for (int d = 0; d < D; d++) G[3 * d + k] -= dot * G[3 * d + j];
-                }
-                double nrm = 0.0;
-                for (int d = 0; d < D; d++) nrm += G[3 * d + k] * G[3 * d + k];
-                nrm = std::sqrt(nrm);
-                if (!(nrm > 1e-30)) {
-                    for (int d = 0; d < D; d++) G[3 * d + k] = 0.0;  // later columns ignore it
-                    continue;
-                }
-                for (int d = 0; d < D; d++) {
-                    G[3 * d + k] /= nrm;
-                    R[3 * d + k] = (float)G[3 * d + k];
-                }
-            }
-            TKA_HIP(hipMemcpyAsync(Rd.get(), R.data(), R.size() * sizeof(float), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(gram_sum, dim3(ceil_div(3 * D, 4)), dim3(256), 0, s, gpart.get(), ngr, D, G.get());
+            TKA_LAUNCH_CHECK();
+            hipLaunchKernelGGL(orth3, dim3(1), dim3(256), 0, s, D, G.get(), Rd.get());
+            TKA_LAUNCH_CHECK();
         }
         hipLaunchKernelGGL(proj3, dim3(nbr), dim3(256), 0, s, Xn, N, D, Rd.get(), P.get(), part.get());
         TKA_LAUNCH_CHECK();
-        DevBuf<uint64_t> k0(N, mb), k1(N, mb);
-        DevBuf<uint32_t> v0(N, mb), v1(N, mb), scr(radix_sort_scratch_words(N), mb);
+        DevBuf<uint32_t> k0(N, mb), k1(N, mb), v0(N, mb), v1(N, mb), scr(radix_sort_scratch_words(N), mb);
         TKA_HIP(hipMemsetAsync(scr.get(), 0, scr.size() * sizeof(uint32_t), s));
-        hipLaunchKernelGGL(morton3, dim3(nbr), dim3(256), 0, s, P.get(), N, part.get(), nbr, k0.get(), v0.get());
+        DevBuf<float> lim(6, mb);
+        hipLaunchKernelGGL(reduce_lim, dim3(1), dim3(256), 0, s, part.get(), nbr, lim.get());
         TKA_LAUNCH_CHECK();
-        const bool alt = radix_sort_pairs<uint64_t, uint32_t>(k0.get(), v0.get(), k1.get(), v1.get(), N, 63, scr.get(), s);
-        // the order gathers rows of X: never from a sort that gave up
-        require(radix_sort_fault(scr.get(), s) == 0, TKA_E_INTERNAL, "radix sort look-back gave up");
-        perm.alloc(N, mb);
-        TKA_HIP(hipMemcpyAsync(perm.get(), alt ? v1.get() : v0.get(), (size_t)N * sizeof(uint32_t),
-                               hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(morton3, dim3(nbr), dim3(256), 0, s, P.get(), N, lim.get(), k0.get(), v0.get());
+        TKA_LAUNCH_CHECK();
+        const bool alt = radix_sort_pairs<uint32_t, uint32_t>(k0.get(), v0.get(), k1.get(), v1.get(), N, 30, scr.get(), s);
+        // the order gathers rows of X: a sort that gave up is reported at the
+        // failure count's read (gather_rows keeps every read in bounds)
+        hipLaunchKernelGGL(copy_word, dim3(1), dim3(1), 0, s, scr.get() + 1, status.get() + 1);
+        TKA_LAUNCH_CHECK();
+        perm = std::move(alt ? v1 : v0);
         Xp.alloc((size_t)N * D, mb);
         n2p.alloc(N, mb);
         hipLaunchKernelGGL(gather_rows, dim3(gprep), dim3(256), 0, s, Xn, n2.get(), perm.get(), N, D, Xp.get(),
                            n2p.get());
         TKA_LAUNCH_CHECK();
-        TKA_HIP(hipStreamSynchronize(s));  // the temporaries are released on return
         Xs = Xp.get();
         ns = n2p.get();
     };
@@ -1856,7 +1940,6 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
             eps_a = 2.0 * (2.0 * (8 * kWideKS + NVo / kWideKS + 2) * u32);
         }
         eps_b = eps_a;
-        TKA_HIP(hipStreamSynchronize(s));  // Ah, Bh are released on return
     };
     if (dtype == TKA_DIST_BF16) {
         Kp = ((3 * D + 2) + 15) / 16 * 16;
@@ -1908,7 +1991,6 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
                                Kp / 16, NV, A8h.get());
             TKA_LAUNCH_CHECK();
             run_tiles_reg<OpFP8, 2, 1, 4>(A8h.get(), A8h.get(), NV, N, kprime, nullptr, N, tr, mb, s);
-            TKA_HIP(hipStreamSynchronize(s));  // A8h is released on return
         } else {
             run_tiles_fp8(A8.get(), scale8.get(), nrm8.get(), N, Kp, kprime, tr, mb, s);
         }
@@ -1921,13 +2003,12 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         locality();
         fp32_tiles(nullptr, N, kprime, tr);
     }
-    TKA_HIP(hipStreamSynchronize(s));  // rmax is on the host now
     if (order_out && perm.get()) {
         TKA_HIP(hipMemcpyAsync(order_out, perm.get(), (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
         if (st) st->order_written = true;
     }
     RerankArgs ra{Xs, ns ? ns : n2.get(), N, D, K, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get(),
-                  eps_a, eps_b, rmax, idx, dist, tr.fail.get(), perm.get()};
+                  eps_a, eps_b, rmaxd.get(), idx, dist, tr.fail.get(), perm.get()};
     hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(N, 4)), dim3(256), 0, s, ra);
     TKA_LAUNCH_CHECK();
 
@@ -1935,14 +2016,14 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     // no certificate needed), with X in the caller's order so that ties go
     // by the caller's index.  Their cost is the rows' share of the N^2 D
     // work, where a tile pass costs a whole stream of the candidates.
-    DevBuf<int> list(N, mb), nfail(1, mb);
-    TKA_HIP(hipMemsetAsync(nfail.get(), 0, sizeof(int), s));
+    DevBuf<int> list(N, mb);
     hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), N, perm.get(),
-                       list.get(), nfail.get());
+                       list.get(), status.get());
     TKA_LAUNCH_CHECK();
-    int hf = 0;
-    TKA_HIP(hipMemcpyAsync(&hf, nfail.get(), sizeof(int), hipMemcpyDeviceToHost, s));
-    TKA_HIP(hipStreamSynchronize(s));
+    int hs[2] = {0, 0};
+    read_to_host(hs, status.get(), sizeof hs, s);
+    require(hs[1] == 0, TKA_E_INTERNAL, "radix sort look-back gave up");
+    const int hf = hs[0];
     if (st) st->certificate_failures = hf;
     if (st) st->unresolved = dtype == TKA_DIST_FP8 ? hf : 0;
     if (hf > 0 && dtype != TKA_DIST_FP8) knn_exact_rows(Xn, N, D, K, list.get(), hf, idx, dist, mb, s);

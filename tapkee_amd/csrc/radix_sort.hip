@@ -357,8 +357,7 @@ template bool radix_sort_pairs<uint64_t, uint32_t>(uint64_t*, uint32_t*, uint64_
 uint32_t radix_sort_fault(const uint32_t* scratch, hipStream_t stream)
 {
     uint32_t f = 0;
-    TKA_HIP(hipMemcpyAsync(&f, scratch + 1, sizeof(f), hipMemcpyDeviceToHost, stream));
-    TKA_HIP(hipStreamSynchronize(stream));
+    read_to_host(&f, scratch + 1, sizeof(f), stream);
     return f;
 }
 

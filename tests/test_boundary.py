@@ -176,3 +176,6 @@ def test_precomputed_inputs_are_checked_before_device_work():
     with pytest.raises(tapkee_amd.WrongParameterError, match="finite"):
         tapkee_amd.Session.from_knn(np.array([[1], [2], [3], [0]]), np.array([[1.0], [np.nan], [1.0], [1.0]]),
                                     perplexity=0.3, theta=0.0)
+    with pytest.raises(tapkee_amd.WrongParameterError, match="listed twice"):
+        tapkee_amd.Session.from_knn(np.array([[1, 2], [2, 0], [3, 3], [0, 1]]), np.ones((4, 2)),
+                                    perplexity=0.6, theta=0.0)

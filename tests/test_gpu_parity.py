@@ -186,6 +186,49 @@ def test_symmetrize_many_tiles(engine, oracle):
     np.testing.assert_array_equal(sval, rsval[o])
 
 
+@pytest.mark.parametrize("K", [200, 300])
+def test_symmetrize_wide_rows(engine, oracle, K):
+    """K = 200 (rows of 256 sorted in a wave) and K = 300 (the sort path)."""
+    g = rng(9)
+    N = 3000
+    idx = np.stack([g.choice(N - 1, K, replace=False) for _ in range(N)]).astype(np.int64)
+    idx += idx >= np.arange(N)[:, None]
+    val = g.random((N, K))
+    row, col, sval = engine.symmetrize(idx, val)
+    rrow, rcol, rsval = oracle.symmetrize(idx, val)
+    np.testing.assert_array_equal(row, rrow)
+    rid = np.repeat(np.arange(N), np.diff(rrow))
+    o = np.lexsort((rcol, rid))
+    np.testing.assert_array_equal(col, rcol[o])
+    np.testing.assert_array_equal(sval, rsval[o])
+
+
+def test_symmetrize_hub_rows(engine, oracle):
+    """Points listed by thousands of rows they do not list themselves: a row
+    whose reverse-only entries exceed what a wave stages in LDS (1024)."""
+    g = rng(10)
+    N, K = 5000, 20
+    idx = np.stack([g.choice(N - 1, K, replace=False) for _ in range(N)]).astype(np.int64)
+    idx += idx >= np.arange(N)[:, None]
+    # rows 10.. list point 0 (in-degree ~2500) and rows 3000.. list point 1
+    for i in range(10, N, 2):
+        if 0 not in idx[i]:
+            idx[i, 0] = 0
+    for i in range(3000, N):
+        if 1 not in idx[i] and i != 1:
+            idx[i, 1] = 1
+    assert all(len(set(r)) == K for r in idx)
+    val = g.random((N, K))
+    row, col, sval = engine.symmetrize(idx, val)
+    rrow, rcol, rsval = oracle.symmetrize(idx, val)
+    np.testing.assert_array_equal(row, rrow)
+    assert row[1] - row[0] > 1024
+    rid = np.repeat(np.arange(N), np.diff(rrow))
+    o = np.lexsort((rcol, rid))
+    np.testing.assert_array_equal(col, rcol[o])
+    np.testing.assert_array_equal(sval, rsval[o])
+
+
 @pytest.mark.parametrize("dtype", ["fp32", "bf16"])
 def test_build_p_matches_oracle(engine, oracle, dtype):
     """The whole input-similarity stage: identical pattern, values to 1e-13."""
