@@ -4,7 +4,8 @@
 // Pipeline per call:
 //   1. prep:   rows put in a locality order (Morton order of a projection on
 //              their estimated top-3 principal subspace), then low-precision
-//              operands (FP32, BF16 hi/lo splits or FP8) laid out for MFMA,
+//              operands laid out for MFMA (FP32; BF16 centred on each query
+//              block's mean, or hi/lo splits for rows too wide for that; FP8),
 //              plus FP64 row norms.
 //   2. tiles:  each wave owns 32 query rows; the block streams all candidate
 //              rows through LDS in chunks of 128 and computes 32x32 distance
@@ -18,10 +19,12 @@
 //              distances in FP64 (same summation as the reference's
 //              euclidean_distance, vptree.hpp:95-101), sorts by
 //              (distance, index) and checks an a-priori error bound: the K-th
-//              exact distance must lie below (threshold - bound), which proves
-//              no dropped candidate can belong to the top K.
+//              exact distance must lie below the smallest exact distance a
+//              value at the threshold can stand for, which proves no dropped
+//              candidate can belong to the top K.
 //   4. rows failing the bound get every distance in FP64 and an exact
-//      selection (knn_exact_rows), as do all rows when K exceeds the buffers.
+//      selection among those within the K-th distance already found
+//      (knn_exact_rows), as do all rows when K exceeds the buffers.
 #include "knn.hpp"
 #include "radix_sort.hpp"
 
@@ -556,20 +559,14 @@ __global__ __launch_bounds__(256) void prep_fp8(const double* __restrict__ X, co
 // copy of chunk c+1 running while chunk c is multiplied and filtered.  The
 // odd row stride (2*NV + 1 vectors) spreads consecutive rows over the LDS
 // banks.
-constexpr int kNVLimit = 20;       // widest row (vectors per half) streamed as whole rows
-constexpr int kNVSplitLimit = 28;  // widest row kept in registers, streamed in two k-stages
+constexpr int kNVLimit = 20;       // widest row (vectors per half) kept in registers
 
-// Rows wider than kNVLimit are streamed in two k-stages: each chunk's first
-// split_nv1(NV) vectors per half, then the rest, through the same two LDS
-// buffers, so the buffers hold about half a row each.
-inline int split_nv1(int NV) { return NV <= kNVLimit ? NV : (NV + 1) / 2; }
-
-// The register tiles' double-buffered chunks (or chunk stages) plus the
-// kernel's own LDS (row counters and thresholds) must fit the 160 KiB of a CU.
-inline bool reg_tiles_fit(int NV, size_t vec_bytes, bool allow_split = false)
+// The register tiles' double-buffered chunks plus the kernel's own LDS (row
+// counters and thresholds) must fit the 160 KiB of a CU.
+inline bool reg_tiles_fit(int NV, size_t vec_bytes)
 {
-    if (NV > (allow_split ? kNVSplitLimit : kNVLimit)) return false;
-    const size_t dyn = 2 * (size_t)kCB * (2 * split_nv1(NV) + 1) * vec_bytes;
+    if (NV > kNVLimit) return false;
+    const size_t dyn = 2 * (size_t)kCB * (2 * NV + 1) * vec_bytes;
     return dyn + 256 * 8 <= 160 * 1024;
 }
 
@@ -635,23 +632,6 @@ TKA_DEV void dma_to_lds(const unsigned char* src, unsigned char* dst, int bytes,
         __builtin_amdgcn_global_load_lds(src + (size_t)e * 1024 + 16 * lane, (lds_void*)(dst + e * 1024), 16, 0, 0);
 }
 
-// One k-stage of a chunk into LDS: kCB rows of RSS = 2*NVS + 1 vectors, the
-// vectors [G0, G0 + NVS) of each half of the global rows (stride RS = 2*NV +
-// 1) and their pad vector.  Each lane of a direct LDS load names its own
-// source, so the stage image is gathered in one pass.
-template <class V, int NV, int NVS, int G0, int WAVES>
-TKA_DEV void dma_stage_to_lds(const V* rows, unsigned char* dst)
-{
-    constexpr int RS = 2 * NV + 1, RSS = 2 * NVS + 1;
-    static_assert(sizeof(V) == 16, "one vector per lane per direct load");
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int e = w; e < kCB * RSS / 64; e += WAVES) {
-        const int v = e * 64 + lane, r = v / RSS, k = v % RSS;
-        const int gk = k < NVS ? G0 + k : (k < 2 * NVS ? NV + G0 + (k - NVS) : 2 * NV);
-        __builtin_amdgcn_global_load_lds(rows + (size_t)r * RS + gk, (lds_void*)(dst + e * 1024), 16, 0, 0);
-    }
-}
-
 // acc[u][t] += the products of k-steps [GQ, GQ + NVS) of the chunk image sA
 // (row stride RSS, NVS vectors per half) with the wave's query operand.  Step
 // g+1's four candidate fragments are read from LDS before step g's MFMAs
@@ -691,26 +671,20 @@ TKA_DEV void tile_mma(const typename Op::V* sA, const typename Op::V (&bq)[QT][N
 // MFMAs; every chunk copied into LDS serves all the block's queries.  Two
 // waves per SIMD (WAVES = 8, or two blocks per CU) let one wave's filter run
 // under another's MFMAs.
-// NV1 < NV: rows streamed in two k-stages of NV1 and NV - NV1 vectors per half.
-template <class Op, int kMode, int NV, int QT, int WAVES, int NV1 = NV>
+template <class Op, int kMode, int NV, int QT, int WAVES>
 __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V* __restrict__ A,
                                                             const typename Op::V* __restrict__ B, TileArgs a)
 {
     typedef typename Op::V V;
-    constexpr bool kSplit = NV1 < NV;
-    constexpr int NV2 = NV - NV1;
-    static_assert(!kSplit || NV2 <= NV1, "the first stage is the wider one");
     constexpr int QB = 32 * WAVES * QT;  // query rows per block
-    // two chunks of kCB rows x (2*NV + 1) vectors, or two stage images of
-    // kCB rows x (2*NV1 + 1) vectors
+    // two chunks of kCB rows x (2*NV + 1) vectors
     extern __shared__ __align__(16) unsigned char dyn_lds[];
     __shared__ int s_cnt[QB];
     __shared__ float s_tau[QB];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
     const int slot0 = blockIdx.x * QB;
     constexpr int RS = 2 * NV + 1;
-    constexpr int RS1 = 2 * NV1 + 1, RS2 = 2 * NV2 + 1;
-    const int chunk_bytes = kCB * RS1 * (int)sizeof(V);  // one LDS buffer
+    const int chunk_bytes = kCB * RS * (int)sizeof(V);  // one LDS buffer
     for (int i = tid; i < QB; i += 64 * WAVES) {
         s_cnt[i] = 0;
         s_tau[i] = HUGE_VALF;
@@ -732,10 +706,7 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
     const unsigned char* src = reinterpret_cast<const unsigned char*>(A);
     const int nch = (a.N + kCB - 1) / kCB;
     const ChunkOrder order(seed_chunk(a, slot0, nch), nch);
-    if constexpr (kSplit)
-        dma_stage_to_lds<V, NV, NV1, 0, WAVES>(A + (size_t)order.at(0) * kCB * RS, dyn_lds);
-    else
-        dma_to_lds(src + (size_t)order.at(0) * kCB * RS * sizeof(V), dyn_lds, chunk_bytes, WAVES);
+    dma_to_lds(src + (size_t)order.at(0) * kCB * RS * sizeof(V), dyn_lds, chunk_bytes, WAVES);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     for (int c = 0; c < nch; c++) {
@@ -745,28 +716,13 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
         for (int u = 0; u < QT; u++)
 #pragma unroll
             for (int t = 0; t < 4; t++) acc[u][t] = f32x16{};
-        const V* sA;  // image holding the pad vectors, for the filter
-        int sRS;
-        if constexpr (kSplit) {
-            // stage 1 is in buffer 0; stage 2 goes to buffer 1 meanwhile
-            unsigned char* b1 = dyn_lds + chunk_bytes;
-            dma_stage_to_lds<V, NV, NV2, NV1, WAVES>(A + (size_t)cb * RS, b1);
-            tile_mma<Op, NV1, RS1, 0, QT, NV>(reinterpret_cast<const V*>(dyn_lds), bq, acc);
-            __builtin_amdgcn_s_waitcnt(0);
-            __syncthreads();  // stage 2 landed; buffer 0 is free
-            if (c + 1 < nch) dma_stage_to_lds<V, NV, NV1, 0, WAVES>(A + (size_t)order.at(c + 1) * kCB * RS, dyn_lds);
-            tile_mma<Op, NV2, RS2, NV1, QT, NV>(reinterpret_cast<const V*>(b1), bq, acc);
-            sA = reinterpret_cast<const V*>(b1);
-            sRS = RS2;
-        } else {
-            unsigned char* cur = dyn_lds + (c & 1) * chunk_bytes;
-            if (c + 1 < nch)
-                dma_to_lds(src + (size_t)order.at(c + 1) * kCB * RS * sizeof(V),
-                           dyn_lds + ((c + 1) & 1) * chunk_bytes, chunk_bytes, WAVES);
-            tile_mma<Op, NV, RS, 0, QT, NV>(reinterpret_cast<const V*>(cur), bq, acc);
-            sA = reinterpret_cast<const V*>(cur);
-            sRS = RS;
-        }
+        unsigned char* cur = dyn_lds + (c & 1) * chunk_bytes;
+        if (c + 1 < nch)
+            dma_to_lds(src + (size_t)order.at(c + 1) * kCB * RS * sizeof(V), dyn_lds + ((c + 1) & 1) * chunk_bytes,
+                       chunk_bytes, WAVES);
+        tile_mma<Op, NV, RS, 0, QT, NV>(reinterpret_cast<const V*>(cur), bq, acc);
+        const V* sA = reinterpret_cast<const V*>(cur);  // the chunk with its pad vectors, for the filter
+        constexpr int sRS = RS;
         const bool seeding = c < order.nseed;
 #pragma unroll
         for (int u = 0; u < QT; u++) {
@@ -807,6 +763,185 @@ __global__ __launch_bounds__(64 * WAVES) void knn_tiles_reg(const typename Op::V
             a.cnt_out[slot0 + sl[u]] = s_cnt[sl[u]];
             a.tau_out[slot0 + sl[u]] = s_tau[sl[u]];
         }
+    }
+}
+
+// ---------------------------------------------------------------- BF16, block-centred
+// One BF16 product per pair on values centred at the query block's mean m
+// (a block's 256 queries are consecutive rows of the locality order, so they
+// lie close together): candidate y~ = bf16(y - m), query q~ = bf16(q - m),
+// value v = ||y~||^2 - 2 q~.y~ = ||q~ - y~||^2 - ||q~||^2 (the image row holds
+// the D values, zeros, then ||y~||^2 as a hi/lo pair in its last two slots,
+// against query values of 1).  The rounding error
+// is relative to ||q - m|| and ||y - m|| instead of the largest row norm, so
+// one product certifies where values on the global scale need the hi/lo
+// split's three (LocalBound in the re-rank).  Candidate rows stream from HBM
+// as FP32 into registers one chunk ahead; each chunk is converted to its BF16
+// image (values, then the norm as a hi/lo pair against query values of 1) in
+// one of two LDS buffers while the other feeds the MFMAs.  Filtering, seeds
+// and compaction are knn_tiles_reg's.
+constexpr int kLocNVMax = 10;            // widest image row: 16 x 10 values >= D + 2 (D <= 158)
+constexpr int kLocWaves = 8;
+constexpr int kLocQB = 32 * kLocWaves;   // query rows per block (and per centre)
+
+inline int local_nv(int D) { return (D + 2 + 15) / 16; }  // D values + the norm pair
+inline int local_dp(int D) { return 16 * local_nv(D); }  // FP32 row stride: the image width, zero past D
+
+template <int NV>
+__global__ __launch_bounds__(64 * kLocWaves) void knn_tiles_local(const float* __restrict__ Xf, int D,
+                                                                 const float* __restrict__ centres, TileArgs a)
+{
+    constexpr int KP = 16 * NV;                     // values per image row = FP32 row stride
+    constexpr int RS = 2 * NV + 1;                  // image row stride in vectors (odd: LDS banks)
+    constexpr int VPT = (2 * NV + 3) / 4;           // image vectors a thread converts per chunk
+    extern __shared__ __align__(16) unsigned char dyn_lds[];
+    bf16x8* img = reinterpret_cast<bf16x8*>(dyn_lds);  // two kCB x RS images
+    __shared__ int s_cnt[kLocQB];
+    __shared__ float s_tau[kLocQB];
+    __shared__ __align__(16) float s_cen[16 * kLocNVMax + 32];  // the block's centre, zero past D
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, half = lane >> 5;
+    const int slot0 = blockIdx.x * kLocQB;
+    for (int i = tid; i < kLocQB; i += 64 * kLocWaves) {
+        s_cnt[i] = 0;
+        s_tau[i] = HUGE_VALF;
+    }
+    for (int i = tid; i < 16 * kLocNVMax + 32; i += 64 * kLocWaves)
+        s_cen[i] = i < KP ? centres[(size_t)blockIdx.x * KP + i] : 0.f;
+    __syncthreads();
+    // the query operand: -2 bf16(q - m) for the values, 1 against the norm pair
+    const int sl = 32 * w + (lane & 31);
+    const int q = query_row(a, slot0 + sl);
+    bf16x8 bq[1][NV];
+    {
+        const float* xr = Xf + (size_t)(q >= 0 ? q : 0) * KP;
+#pragma unroll
+        for (int g = 0; g < NV; g++) {
+#pragma unroll
+            for (int el = 0; el < 8; el++) {
+                const int p = (half * NV + g) * 8 + el;
+                float v = -2.f * (float)(__bf16)(xr[p] - s_cen[p]);  // 0 past D (both are)
+                v = p >= KP - 2 ? 1.f : v;                            // against the norm pair
+                bq[0][g][el] = (__bf16)(q >= 0 ? v : 0.f);
+            }
+        }
+    }
+    uint64_t* qbuf = a.buf + (size_t)(slot0 + sl) * kCap;
+    // conversion roles: row r of a chunk, image vectors pp, pp + 4, ...; a
+    // vector index past the row (2 NV not a multiple of 4) is clamped to the
+    // last vector and its store skipped
+    const int r = tid >> 2, pp = tid & 3;
+    int vis[VPT];
+#pragma unroll
+    for (int k = 0; k < VPT; k++) vis[k] = min(pp + 4 * k, 2 * NV - 1);
+    float4 pre[VPT][2];
+    auto fetch = [&](int chunk) {
+        const float4* xr = reinterpret_cast<const float4*>(Xf + ((size_t)chunk * kCB + r) * KP);
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            pre[k][0] = xr[2 * vis[k]];
+            pre[k][1] = xr[2 * vis[k] + 1];
+        }
+    };
+    auto convert = [&](bf16x8* dst) {
+        bf16x8 out[VPT];
+        float part = 0.f;
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            const float4 c0 = reinterpret_cast<const float4*>(s_cen)[2 * vis[k]];
+            const float4 c1 = reinterpret_cast<const float4*>(s_cen)[2 * vis[k] + 1];
+            const float d[8] = {pre[k][0].x - c0.x, pre[k][0].y - c0.y, pre[k][0].z - c0.z, pre[k][0].w - c0.w,
+                                pre[k][1].x - c1.x, pre[k][1].y - c1.y, pre[k][1].z - c1.z, pre[k][1].w - c1.w};
+            const bool dup = pp + 4 * k >= 2 * NV;  // clamped vector: counted once
+#pragma unroll
+            for (int el = 0; el < 8; el++) {
+                const __bf16 b = (__bf16)d[el];
+                const float f = (float)b;
+                part = dup ? part : fmaf(f, f, part);
+                out[k][el] = b;
+            }
+        }
+        part += __shfl_xor(part, 1, 64);  // the row's four parts, (p0 + p1) + (p2 + p3) on every lane
+        part += __shfl_xor(part, 2, 64);
+        const __bf16 nh = (__bf16)part;
+        const __bf16 nl = (__bf16)(part - (float)nh);
+#pragma unroll
+        for (int k = 0; k < VPT; k++) {
+            const bool last = vis[k] == 2 * NV - 1;  // the norm pair is the row's last two values
+            out[k][6] = last ? nh : out[k][6];
+            out[k][7] = last ? nl : out[k][7];
+            if (pp + 4 * k < 2 * NV) dst[r * RS + vis[k]] = out[k];
+        }
+    };
+    const int nch = (a.N + kCB - 1) / kCB;
+    const ChunkOrder order(seed_chunk(a, slot0, nch), nch);
+    fetch(order.at(0));
+    convert(img);
+    if (nch > 1) fetch(order.at(1));
+    __syncthreads();
+    for (int c = 0; c < nch; c++) {
+        const int cb = order.at(c) * kCB;
+        const bf16x8* cur = img + (size_t)(c & 1) * kCB * RS;
+        f32x16 acc[1][4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) acc[0][t] = f32x16{};
+        tile_mma<OpBF16, NV, RS, 0, 1, NV>(cur, bq, acc);
+        if (c + 1 < nch) {  // the next chunk's image (its buffer was last read before the previous barrier)
+            convert(img + (size_t)((c + 1) & 1) * kCB * RS);
+            if (c + 2 < nch) fetch(order.at(c + 2));
+        }
+        const bool seeding = c < order.nseed;
+        const float tau = s_tau[sl];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            float v[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) v[i] = acc[0][t][i];
+            if (seeding) {
+                if (q >= 0) seed_store(v, t, cb, q, c, a, qbuf);
+            } else {
+                filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
+            }
+        }
+        if (c == order.nseed - 1 && lane < 32 && q >= 0) s_cnt[sl] = order.nseed * kCB;
+        if (c >= order.nseed - 1) compact_if_needed<32>(w, a, slot0, s_cnt, s_tau);
+        __syncthreads();
+    }
+    if (lane < 32 && slot0 + sl < a.nq) {
+        a.cnt_out[slot0 + sl] = s_cnt[sl];
+        a.tau_out[slot0 + sl] = s_tau[sl];
+    }
+}
+
+// FP32 copy of the rows (Npad x Dp, zero padding) and every query block's
+// mean (FP64 sums, stored as FP32): the centres of knn_tiles_local.
+__global__ __launch_bounds__(256) void prep_local_rows(const double* __restrict__ X, int N, int Npad, int D, int Dp,
+                                                       float* __restrict__ Xf)
+{
+    const size_t total = (size_t)Npad * Dp;
+    for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (size_t)gridDim.x * 256) {
+        const int n = (int)(e / Dp), d = (int)(e % Dp);
+        Xf[e] = (n < N && d < D) ? (float)X[(size_t)n * D + d] : 0.f;
+    }
+}
+
+__global__ __launch_bounds__(256) void prep_local_centres(const double* __restrict__ X, int N, int D, int Dp,
+                                                          float* __restrict__ centres)
+{
+    __shared__ double sh[4][64];
+    const int r0 = blockIdx.x * kLocQB, r1 = min(N, r0 + kLocQB);
+    const int dl = threadIdx.x & 63, ph = threadIdx.x >> 6;  // column, row phase (every fourth row)
+    for (int d0 = 0; d0 < Dp; d0 += 64) {
+        const int d = d0 + dl;
+        double sum = 0.0;
+        if (d < D)
+            for (int r = r0 + ph; r < r1; r += 4) sum += X[(size_t)r * D + d];
+        sh[ph][dl] = sum;
+        __syncthreads();
+        if (ph == 0 && d < Dp) {
+            const double t = ((sh[0][dl] + sh[1][dl]) + sh[2][dl]) + sh[3][dl];
+            centres[(size_t)blockIdx.x * Dp + d] = d < D && r1 > r0 ? (float)(t / (r1 - r0)) : 0.f;
+        }
+        __syncthreads();
     }
 }
 
@@ -1107,7 +1242,51 @@ struct RerankArgs {
     int* idx;             // N x K (indexed by row)
     double* dist;
     int* fail;            // per slot: 1 if the bound failed
+    double* ubound;       // per slot: the K-th exact distance found (>= the true one), +inf if fewer than K
     const uint32_t* perm; // optional: X rows are the caller's rows perm[0..N) (outputs and ties use those)
+    // block-centred BF16 tiles (knn_tiles_local): per-row bound instead of eps_a, eps_b
+    const float* centres; // null: global bound
+    int cdim;             // centre stride (Dp)
+    int kp;               // values per image row (16 NV)
+};
+
+// Bounds on the exact squared distance d of a candidate from its value v in
+// knn_tiles_local, for a query q of centre m.  q~, y~ are the BF16 values
+// (FP32 copy, FP32 subtraction of m, round to nearest BF16: per value
+// |y~ - (y - m)| <= u |y - m| + 2^-24 |y|), a = ||q - m||, Q = ||q~||^2 and
+// e = ||q~ - (q - m)|| (both exact: the re-rank repeats the query's
+// rounding), s = ||q~ - y~||, w = 2^-24 Rmax:
+//   |sqrt(d) - s| <= u ||y - m|| + w + e,  ||y - m|| <= sqrt(d) + a;
+//   v = s^2 - Q + E,  |E| <= g (||y~||^2 + 2 sqrt(Q) ||y~||) (FP32
+//   accumulation of the KP products and of the norm, hi/lo split of the
+//   norm), ||y~|| <= s + sqrt(Q).
+// Solving these for s, then for sqrt(d), gives lower(v) <= d for every
+// candidate of value >= v, and upper(v) >= d for every candidate of value <= v.
+struct LocalBound {
+    double u, g, t, Q, qn;  // t: the additive part u a + w + e
+    TKA_DEV LocalBound(double a, double Q_, double e, double rmax, int kp, int D) : Q(Q_)
+    {
+        u = 0x1.01p-9;
+        g = (2.0 * kp + D + 4) * 0x1.0p-24 + 0x1.0p-17;
+        t = u * a + 0x1.0p-24 * rmax + e;
+        qn = sqrt(Q);
+    }
+    TKA_DEV double lower(double v) const
+    {
+        const double R = v + Q - 3.0 * g * Q;
+        if (!(R > 0.0)) return 0.0;
+        const double sl = (-4.0 * g * qn + sqrt(16.0 * g * g * Q + 4.0 * (1.0 + g) * R)) / (2.0 * (1.0 + g));
+        const double r = (sl - t) / (1.0 + u);
+        return r > 0.0 ? r * r * (1.0 - 1e-12) : 0.0;
+    }
+    TKA_DEV double upper(double v) const
+    {
+        double R = v + Q + 3.0 * g * Q;
+        R = R > 0.0 ? R : 0.0;
+        const double sh = (4.0 * g * qn + sqrt(16.0 * g * g * Q + 4.0 * (1.0 - g) * R)) / (2.0 * (1.0 - g));
+        const double r = (sh + t) / (1.0 - u);
+        return r * r * (1.0 + 1e-12);
+    }
 };
 
 
@@ -1260,7 +1439,8 @@ TKA_DEV int block_excl_scan(int v, int* sh, int* total)
 // (bits, index).  One workgroup per row; K <= kLargeKMax.  Ties are broken
 // by row index (X in the caller's order).
 __global__ __launch_bounds__(256) void select_k_rows(const double* __restrict__ dist, int N, int B, int K,
-                                                     const int* __restrict__ qlist, int q0, int* __restrict__ idx,
+                                                     const int* __restrict__ qlist, int q0,
+                                                     const double* __restrict__ ubound, int* __restrict__ idx,
                                                      double* __restrict__ dout)
 {
     __shared__ uint64_t skey[kLargeKMax];
@@ -1271,6 +1451,31 @@ __global__ __launch_bounds__(256) void select_k_rows(const double* __restrict__ 
     const int r = blockIdx.x;
     if (r >= B) return;
     const uint64_t* row = reinterpret_cast<const uint64_t*>(dist + (size_t)r * N);
+    // With an upper bound U on the row's K-th distance (the K-th exact
+    // distance of candidates already seen), the K smallest are among the
+    // distances <= U: gathered in one pass and sorted, when they fit.
+    int nsort = K;
+    bool bounded = false;
+    if (ubound != nullptr && isfinite(ubound[q0 + r])) {
+        const uint64_t U = (uint64_t)__double_as_longlong(ubound[q0 + r]);
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        for (int c = threadIdx.x; c < N; c += 256) {
+            const uint64_t key = row[c];
+            if (key <= U) {
+                const int pos = atomicAdd(&s_cnt, 1);
+                if (pos < kLargeKMax) {
+                    skey[pos] = key;
+                    sidx[pos] = c;
+                }
+            }
+        }
+        __syncthreads();
+        bounded = s_cnt >= K && s_cnt <= kLargeKMax;
+        nsort = bounded ? s_cnt : K;
+        __syncthreads();
+    }
+    if (!bounded) {
     uint64_t prefix = 0ull, pmask = 0ull;
     int k = K;  // rank (1-based) of the wanted key among the keys that match the prefix
     for (int round = 0; round < 6; round++) {
@@ -1331,11 +1536,12 @@ __global__ __launch_bounds__(256) void select_k_rows(const double* __restrict__ 
         }
         eq_taken += tot;
     }
+    }  // radix select
     __syncthreads();
-    // bitonic sort of (key, index) over the next power of two >= K, padded with +inf
+    // bitonic sort of (key, index) over the next power of two >= nsort, padded with +inf
     int P2 = 1;
-    while (P2 < K) P2 <<= 1;
-    for (int i = K + threadIdx.x; i < P2; i += 256) {
+    while (P2 < nsort) P2 <<= 1;
+    for (int i = nsort + threadIdx.x; i < P2; i += 256) {
         skey[i] = ~0ull;
         sidx[i] = INT_MAX;
     }
@@ -1386,6 +1592,22 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     const double rmax = *a.rmax;
     const double eps = a.eps_a * sqrt(qn2) * rmax + a.eps_b * rmax * rmax + 1e-12 * (qn2 + rmax * rmax);
     const int K = a.K;
+    const bool local = a.centres != nullptr;
+    double ac = 0.0, Q = 0.0, eq = 0.0;  // ||q - m||, ||q~||^2, ||q~ - (q - m)|| for the block-centred tiles
+    if (local) {
+        const float* m = a.centres + (size_t)(slot / kLocQB) * a.cdim;
+        for (int d = lane; d < a.D; d += 64) {
+            const double t = xq[d] - (double)m[d];
+            ac += t * t;
+            const double b = (double)(float)(__bf16)((float)xq[d] - m[d]);  // the tiles' rounding
+            Q += b * b;
+            eq += (b - t) * (b - t);
+        }
+        ac = sqrt(wave_sum(ac)) * (1.0 + 1e-12);
+        Q = wave_sum(Q);
+        eq = sqrt(wave_sum(eq)) * (1.0 + 1e-12) + 1e-15 * ac;  // t carries one FP64 rounding per value
+    }
+    const LocalBound lb(ac, Q, eq, rmax, a.kp, a.D);
 
     __shared__ uint64_t s_list[4][kCap];
     const int w = threadIdx.x >> 6;
@@ -1408,12 +1630,14 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
     foreign = __ballot(foreign) != 0ull;
     // v_(K) exactly (radix select), then the candidates within thr, packed
     const uint32_t vk_bits = wave_select<8>(u, K, 4);
-    const double thr = c >= K ? (double)ord2f(vk_bits) + 2.0 * eps : HUGE_VAL;
+    const double thr = c >= K ? (local ? lb.upper((double)ord2f(vk_bits)) : (double)ord2f(vk_bits) + 2.0 * eps)
+                              : HUGE_VAL;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     int m = 0;
 #pragma unroll
     for (int r = 0; r < 8; r++) {
-        const bool in = u[r] != ~0ull && (double)ord2f((uint32_t)(u[r] >> 32)) <= thr;
+        const double vr = (double)ord2f((uint32_t)(u[r] >> 32));
+        const bool in = u[r] != ~0ull && (local ? lb.lower(vr) : vr) <= thr;
         const uint64_t im = __ballot(in);
         if (in) s_list[w][m + __popcll(im & lt)] = u[r];
         m += __popcll(im);
@@ -1453,17 +1677,23 @@ __global__ __launch_bounds__(256) void knn_rerank(RerankArgs a)
         if (c < K || foreign) {
             bad = 1;  // cannot happen with K <= N-1 unless values were non-finite
         } else if (isfinite(tau)) {
-            bad = !(dk < (double)tau + qn2 - eps);
+            bad = local ? !(dk < lb.lower((double)tau)) : !(dk < (double)tau + qn2 - eps);
         }
         a.fail[slot] = bad;
+        a.ubound[slot] = c >= K && !foreign ? dk : HUGE_VAL;
     }
 }
 
-__global__ void collect_failures(const int* __restrict__ fail, int n, const uint32_t* __restrict__ perm,
-                                 int* __restrict__ list, int* __restrict__ count)
+__global__ void collect_failures(const int* __restrict__ fail, const double* __restrict__ ubound, int n,
+                                 const uint32_t* __restrict__ perm, int* __restrict__ list, double* __restrict__ ulist,
+                                 int* __restrict__ count)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && fail[i]) list[atomicAdd(count, 1)] = perm ? (int)perm[i] : i;
+    if (i < n && fail[i]) {
+        const int p = atomicAdd(count, 1);
+        list[p] = perm ? (int)perm[i] : i;
+        ulist[p] = ubound[i];
+    }
 }
 
 // ---------------------------------------------------------------- locality order
@@ -1728,8 +1958,7 @@ template <class Op, int kMode, int NV, int QT, int WAVES>
 void launch_tiles_reg(const void* A, const void* B, dim3 grid, size_t lds, const TileArgs& ta, hipStream_t s)
 {
     typedef typename Op::V V;
-    constexpr int NV1 = NV <= kNVLimit ? NV : (NV + 1) / 2;  // split_nv1
-    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, NV, QT, WAVES, NV1>), grid, dim3(64 * WAVES), lds, s, (const V*)A,
+    hipLaunchKernelGGL((knn_tiles_reg<Op, kMode, NV, QT, WAVES>), grid, dim3(64 * WAVES), lds, s, (const V*)A,
                        (const V*)B, ta);
 }
 
@@ -1738,9 +1967,7 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, cons
                    MemBudget* mb, hipStream_t s)
 {
     typedef typename Op::V V;
-    constexpr bool kSplits = std::is_same<Op, OpBF16>::value;  // two-stage rows: BF16 only
-    require(NV >= 1 && NV <= (kSplits ? kNVSplitLimit : kNVLimit), TKA_E_INTERNAL,
-            "register tiles: bad operand width");
+    require(NV >= 1 && NV <= kNVLimit, TKA_E_INTERNAL, "register tiles: bad operand width");
     constexpr int QB = 32 * WAVES * QT;
     const int nslots = (int)ceil_div(nq, QB) * QB;
     tr.buf.alloc((size_t)nslots * kCap, mb);
@@ -1748,7 +1975,7 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, cons
     tr.tau.alloc(nslots, mb);
     tr.fail.alloc(nslots, mb);
     TileArgs ta{N, kprime, 1, qlist, nq, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
-    const size_t lds = 2 * (size_t)kCB * (2 * split_nv1(NV) + 1) * sizeof(V);
+    const size_t lds = 2 * (size_t)kCB * (2 * NV + 1) * sizeof(V);
     const dim3 grid(ceil_div(nq, QB));
 #define TKA_W(W) \
     case W: launch_tiles_reg<Op, kMode, W, QT, WAVES>(A, B, grid, lds, ta, s); break
@@ -1757,12 +1984,43 @@ void run_tiles_reg(const void* A, const void* B, int NV, int N, int kprime, cons
         TKA_W(11); TKA_W(12); TKA_W(13); TKA_W(14); TKA_W(15); TKA_W(16); TKA_W(17); TKA_W(18); TKA_W(19);
         TKA_W(20);
     }
-    if constexpr (kSplits) {
-        switch (NV) {
-            TKA_W(21); TKA_W(22); TKA_W(23); TKA_W(24); TKA_W(25); TKA_W(26); TKA_W(27); TKA_W(28);
-        }
-    }
 #undef TKA_W
+    TKA_LAUNCH_CHECK();
+}
+
+// All rows against all rows on the block-centred BF16 tiles (knn_tiles_local):
+// Xs N x D FP64 rows in locality order.  Fills the centres (one per block of
+// kLocQB query rows, stride Dp) for the re-rank's bound.
+void run_tiles_local(const double* Xs, int N, int D, int kprime, TileRun& tr, DevBuf<float>& centres, MemBudget* mb,
+                     hipStream_t s)
+{
+    const int NV = local_nv(D), Dp = local_dp(D);
+    require(NV >= 1 && NV <= kLocNVMax, TKA_E_INTERNAL, "block-centred tiles: bad operand width");
+    const int Npad = (int)ceil_div(N, kCB) * kCB;
+    const int nb = (int)ceil_div(N, kLocQB);
+    DevBuf<float> Xf((size_t)Npad * Dp, mb);
+    centres.alloc((size_t)nb * Dp, mb);
+    hipLaunchKernelGGL(prep_local_rows, dim3(std::min<size_t>(8192, ceil_div((size_t)Npad * Dp, 256))), dim3(256), 0,
+                       s, Xs, N, Npad, D, Dp, Xf.get());
+    TKA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(prep_local_centres, dim3(nb), dim3(256), 0, s, Xs, N, D, Dp, centres.get());
+    TKA_LAUNCH_CHECK();
+    const int nslots = nb * kLocQB;
+    tr.buf.alloc((size_t)nslots * kCap, mb);
+    tr.cnt.alloc(nslots, mb);
+    tr.tau.alloc(nslots, mb);
+    tr.fail.alloc(nslots, mb);
+    TileArgs ta{N, kprime, 1, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get()};
+    const size_t lds = 2 * (size_t)kCB * (2 * NV + 1) * sizeof(bf16x8);
+#define TKA_L(W)                                                                                                  \
+    case W:                                                                                                       \
+        hipLaunchKernelGGL(knn_tiles_local<W>, dim3(nb), dim3(64 * kLocWaves), lds, s, Xf.get(), D, centres.get(), \
+                           ta);                                                                                   \
+        break
+    switch (NV) {
+        TKA_L(1); TKA_L(2); TKA_L(3); TKA_L(4); TKA_L(5); TKA_L(6); TKA_L(7); TKA_L(8); TKA_L(9); TKA_L(10);
+    }
+#undef TKA_L
     TKA_LAUNCH_CHECK();
 }
 
@@ -1808,9 +2066,10 @@ void run_tiles_wide(const void* A, const void* B, int NV, int N, int kprime, con
 // Every distance in FP64, selected exactly per row (X in the caller's
 // order, so ties go by the caller's index): for K beyond the candidate
 // buffers (all rows), and to re-solve the rows whose certificate failed
-// (qlist, nq rows).  Query rows in batches of at most 512 MB of distances.
+// (qlist, nq rows, with ubound[i] >= their K-th distance).  Query rows in
+// batches of at most 512 MB of distances.
 static void knn_exact_rows(const double* Xn, int N, int D, int K, const int* qlist, int nq, int* idx, double* dist,
-                           MemBudget* mb, hipStream_t s)
+                           MemBudget* mb, hipStream_t s, const double* ubound = nullptr)
 {
     require(K <= kLargeKMax, TKA_E_WRONG_PARAMETER,
             "K = " + std::to_string(K) + " exceeds the engine's limit of " + std::to_string(kLargeKMax) +
@@ -1824,8 +2083,8 @@ static void knn_exact_rows(const double* Xn, int N, int D, int K, const int* qli
         hipLaunchKernelGGL(pair_sqdist_rows, dim3(ceil_div(N, kPairTile), ceil_div(b, kPairTile)), dim3(256), 0, s,
                            Xn, N, D, qlist, q0, b, buf.get());
         TKA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(select_k_rows, dim3(b), dim3(256), 0, s, (const double*)buf.get(), N, b, K, qlist, q0, idx,
-                           dist);
+        hipLaunchKernelGGL(select_k_rows, dim3(b), dim3(256), 0, s, (const double*)buf.get(), N, b, K, qlist, q0,
+                           ubound, idx, dist);
         TKA_LAUNCH_CHECK();
     }
 }
@@ -1840,9 +2099,15 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         knn_exact_rows(Xn, N, D, K, nullptr, N, idx, dist, mb, s);
         return;
     }
-    // default candidates per row: 1.5 K for the certified tiles (rows the
-    // bound cannot certify are re-solved exactly), 2 K for FP8 (not re-solved)
-    if (kprime <= 0) kprime = dtype == TKA_DIST_FP8 ? 2 * K : std::max(K + (K + 1) / 2, K + 16);
+    // default candidates per row: 1.5 K for the FP32 and split-BF16 tiles,
+    // 5/3 K for the block-centred BF16 tiles (their bound grows with a row's
+    // distance to its block's centre), rows the bound cannot certify being
+    // re-solved exactly; 2 K for FP8 (not re-solved)
+    const bool centred = dtype == TKA_DIST_BF16 && local_nv(D) <= kLocNVMax;
+    if (kprime <= 0)
+        kprime = dtype == TKA_DIST_FP8 ? 2 * K
+                 : centred            ? std::max(K + (2 * K + 2) / 3, K + 16)
+                                      : std::max(K + (K + 1) / 2, K + 16);
     kprime = std::max(kprime, K);
     kprime = std::min(kprime, kMaxKprime);
     kprime = std::min(kprime, N - 1);
@@ -1941,35 +2206,27 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         }
         eps_b = eps_a;
     };
-    if (dtype == TKA_DIST_BF16) {
+    DevBuf<float> centres;  // block-centred BF16 tiles
+    if (centred) {
+        locality();
+        Kp = 16 * local_nv(D);
+        run_tiles_local(Xs, N, D, kprime, tr, centres, mb, s);
+    } else if (dtype == TKA_DIST_BF16) {
+        // rows too wide for the centred tiles' registers: the hi/lo splits on
+        // the global scale, both operands streamed in k-stages (NV padded
+        // with zero vectors to whole stages)
         Kp = ((3 * D + 2) + 15) / 16 * 16;
-        const int NV = Kp / 16;
-        int nsum = Kp;  // roundings in one value's FP32 sum
-        if (reg_tiles_fit(NV, sizeof(bf16x8), true)) {
-            locality();
-            const size_t rows = (size_t)Npad * 8 * (2 * NV + 1);
-            A.alloc(rows, mb);
-            B.alloc(rows, mb);
-            hipLaunchKernelGGL(prep_bf16x3_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NV, A.get(), B.get());
-            TKA_LAUNCH_CHECK();
-            run_tiles_reg<OpBF16, 0, 1, 8>(A.get(), B.get(), NV, N, kprime, nullptr, N, tr, mb, s);
-        } else {
-            // both operands streamed in k-stages (the same values, NV padded
-            // with zero vectors to whole stages)
-            locality();
-            const int NVw = wide_nv(NV);
-            const size_t rows = (size_t)Npad * 8 * (2 * NVw + 1);
-            A.alloc(rows, mb);
-            B.alloc(rows, mb);
-            hipLaunchKernelGGL(prep_bf16x3_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NVw, A.get(),
-                               B.get());
-            TKA_LAUNCH_CHECK();
-            run_tiles_wide<OpBF16>(A.get(), B.get(), NVw, N, kprime, nullptr, N, tr, mb, s);
-            // each stage's 16 kWideKS products are summed from zero, then
-            // added to the running sum: at most that many roundings plus one
-            // per stage
-            nsum = 16 * kWideKS + NVw / kWideKS;
-        }
+        const int NVw = wide_nv(Kp / 16);
+        locality();
+        const size_t rows = (size_t)Npad * 8 * (2 * NVw + 1);
+        A.alloc(rows, mb);
+        B.alloc(rows, mb);
+        hipLaunchKernelGGL(prep_bf16x3_hm, dim3(gprep), dim3(256), 0, s, Xs, ns, N, Npad, D, NVw, A.get(), B.get());
+        TKA_LAUNCH_CHECK();
+        run_tiles_wide<OpBF16>(A.get(), B.get(), NVw, N, kprime, nullptr, N, tr, mb, s);
+        // each stage's 16 kWideKS products are summed from zero, then added to
+        // the running sum: at most that many roundings plus one per stage
+        const int nsum = 16 * kWideKS + NVw / kWideKS;
         const double ub2 = std::ldexp(1.0, -16);
         const double gam = nsum * u32;
         eps_a = 2.0 * (6.2 * ub2 + 2.1 * gam);
@@ -2007,8 +2264,10 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
         TKA_HIP(hipMemcpyAsync(order_out, perm.get(), (size_t)N * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
         if (st) st->order_written = true;
     }
+    DevBuf<double> ubound(N, mb), ulist(N, mb);
     RerankArgs ra{Xs, ns ? ns : n2.get(), N, D, K, nullptr, N, tr.buf.get(), tr.cnt.get(), tr.tau.get(),
-                  eps_a, eps_b, rmaxd.get(), idx, dist, tr.fail.get(), perm.get()};
+                  eps_a, eps_b, rmaxd.get(), idx, dist, tr.fail.get(), ubound.get(), perm.get(),
+                  centres.get(), local_dp(D), Kp};
     hipLaunchKernelGGL(knn_rerank, dim3(ceil_div(N, 4)), dim3(256), 0, s, ra);
     TKA_LAUNCH_CHECK();
 
@@ -2017,8 +2276,8 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     // by the caller's index.  Their cost is the rows' share of the N^2 D
     // work, where a tile pass costs a whole stream of the candidates.
     DevBuf<int> list(N, mb);
-    hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), N, perm.get(),
-                       list.get(), status.get());
+    hipLaunchKernelGGL(collect_failures, dim3(ceil_div(N, 256)), dim3(256), 0, s, tr.fail.get(), ubound.get(), N,
+                       perm.get(), list.get(), ulist.get(), status.get());
     TKA_LAUNCH_CHECK();
     int hs[2] = {0, 0};
     read_to_host(hs, status.get(), sizeof hs, s);
@@ -2026,7 +2285,7 @@ void knn_device(const double* Xn, int N, int D, int K, int dtype, int kprime, in
     const int hf = hs[0];
     if (st) st->certificate_failures = hf;
     if (st) st->unresolved = dtype == TKA_DIST_FP8 ? hf : 0;
-    if (hf > 0 && dtype != TKA_DIST_FP8) knn_exact_rows(Xn, N, D, K, list.get(), hf, idx, dist, mb, s);
+    if (hf > 0 && dtype != TKA_DIST_FP8) knn_exact_rows(Xn, N, D, K, list.get(), hf, idx, dist, mb, s, ulist.get());
 }
 
 }  // namespace tka

@@ -29,9 +29,11 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def check_knn_p_sample(engine, oracle, Xn, K, perp, rows, dtypes):
+def check_knn_p_sample(engine, oracle, Xn, K, perp, rows, dtypes, max_resolved=0):
     """Sampled rows of the kNN (each dtype) and of the conditional P against
-    the oracle; returns the engine's FP32-tile idx, dist and the FP8 recall."""
+    the oracle; returns the engine's FP32-tile idx, dist and the FP8 recall.
+    At most max_resolved rows may fail their certificate (they are re-solved
+    with every distance in FP64; every row is still compared with FP32's)."""
     print(f"[config] oracle brute force on {len(rows)} query rows of {Xn.shape}", flush=True)
     ridx, rdist = oracle.knn_brute_rows(Xn, K, rows)
     out = {}
@@ -41,7 +43,9 @@ def check_knn_p_sample(engine, oracle, Xn, K, perp, rows, dtypes):
         out[dtype] = (idx, dist, fails)
         if dtype == "fp8":
             continue   # approximate mode: rows failing the bound are counted, not re-solved
-        assert fails == 0, f"{dtype}: {fails} rows not certified"
+        limit = max_resolved if dtype == "bf16" else 0
+        print(f"[config] {dtype}: {fails} rows not certified (re-solved exactly)", flush=True)
+        assert fails <= limit, f"{dtype}: {fails} rows not certified"
         np.testing.assert_array_equal(idx[rows], ridx)
         np.testing.assert_array_equal(dist[rows], rdist)
     idx, dist, _ = out["fp32"]
@@ -152,7 +156,10 @@ def test_config4_precision_sweep(engine, oracle):
     Xn = engine.normalize(X)
     np.testing.assert_array_equal(Xn, oracle.normalize(X))
     rows = np.arange(7, X.shape[0], 1563)  # 160 query rows
-    idx, dist, val, recall = check_knn_p_sample(engine, oracle, Xn, 90, 30.0, rows, ("fp32", "bf16", "fp8"))
+    # the block-centred BF16 tiles leave a few high-dimensional rows (50
+    # overlapping clusters in 128-D) to the FP64 re-solve: at most 0.05%
+    idx, dist, val, recall = check_knn_p_sample(engine, oracle, Xn, 90, 30.0, rows, ("fp32", "bf16", "fp8"),
+                                                max_resolved=Xn.shape[0] // 2000)
     assert recall > 0.9
     check_symmetric_p(engine, X, idx, val, 30.0, 90)
 

@@ -36,11 +36,11 @@ def test_normalize_bit_exact(engine, oracle, shape):
 
 # ---------------------------------------------------------------- kNN
 @pytest.mark.parametrize("dtype", ["fp32", "bf16"])
-# d = 128: FP32 operand rows of 17 vectors (D+1 k-values), BF16 rows of 25
-# vectors (3D+2 k-values) streamed in two k-stages; d = 93: BF16 rows of 18
-# vectors; d = 107 and 148: the narrowest (21) and widest (28) two-stage BF16
-# rows; d = 150 (BF16 29 vectors, padded to 32), 333 (BF16 63 -> 64, FP32
-# 42 -> 44) and 784: rows streamed through LDS in k-stages of 4 vectors
+# BF16 up to d = 158 runs on the block-centred tiles (d + 2 values per image
+# row in vectors of 16: d = 3 -> 1, 20 -> 2, 50 -> 4, 93 -> 6, 107 -> 7,
+# 128 -> 9, 148 and 150 -> 10, the widest); beyond, on hi/lo splits streamed
+# in k-stages (333, 784).  FP32: d = 128 gives rows of 17 vectors (D+1
+# k-values); 333 (42 -> 44) and 784 stream through LDS in k-stages
 @pytest.mark.parametrize("n,d,K", [(700, 20, 30), (1500, 50, 90), (300, 784, 20), (129, 3, 128), (600, 128, 40),
                                    (500, 93, 30), (400, 107, 25), (350, 148, 20), (450, 150, 20), (260, 333, 30)])
 def test_knn_bit_exact(engine, oracle, dtype, n, d, K):
