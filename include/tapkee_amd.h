@@ -85,7 +85,8 @@ typedef struct tka_tsne_options {
     double exaggeration;      /* 12   */
     int32_t knn_k;            /* 0 -> (int)(3 * perplexity) */
     int32_t dist_dtype;       /* tka_dist_dtype, default BF16 */
-    int32_t candidates;       /* k' candidates kept per row before re-rank; 0 -> 2K */
+    int32_t candidates;       /* k' candidates kept per row before re-rank; 0 -> 1.5 K (FP32 tiles), 5/3 K
+                                 (BF16 tiles centred per query block, D <= 158), 1.5 K (BF16 beyond), 2 K (FP8) */
     int32_t loop_precision;   /* tka_precision of the gradient loop, default FP32 */
     int32_t exact_p_mode;     /* tka_exact_p_mode, default DENSE (as the reference) */
     int32_t dd_mode;          /* tka_dd_mode, default TEXTBOOK */
@@ -200,8 +201,9 @@ TKA_API tka_status tka_session_create(const double* X, int32_t N, int32_t D, int
  *    row_ptr[N] entries), used after division by its sum (tsne.hpp:119-124);
  *    it should be symmetric, as the reference's is.
  * Both give a sparse P also at theta == 0 (exact repulsion, CSR attraction).
- * Indices, distances and values are checked on the host (out of range,
- * negative or non-finite -> TKA_E_WRONG_PARAMETER).  Y0 as in the options;
+ * Indices, distances and values are checked on the host (out of range, the
+ * object itself or a neighbour listed twice in a row, negative or non-finite
+ * -> TKA_E_WRONG_PARAMETER).  Y0 as in the options;
  * TKA_Y0_RAND_COMPAT draws from std::rand() with no VP-tree draws before. */
 TKA_API tka_status tka_session_create_knn(const int32_t* idx, const double* dist, int32_t N, int32_t K, int32_t no_dims,
                                           double perplexity, double theta, const tka_tsne_options* opt,
@@ -235,7 +237,8 @@ TKA_API tka_status tka_normalize(double* X, int32_t N, int32_t D);
 /* Brute-force K nearest neighbours on squared Euclidean distance, self
  * excluded, rows sorted by (FP64 distance, index).  Replaces the VP-tree
  * search (tsne.hpp:628-646, vptree.hpp:126-293) with an exact answer.
- * candidates = k' kept from the low-precision tiles (0 -> 2K).
+ * candidates = k' kept from the low-precision tiles (0 -> the default of
+ * tka_tsne_options.candidates).
  * certificate_failures (may be NULL) counts rows whose error-bound test
  * failed and were re-solved in FP64. */
 TKA_API tka_status tka_knn(const double* X, int32_t N, int32_t D, int32_t K, int32_t dist_dtype,
