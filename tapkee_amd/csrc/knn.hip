@@ -663,6 +663,32 @@ TKA_DEV void tile_mma(const typename Op::V* sA, const typename Op::V (&bq)[QT][N
     }
 }
 
+// The same for the two candidate tiles T0, T0 + 1 of the chunk and one
+// query tile: half the accumulators, so a kernel can hold four waves per SIMD.
+template <class Op, int NVS, int RSS, int NV, int T0>
+TKA_DEV void tile_mma2(const typename Op::V* sA, const typename Op::V (&bq)[1][NV], f32x16 (&acc)[2])
+{
+    typedef typename Op::V V;
+    const int lane = threadIdx.x & 63, half = lane >> 5;
+    const V* rowp = sA + (lane & 31) * RSS + half * NVS + 32 * T0 * RSS;
+    V af[2], an[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) af[t] = rowp[32 * t * RSS];
+#pragma unroll
+    for (int g = 0; g < NVS; g++) {
+        if (g + 1 < NVS) {
+#pragma unroll
+            for (int t = 0; t < 2; t++) an[t] = rowp[32 * t * RSS + g + 1];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 2; t++) acc[t] = Op::mma(af[t], bq[0][g], acc[t]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 2; t++) af[t] = an[t];
+    }
+}
+
 // kMode 0: the value is the accumulator (BF16: ||c||^2 folded into the operands)
 //       1: ||c||^2 - 2 acc (FP32)
 //       2: ||c||^2 - 2 s_q s_c acc (FP8 with per-row scales)
@@ -787,9 +813,11 @@ constexpr int kLocQB = 32 * kLocWaves;   // query rows per block (and per centre
 inline int local_nv(int D) { return (D + 2 + 15) / 16; }  // D values + the norm pair
 inline int local_dp(int D) { return 16 * local_nv(D); }  // FP32 row stride: the image width, zero past D
 
+// Up to D = 62 (NV <= 4) the kernel fits 128 VGPRs: four waves per SIMD, so
+// two blocks share a CU and 391 blocks at N=100k run in one round.
 template <int NV>
-__global__ __launch_bounds__(64 * kLocWaves) void knn_tiles_local(const float* __restrict__ Xf, int D,
-                                                                 const float* __restrict__ centres, TileArgs a)
+__global__ __launch_bounds__(64 * kLocWaves) __attribute__((amdgpu_waves_per_eu(NV <= 4 ? 4 : 1)))
+void knn_tiles_local(const float* __restrict__ Xf, int D, const float* __restrict__ centres, TileArgs a)
 {
     constexpr int KP = 16 * NV;                     // values per image row = FP32 row stride
     constexpr int RS = 2 * NV + 1;                  // image row stride in vectors (odd: LDS banks)
@@ -881,26 +909,36 @@ __global__ __launch_bounds__(64 * kLocWaves) void knn_tiles_local(const float* _
     for (int c = 0; c < nch; c++) {
         const int cb = order.at(c) * kCB;
         const bf16x8* cur = img + (size_t)(c & 1) * kCB * RS;
-        f32x16 acc[1][4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) acc[0][t] = f32x16{};
-        tile_mma<OpBF16, NV, RS, 0, 1, NV>(cur, bq, acc);
-        if (c + 1 < nch) {  // the next chunk's image (its buffer was last read before the previous barrier)
-            convert(img + (size_t)((c + 1) & 1) * kCB * RS);
-            if (c + 2 < nch) fetch(order.at(c + 2));
-        }
         const bool seeding = c < order.nseed;
         const float tau = s_tau[sl];
+        // the chunk's four candidate tiles in two halves of two (half the
+        // accumulators: four waves per SIMD)
+        auto filter2 = [&](const f32x16 (&acc)[2], int t0) {
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            float v[16];
+            for (int t = 0; t < 2; t++) {
+                float v[16];
 #pragma unroll
-            for (int i = 0; i < 16; i++) v[i] = acc[0][t][i];
-            if (seeding) {
-                if (q >= 0) seed_store(v, t, cb, q, c, a, qbuf);
-            } else {
-                filter_tile(v, t, cb, q, sl, tau, a, s_cnt, qbuf);
+                for (int i = 0; i < 16; i++) v[i] = acc[t][i];
+                if (seeding) {
+                    if (q >= 0) seed_store(v, t0 + t, cb, q, c, a, qbuf);
+                } else {
+                    filter_tile(v, t0 + t, cb, q, sl, tau, a, s_cnt, qbuf);
+                }
             }
+        };
+        {
+            f32x16 acc[2] = {f32x16{}, f32x16{}};
+            tile_mma2<OpBF16, NV, RS, NV, 0>(cur, bq, acc);
+            if (c + 1 < nch) {  // the next chunk's image (its buffer was last read before the previous barrier)
+                convert(img + (size_t)((c + 1) & 1) * kCB * RS);
+                if (c + 2 < nch) fetch(order.at(c + 2));
+            }
+            filter2(acc, 0);
+        }
+        {
+            f32x16 acc[2] = {f32x16{}, f32x16{}};
+            tile_mma2<OpBF16, NV, RS, NV, 2>(cur, bq, acc);
+            filter2(acc, 2);
         }
         if (c == order.nseed - 1 && lane < 32 && q >= 0) s_cnt[sl] = order.nseed * kCB;
         if (c >= order.nseed - 1) compact_if_needed<32>(w, a, slot0, s_cnt, s_tau);
