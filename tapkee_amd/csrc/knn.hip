@@ -984,8 +984,9 @@ __global__ __launch_bounds__(256) void prep_local_centres(const double* __restri
 }
 
 // ---------------------------------------------------------------- wide rows
-// Rows too wide to keep a query operand in registers (BF16 beyond 28 vectors
-// per half row, FP32 beyond 20: D above about 146 and 77): both operands are
+// Rows too wide to keep a query operand in registers (BF16 beyond the centred
+// tiles' D = 158, on the hi/lo splits; FP32 beyond 20 vectors per half row,
+// D above 77): both operands are
 // streamed through LDS in k-stages of KS vectors per half row.  The operands
 // are stage-major (hm_to_stages): stage st of row r is the 2 KS vectors
 // (h, st KS + g), h = 0, 1, g < KS, at ((st Npad) + r) 2 KS, so a stage of a

@@ -170,7 +170,8 @@ def test_symmetrize_matches_reference_as_sets(engine, oracle):
 
 
 def test_symmetrize_many_tiles(engine, oracle):
-    # 2*N*K = 3.6M edge keys: thousands of radix-sort tiles
+    # N*K = 1.8M edges of a random graph (few mutual pairs): 1.8M one-way
+    # edges radix-sorted and merged into 20k rows
     g = rng(5)
     N, K = 20_000, 90
     idx = np.stack([g.choice(N - 1, K, replace=False) for _ in range(N)]).astype(np.int64)
