@@ -367,7 +367,10 @@ class Session:
             self._h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module's globals may be cleared before the
+        # session goes; the process is ending and frees the device anyway
+        if _lib is not None:
+            self.close()
 
     def __enter__(self):
         return self

@@ -16,14 +16,15 @@
 //          ceil(t * 2^32) - 1, so coordinates on a split line go west/north;
 //          its top 16 bits equal ceil(t * 2^16) - 1, so the key's high word
 //          is the 16+16 bit key of the level-16 cell.
-//   sort   stable LSD radix sort of (top 8*passes key bits, point): 3
-//          passes (levels 1-12), or 4 when the last build saw many sorted
-//          neighbours with equal top 24 bits, and always beyond 2^22 points.
-//   fix    runs of equal top bits (points within 2^-12 of the box of each
-//          other, a few at most in t-SNE embeddings) are put in full-key
-//          order: short runs by a stable insertion sort, long ones by a
-//          wave-wide bitonic sort of (key, point); the result equals a stable
-//          sort of the 64-bit keys.
+//   sort   stable LSD radix sort of (top 8*passes key bits, point): 2
+//          passes (levels 1-8) while the last build had no run of more than
+//          64 keys with equal top 16 bits, 3 (levels 1-12) or 4 when runs
+//          are longer, and always 4 beyond 2^22 points.
+//   fix    runs of equal top bits (points in one level-8 cell, a few dozen
+//          at most in t-SNE embeddings of 1e5 points) are put in full-key
+//          order: runs of at most 64 points by each point's rank in the run,
+//          longer ones by a wave-wide bitonic sort of (key, point); the
+//          result equals a stable sort of the 64-bit keys.
 //   pairs  adjacent sorted keys i, i+1 first differ at quadtree level
 //          l = clz64(k_i ^ k_i+1)/2; the level-l cell holding both has >= 2
 //          non-empty children, i.e. it is a node of the compressed tree.
@@ -185,9 +186,13 @@ __global__ __launch_bounds__(kRsBlock) void bh_keys(T* __restrict__ Y, int N, co
 
 // Full 64-bit keys in sorted order, the final point order and the sorted
 // coordinates.  The radix sort ordered the points by the top bits of their
-// keys, keeping equal top bits in point order; a run of equal top bits is
-// then sorted by (full key, point), which equals a stable sort by full key:
-//  - a run of at most kShortRun points by its first thread (insertion sort);
+// keys, keeping equal top bits in point order; each run of equal top bits is
+// then put in (full key, point) order, which equals a stable sort by full key:
+//  - a run of at most kFixHalo points: each of its points goes to the run's
+//    start plus its rank among the run's (key, point) pairs, counted in LDS.
+//    A workgroup stages its 256 positions and kFixHalo more on either side,
+//    which hold the whole run of each of its points, and places its own
+//    points (a run that crosses workgroups is ranked alike by each);
 //  - a longer run by the whole wave holding its first point, as a bitonic
 //    sort in the wave's LDS (at most kWaveRun points) or in place in the
 //    outputs in global memory.
@@ -195,7 +200,8 @@ __global__ __launch_bounds__(kRsBlock) void bh_keys(T* __restrict__ Y, int N, co
 // (the first step of each merge compares mirror positions), so a run of any
 // length sorts as if padded with +infinity: comparisons with b past the run
 // are skipped.
-constexpr int kShortRun = 16;
+constexpr int kFixHalo = 64;
+constexpr int kFixWin = 256 + 2 * kFixHalo;
 constexpr int kWaveRun = 1024;
 
 TKA_DEV void wave_sync_lds()
@@ -257,63 +263,113 @@ __global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ ktop,
         side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
         return;
     }
+    using V2 = typename Vec2<T>::type;
     __shared__ uint64_t sk[4][kWaveRun];
     __shared__ uint32_t sp[4][kWaveRun];
-    const int j = blockIdx.x * 256 + threadIdx.x;
+    __shared__ uint64_t wk[kFixWin];  // window: full keys, top bits, points
+    __shared__ uint32_t wt[kFixWin], wp[kFixWin];
+    __shared__ uint64_t wstart[kFixWin / 64];  // bit k % 64 of word k / 64: a run starts at window position k
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // does a run start at j, and where does it end (gallop, then bisect)
-    bool start = false;
-    int e = j + 1;
-    if (j < N) {
-        const uint32_t h = ktop[j];
-        start = (j == 0) || (ktop[j - 1] != h);
-        if (start && e < N && ktop[e] == h) {
-            int lo = e, hi = N, step = 2;  // ktop[lo] == h; hi: first index known past the run
-            while (j + step < N) {
-                if (ktop[j + step] != h) {
-                    hi = j + step;
-                    break;
-                }
-                lo = j + step;
-                step <<= 1;
-            }
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (ktop[mid] == h)
-                    lo = mid;
-                else
-                    hi = mid;
-            }
-            e = hi;
-        }
-    }
+    const int w0 = (int)blockIdx.x * 256 - kFixHalo;  // window position 0
+    const int lo = max(0, -w0), hi = min(kFixWin, N - w0);  // valid window positions
     // Sorted point indices are clamped to N-1 when read: a sort whose tile
     // look-back gave up (its fault word set) may leave positions unwritten,
     // and every gather and scatter downstream then still stays in bounds.
     const uint32_t pmax = (uint32_t)(N - 1);
-    if (start && e - j <= kShortRun) {
-        for (int i = j; i < e; i++) {
-            const uint32_t p = min(perm[i], pmax);
-            const uint64_t key = kfull[p];
-            int k = i;
-            while (k > j && k64[k - 1] > key) {
-                k64[k] = k64[k - 1];
-                p2[k] = p2[k - 1];
-                k--;
-            }
-            k64[k] = key;
-            p2[k] = p;
-        }
-        for (int i = j; i < e; i++) {
-            const uint32_t p = p2[i];
-            typename Vec2<T>::type v;
-            v.x = Y[2 * (size_t)p];
-            v.y = Y[2 * (size_t)p + 1];
-            ys[i] = v;
+    static_assert(kFixWin <= 512 && kFixWin % 64 == 0, "two window positions per thread, whole words of flags");
+    {
+        // two positions per thread, both points requested before their keys
+        const int t1 = (int)threadIdx.x + 256;
+        const int g0 = w0 + (int)threadIdx.x, g1 = w0 + t1;
+        const bool in0 = g0 >= 0 && g0 < N, in1 = t1 < kFixWin && g1 >= 0 && g1 < N;
+        const uint32_t p0 = in0 ? min(perm[g0], pmax) : 0u, p1 = in1 ? min(perm[g1], pmax) : 0u;
+        const uint32_t h0 = in0 ? ktop[g0] : 0u, h1 = in1 ? ktop[g1] : 0u;
+        const uint64_t f0 = in0 ? kfull[p0] : 0ull, f1 = in1 ? kfull[p1] : 0ull;
+        wt[threadIdx.x] = h0;
+        wp[threadIdx.x] = p0;
+        wk[threadIdx.x] = f0;
+        if (t1 < kFixWin) {
+            wt[t1] = h1;
+            wp[t1] = p1;
+            wk[t1] = f1;
         }
     }
-    // long runs, one at a time, by the whole wave
-    uint64_t longs = __ballot(start && e - j > kShortRun);
+    __syncthreads();
+    // run starts (every position outside [lo, hi) counts as one, so no run
+    // reaches past the valid keys); the second positions are whole waves
+    for (int k = (int)threadIdx.x; k < kFixWin; k += 256) {
+        const bool f = k <= lo || k >= hi || wt[k] != wt[k - 1];
+        const uint64_t m = __ballot(f);
+        if (lane == 0) wstart[k >> 6] = m;
+    }
+    __syncthreads();
+    const int j = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    const int lj = (int)threadIdx.x + kFixHalo;
+    bool start = false;  // j starts a run longer than kFixHalo
+    if (j < N) {
+        // The run around j from the start bits: its start is the last one at
+        // or before lj, its end the first one after lj, each looked for in
+        // lj's word and the next one out (a run of at most kFixHalo = 64
+        // points has both there; one not found means a longer run).
+        const int wi = lj >> 6, bi = lj & 63;
+        const uint64_t upto = (2ull << bi) - 1ull;  // bits 0..bi (all for bi = 63)
+        const uint64_t mb = wstart[wi] & upto, ma = wstart[wi] & ~upto;
+        int s = -1, e = INT_MAX;
+        if (mb != 0ull) {
+            s = wi * 64 + 63 - __clzll((long long)mb);
+        } else {
+            const uint64_t m2 = wstart[wi - 1];  // wi >= 1: lj >= kFixHalo
+            if (m2 != 0ull) s = (wi - 1) * 64 + 63 - __clzll((long long)m2);
+        }
+        if (ma != 0ull) {
+            e = wi * 64 + __ffsll((long long)ma) - 1;
+        } else {
+            const uint64_t m2 = wstart[wi + 1];  // wi + 1 < kFixWin / 64: lj < 256 + kFixHalo
+            if (m2 != 0ull) e = (wi + 1) * 64 + __ffsll((long long)m2) - 1;
+        }
+        if (s >= 0 && e != INT_MAX && e - s <= kFixHalo) {
+            const uint64_t kj = wk[lj];
+            const uint32_t pj = wp[lj];
+            int r = 0;
+            for (int k = s; k < e; k++) {
+                const uint64_t kk = wk[k];
+                r += (kk < kj || (kk == kj && wp[k] < pj)) ? 1 : 0;
+            }
+            const int d = w0 + s + r;
+            k64[d] = kj;
+            p2[d] = pj;
+            V2 v;
+            v.x = Y[2 * (size_t)pj];
+            v.y = Y[2 * (size_t)pj + 1];
+            ys[d] = v;
+        } else {
+            start = ((wstart[wi] >> bi) & 1ull) != 0ull;
+        }
+    }
+    // long runs, one at a time, by the whole wave; the end by a gallop, then
+    // a bisection, over the sorted top bits
+    int e = j + 1;
+    if (start) {
+        const uint32_t h = wt[lj];
+        int lo2 = e, hi2 = N, step = 2;  // ktop[lo2] == h (the run is longer than 1); hi2: first index known past it
+        while (j + step < N) {
+            if (ktop[j + step] != h) {
+                hi2 = j + step;
+                break;
+            }
+            lo2 = j + step;
+            step <<= 1;
+        }
+        while (hi2 - lo2 > 1) {
+            const int mid = (lo2 + hi2) >> 1;
+            if (ktop[mid] == h)
+                lo2 = mid;
+            else
+                hi2 = mid;
+        }
+        e = hi2;
+    }
+    uint64_t longs = __ballot(start);
     while (longs != 0ull) {
         const int l = __ffsll((long long)longs) - 1;
         longs &= longs - 1ull;
@@ -332,7 +388,7 @@ __global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ ktop,
                 const uint32_t p = P[i];
                 k64[rj + i] = K[i];
                 p2[rj + i] = p;
-                typename Vec2<T>::type v;
+                V2 v;
                 v.x = Y[2 * (size_t)p];
                 v.y = Y[2 * (size_t)p + 1];
                 ys[rj + i] = v;
@@ -350,7 +406,7 @@ __global__ __launch_bounds__(256) void bh_fix(const uint32_t* __restrict__ ktop,
             wave_bitonic<true>(K, P, len);  // ends with a fence: P is complete
             for (int i = lane; i < len; i += 64) {
                 const uint32_t p = P[i];
-                typename Vec2<T>::type v;
+                V2 v;
                 v.x = Y[2 * (size_t)p];
                 v.y = Y[2 * (size_t)p + 1];
                 ys[rj + i] = v;
@@ -380,7 +436,7 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
                                                 unsigned long long* __restrict__ mask, int* __restrict__ node_a,
                                                 int* __restrict__ node_b, int* __restrict__ node_lvl,
                                                 uint32_t* __restrict__ tile_cnt, double* __restrict__ part,
-                                                uint32_t* __restrict__ eq24, int nmain, AttractSlice<T> side)
+                                                uint32_t* __restrict__ runs, int nmain, AttractSlice<T> side)
 {
     if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
         side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
@@ -388,15 +444,17 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
     }
     __shared__ uint64_t win[kPairWin];
     __shared__ uint32_t own_cnt[2];  // owners with a in tile blockIdx.x - 1, blockIdx.x
-    __shared__ uint32_t eq_cnt;      // pairs whose keys share their top 24 bits
+    __shared__ uint32_t run_cnt[2];  // keys that share their top 16 / 24 bits with the key kFixHalo on
     __shared__ double sh[4];
     const int w0 = (int)blockIdx.x * 256 - 256;
     for (int t = threadIdx.x; t < kPairWin; t += 256) {
         const int g = w0 + t;
         win[t] = (g >= 0 && g < N) ? keys[g] : 0ull;
     }
-    if (threadIdx.x < 2) own_cnt[threadIdx.x] = 0u;
-    if (threadIdx.x == 0) eq_cnt = 0u;
+    if (threadIdx.x < 2) {
+        own_cnt[threadIdx.x] = 0u;
+        run_cnt[threadIdx.x] = 0u;
+    }
     __syncthreads();
     // the window's valid keys: [wl, wh)
     const int wl = max(w0, 0), wh = min(w0 + kPairWin, N);
@@ -405,8 +463,17 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
     const uint64_t ki = act ? win[i - w0] : 0ull, kj = act ? win[i + 1 - w0] : 0ull;
     const uint64_t x = ki ^ kj;
     {
-        const uint64_t e = __ballot(act && (x >> 40) == 0ull);
-        if ((threadIdx.x & 63) == 0 && e != 0ull) atomicAdd(&eq_cnt, (uint32_t)__popcll(e));
+        // Radix width of the next build: a key that shares its top 16 (24)
+        // bits with the key kFixHalo positions on lies in a run longer than
+        // kFixHalo points when 2 (3) radix passes sort the keys.
+        // (i + kFixHalo - w0 < kPairWin.)
+        const bool in = i + kFixHalo < N;
+        const uint64_t y = in ? win[i - w0] ^ win[i + kFixHalo - w0] : ~0ull;
+        const uint64_t e16 = __ballot((y >> 48) == 0ull), e24 = __ballot((y >> 40) == 0ull);
+        if ((threadIdx.x & 63) == 0) {
+            if (e16 != 0ull) atomicAdd(&run_cnt[0], (uint32_t)__popcll(e16));
+            if (e24 != 0ull) atomicAdd(&run_cnt[1], (uint32_t)__popcll(e24));
+        }
     }
     const int lvl = (x == 0ull) ? kB : (__clzll((long long)x) >> 1);
     const int lowbits = 2 * (kB - lvl);  // 0..64
@@ -465,7 +532,8 @@ __global__ __launch_bounds__(256) void bh_pairs(const uint64_t* __restrict__ key
         part[2 * (size_t)blockIdx.x + 1] = sy;
         if (own_cnt[1]) atomicAdd(&tile_cnt[blockIdx.x], own_cnt[1]);
         if (own_cnt[0]) atomicAdd(&tile_cnt[blockIdx.x - 1], own_cnt[0]);
-        eq24[blockIdx.x] = eq_cnt;
+        runs[2 * (size_t)blockIdx.x] = run_cnt[0];
+        runs[2 * (size_t)blockIdx.x + 1] = run_cnt[1];
     }
 }
 
@@ -540,20 +608,27 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
                                                const uint32_t* __restrict__ perm, int N,
                                                const uint32_t* __restrict__ pre, const double* __restrict__ psx,
                                                const double* __restrict__ psy, BhNode<T>* __restrict__ nodes,
-                                               const uint32_t* __restrict__ eq24, uint32_t* __restrict__ eq24_host,
+                                               const uint32_t* __restrict__ runs, uint32_t* __restrict__ runs_host,
                                                int nmain, AttractSlice<T> side)
 {
     if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
         side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
         return;
     }
-    if (blockIdx.x == 0 && eq24_host != nullptr) {  // the pairs kernel's count, posted to the host for the next
-                                                     // build's pass count
+    if (blockIdx.x == 0 && runs_host != nullptr) {  // the pairs kernel's counts, posted to the host for the
+                                                     // next build's radix width
         __shared__ double sh[4];
-        double c = 0.0;
-        for (int b = threadIdx.x; b < nmain; b += 256) c += (double)eq24[b];
-        c = block_sum(c, sh);
-        if (threadIdx.x == 0) __hip_atomic_store(eq24_host, (uint32_t)c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        double c16 = 0.0, c24 = 0.0;
+        for (int b = threadIdx.x; b < nmain; b += 256) {
+            c16 += (double)runs[2 * (size_t)b];
+            c24 += (double)runs[2 * (size_t)b + 1];
+        }
+        c16 = block_sum(c16, sh);
+        c24 = block_sum(c24, sh);
+        if (threadIdx.x == 0) {
+            __hip_atomic_store(runs_host, (uint32_t)c16, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(runs_host + 1, (uint32_t)c24, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= N) return;
@@ -601,7 +676,7 @@ __global__ __launch_bounds__(256) void bh_emit(const unsigned long long* __restr
 //    k = b / 8, grid = 8q + r): every XCD walks one contiguous Morton range,
 //    whose cells are a small part of the tree, and its L2 holds them.
 template <class T>
-__global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__ nodes,
+__global__ __launch_bounds__(kTravBlock) void bh_traverse(const BhNode<T>* __restrict__ nodes,
                                                    const uint32_t* __restrict__ pre,
                                                    const typename Vec2<T>::type* __restrict__ ys,
                                                    const uint32_t* __restrict__ perm, int N, T theta2,
@@ -613,7 +688,7 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
     __shared__ double sh[4];
     const int nb = (int)gridDim.x, xcd = (int)blockIdx.x & 7, kx = (int)blockIdx.x >> 3;
     const int lb = xcd * (nb >> 3) + min(xcd, nb & 7) + kx;
-    const int j = lb * 256 + threadIdx.x;
+    const int j = lb * kTravBlock + threadIdx.x;
     // No walk over a tree whose Morton sort gave up (its nodes may hold
     // entries that no kernel wrote): forces stay unwritten, Z partials are
     // 0, and the host reports the fault word.
@@ -643,7 +718,7 @@ __global__ __launch_bounds__(256) void bh_traverse(const BhNode<T>* __restrict__
     {
         const int q = nb >> 3, r = nb & 7, count = q + (xcd < r ? 1 : 0);
         const int first = xcd * q + min(xcd, r);
-        const int j0 = min(N, first * 256), j1 = min(N, (first + count) * 256);
+        const int j0 = min(N, first * kTravBlock), j1 = min(N, (first + count) * kTravBlock);
         const int a = (int)pre[j0], b = (int)pre[j1];
         const int s0 = a + (int)((long long)(b - a) * kx / count);
         const int s1 = a + (int)((long long)(b - a) * (kx + 1) / count);
@@ -736,13 +811,15 @@ void BhTree<T>::alloc(int n, MemBudget* mb)
     perm_alt.alloc(n, mb);
     kfull.alloc(n, mb);
     passes = n <= (1 << 22) ? 3 : 4;
-    eq24.alloc(ceil_div(n, 256), mb);
-    if (!eq24_host) {
+    runs.alloc(2 * (size_t)ceil_div(n, 256), mb);
+    if (!runs_host) {
         void* h = nullptr;
-        TKA_HIP(hipHostMalloc(&h, sizeof(uint32_t), hipHostMallocDefault));
-        eq24_host = static_cast<uint32_t*>(h);
+        TKA_HIP(hipHostMalloc(&h, 2 * sizeof(uint32_t), hipHostMallocDefault));
+        runs_host = static_cast<uint32_t*>(h);
     }
-    __atomic_store_n(eq24_host, 0u, __ATOMIC_RELAXED);
+    // no counts yet: the first build sorts 3 radix passes
+    __atomic_store_n(runs_host, 1u, __ATOMIC_RELAXED);
+    __atomic_store_n(runs_host + 1, 0u, __ATOMIC_RELAXED);
     k64.alloc(n, mb);
     p2.alloc(n, mb);
     sort_scratch.alloc(radix_sort_scratch_words(n, kRsItemsCounted), mb);
@@ -773,8 +850,8 @@ void BhTree<T>::carve(int n, Arena& a)
     a.take(perm_alt, n);
     a.take(kfull, n);
     passes = n <= (1 << 22) ? 3 : 4;
-    a.take(eq24, ceil_div(n, 256));
-    eq24_host = nullptr;
+    a.take(runs, 2 * (size_t)ceil_div(n, 256));
+    runs_host = nullptr;
     a.take(k64, n);
     a.take(p2, n);
     a.take(sort_scratch, radix_sort_scratch_words(n, kRsItemsCounted));
@@ -820,11 +897,16 @@ template <class T>
 void bh_build_keys(T* Y, int N, const double* box_part, int nbox, BhTree<T>& tr, hipStream_t s)
 {
     require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "tree storage too small");
-    if (N <= (1 << 22) && tr.eq24_host) {  // hysteresis between 1/16 and 1/8 of the points
-        const uint32_t eq = __atomic_load_n(tr.eq24_host, __ATOMIC_RELAXED);
-        if (tr.passes == 3 && eq > (uint32_t)N / 8) tr.passes = 4;
-        else if (tr.passes == 4 && eq < (uint32_t)N / 16) tr.passes = 3;
+    if (N <= (1 << 22) && tr.runs_host) {
+        // 2 passes while no run of equal top 16 bits is longer than kFixHalo
+        // (the rank sort takes every run); 3 while runs of equal top 24 bits
+        // leave fewer than kWaveRun - kFixHalo keys in long runs (no run is
+        // then longer than a wave sorts in LDS); else 4
+        const uint32_t c16 = __atomic_load_n(tr.runs_host, __ATOMIC_RELAXED);
+        const uint32_t c24 = __atomic_load_n(tr.runs_host + 1, __ATOMIC_RELAXED);
+        tr.passes = c16 == 0u ? 2 : (c24 < (uint32_t)(kWaveRun - kFixHalo) ? 3 : 4);
     }
+    if (tr.forced_passes > 0) tr.passes = tr.forced_passes;
     const int tiles = (int)radix_tiles((size_t)N, kRsItemsCounted);
     hipLaunchKernelGGL(bh_keys<T>, dim3(tiles), dim3(kRsBlock), 0, s, Y, N, box_part, nbox, tr.root.get(),
                        tr.passes, tr.keys.get(), tr.kfull.get(), tr.perm.get(), tr.mask.get(), tr.tile_cnt.get(),
@@ -886,14 +968,14 @@ void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s, const Attrac
     static_assert(kTreeScanTile == 256, "one scan tile per pairs workgroup");
     hipLaunchKernelGGL(bh_pairs<T>, dim3(g + wgs[kp]), dim3(256), 0, s, tr.k64.get(), tr.ys.get(), N, tr.mask.get(),
                        tr.node_a.get(), tr.node_b.get(), tr.node_lvl.get(), tr.tile_cnt.get(), tr.part.get(),
-                       tr.eq24.get(), (int)g, slice[kp]);
+                       tr.runs.get(), (int)g, slice[kp]);
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(bh_scan_write<T>, dim3(g + wgs[ks]), dim3(256), 0, s, tr.mask.get(), tr.ys.get(), N,
                        tr.tile_cnt.get(), tr.part.get(), tr.pre.get(), tr.sx.get(), tr.sy.get(), (int)g, slice[ks]);
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(bh_emit<T>, dim3(g + wgs[ke]), dim3(256), 0, s, tr.mask.get(), tr.node_a.get(),
                        tr.node_b.get(), tr.node_lvl.get(), tr.ys.get(), p, N, tr.pre.get(), tr.sx.get(), tr.sy.get(),
-                       tr.nodes.get(), tr.eq24.get(), tr.eq24_host, (int)g, slice[ke]);
+                       tr.nodes.get(), tr.runs.get(), tr.runs_host, (int)g, slice[ke]);
     TKA_LAUNCH_CHECK();
 }
 
@@ -903,7 +985,7 @@ int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* 
 {
     (void)Y;
     const int g = bh_zblocks(tr.N);
-    hipLaunchKernelGGL((bh_traverse<T>), dim3(g), dim3(256), 0, s, tr.nodes.get(), tr.pre.get(), tr.ys.get(),
+    hipLaunchKernelGGL((bh_traverse<T>), dim3(g), dim3(kTravBlock), 0, s, tr.nodes.get(), tr.pre.get(), tr.ys.get(),
                        tr.sorted_perm, tr.N, (T)(theta * theta), tr.root.get(), neg, zpart, interactions,
                        tr.sort_fault());
     TKA_LAUNCH_CHECK();

@@ -116,14 +116,15 @@ template <class T>
 struct BhTree {
     int N = 0;
     // Radix passes: the sort orders the top 8*passes key bits, and bh_fix
-    // sorts each run of equal top bits by full key.  3 passes (levels 1-12)
-    // while few sorted neighbours share their top 24 bits; 4 when many do
-    // (tight clusters, e.g. during early exaggeration), so bh_fix's runs stay
-    // short.  The choice reads a count that the previous build posted to
-    // pinned host memory (no synchronisation; results do not depend on it).
+    // sorts each run of equal top bits by full key.  2 passes (levels 1-8)
+    // while every run of equal top 16 bits is short enough for bh_fix's rank
+    // sort, 3 (levels 1-12) or 4 when runs are longer (tight clusters).  The
+    // choice reads counts that the previous build posted to pinned host
+    // memory (no synchronisation; results do not depend on it).
     int passes = 3;
-    uint32_t* eq24_host = nullptr;  // pinned: adjacent sorted keys with equal top 24 bits, last build
-    DevBuf<uint32_t> eq24;          // the same count per pairs workgroup
+    int forced_passes = 0;          // > 0: this many passes, whatever the counts say
+    uint32_t* runs_host = nullptr;  // pinned: keys in long runs at 16 and 24 top bits, last build
+    DevBuf<uint32_t> runs;          // the same counts per pairs workgroup
     DevBuf<uint32_t> keys, keys_alt, perm, perm_alt, sort_scratch;  // top key bits, radix-sorted
     DevBuf<uint64_t> kfull;      // full key per point
     DevBuf<uint64_t> k64;        // full keys, sorted
@@ -154,7 +155,7 @@ struct BhTree {
     BhTree& operator=(const BhTree&) = delete;
     ~BhTree()
     {
-        if (eq24_host) (void)hipHostFree(eq24_host);
+        if (runs_host) (void)hipHostFree(runs_host);
     }
 };
 
@@ -184,7 +185,10 @@ void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s, const Attrac
 template <class T>
 int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* zpart,
                  unsigned long long* interactions, hipStream_t s);
-inline int bh_zblocks(int N) { return (int)ceil_div(N, 256); }
+// Points per traversal workgroup (2-D and 3-D): one wave, so that the waves
+// spread evenly over the CUs; one Z partial per workgroup.
+constexpr int kTravBlock = 64;
+inline int bh_zblocks(int N) { return (int)ceil_div(N, kTravBlock); }
 
 // Synchronises the stream and tells whether any sort of this tree's builds
 // gave up (the forces of such builds are not valid).

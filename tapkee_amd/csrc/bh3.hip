@@ -349,7 +349,7 @@ __global__ __launch_bounds__(256) void bh3_emit(const unsigned long long* __rest
 // logical block x*q + min(x, r) + k (x = b % 8, k = b / 8, grid = 8q + r),
 // so every XCD walks one contiguous Morton range and its L2 holds the cells.
 template <class T>
-__global__ __launch_bounds__(256) void bh3_traverse(const BhNode3<T>* __restrict__ nodes,
+__global__ __launch_bounds__(kTravBlock) void bh3_traverse(const BhNode3<T>* __restrict__ nodes,
                                                     const uint32_t* __restrict__ pre, const T* __restrict__ ys,
                                                     const uint32_t* __restrict__ perm, int N, T theta2,
                                                     const RootBox3* __restrict__ root, T* __restrict__ neg,
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void bh3_traverse(const BhNode3<T>* __restrict
     __shared__ double sh[4];
     const int nb = (int)gridDim.x, xcd = (int)blockIdx.x & 7, kx = (int)blockIdx.x >> 3;
     const int lb = xcd * (nb >> 3) + min(xcd, nb & 7) + kx;
-    const int j = lb * 256 + threadIdx.x;
+    const int j = lb * kTravBlock + threadIdx.x;
     // no walk over a tree whose Morton sort gave up: forces stay unwritten,
     // Z partials are 0, and the host reports the fault word
     const bool active = j < N && *sort_fault == 0u;
@@ -498,7 +498,7 @@ int bh3_repulsion(const BhTree3<T>& tr, double theta, T* neg, double* zpart, uns
                   hipStream_t s)
 {
     const int g = bh_zblocks(tr.N);
-    hipLaunchKernelGGL((bh3_traverse<T>), dim3(g), dim3(256), 0, s, (const BhNode3<T>*)tr.nodes.get(),
+    hipLaunchKernelGGL((bh3_traverse<T>), dim3(g), dim3(kTravBlock), 0, s, (const BhNode3<T>*)tr.nodes.get(),
                        (const uint32_t*)tr.pre.get(), (const T*)tr.ys.get(), tr.sorted_perm, tr.N,
                        (T)(theta * theta), (const RootBox3*)tr.root.get(), neg, zpart, interactions, tr.sort_fault());
     TKA_LAUNCH_CHECK();

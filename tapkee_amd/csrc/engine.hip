@@ -1508,14 +1508,19 @@ TKA_API tka_status tka_bh_tree(const double* Y, int32_t N, int32_t precision, in
             DevBuf<double> box((size_t)kBoxStride * bbox_blocks(N));
             BhTree<T> tree;
             tree.alloc(N, &mb);
-            // Two builds from the same input (the key kernel centres Y in
-            // place): the second sorts as many key bits as the first one's
-            // count of equal neighbours asks for (3 or 4 radix passes), and
-            // the trees must agree node for node.
+            // One build per radix width from the same input (the key kernel
+            // centres Y in place): 2, 3 and 4 passes up to 2^22 points (4
+            // beyond), then one with the width the last build's run counts
+            // ask for; the trees must agree node for node.
+            std::vector<int> widths;
+            if (N <= (1 << 22)) widths = {2, 3, 4};
+            else widths = {4};
+            widths.push_back(0);
             std::vector<BhNode<T>> first;
             uint32_t total = 0;
             RootBox rb;
-            for (int b = 0; b < 2; b++) {
+            for (size_t b = 0; b < widths.size(); b++) {
+                tree.forced_passes = widths[b];
                 cast_device<double, T>(tmp.get(), y.get(), (size_t)N * 2, s);
                 bbox_partials_device<T>(y.get(), N, box.get(), s);
                 bh_build<T>(y.get(), N, box.get(), bbox_blocks(N), tree, s);

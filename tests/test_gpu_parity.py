@@ -403,10 +403,13 @@ def test_bh_tree_matches_reference_tree(engine, oracle, n):
 @pytest.mark.parametrize("m", [300, 3000])
 def test_bh_tree_tight_clusters(engine, oracle, m):
     # m points (and 40 more elsewhere) within ~1e-5 of each other in a box of
-    # width ~9: their keys share the sorted top bits (level-12 cells), so the
-    # full-key order comes from the run sort: a wave's LDS sort for runs of
-    # 17..1024 points, an in-place sort in global memory for the 3000-point
-    # run.  The 64-bit keys are all distinct (checked when the case was made).
+    # width ~9: their keys share the sorted top bits (level-8 to level-16
+    # cells for 2 to 4 radix passes), so the full-key order comes from the
+    # run sort: ranks for runs of at most 64 points (the 40), a wave's LDS
+    # sort for runs of 65..1024 points, an in-place sort in global memory for
+    # the 3000-point run.  The engine builds the tree with each radix width
+    # and checks that they agree.  The 64-bit keys are all distinct (checked
+    # when the case was made).
     r = rng(17)
     Y = r.normal(size=(20000, 2))
     Y[:m] = r.normal(size=(m, 2)) * 1e-5 + [0.3, -0.2]
