@@ -933,8 +933,10 @@ void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s, const Attrac
     // back the extra workgroups.  Other splits (passes 8 or 16; node kernels
     // 14/7/6) measured the same within 0.5%.
     const int passes = N > 1 ? tr.passes : 0;
-    constexpr int kWPass = 12, kWFix = 0, kWPairs = 10, kWScan = 5, kWEmit = 4;
+    constexpr int kWPass = 12, kWPass0 = 8, kWFix = 0, kWPairs = 10, kWScan = 5, kWEmit = 4;
     std::vector<int> weight(passes, kWPass);
+    // pass 0 without look-back (few tiles) is the shorter one
+    if (passes > 0 && radix_tiles((size_t)N, kRsItemsCounted) <= (size_t)kRsDirectTiles) weight[0] = kWPass0;
     weight.insert(weight.end(), {kWFix, kWPairs, kWScan, kWEmit});
     const int nk = (int)weight.size();
     std::vector<AttractSlice<T>> slice(nk);

@@ -62,6 +62,71 @@ __global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys,
     rs_count_flush(passes, L, h);
 }
 
+// Pass 0 of the counted sort in direct mode: the digit totals and the counts
+// in the tiles before `below`, from the per-tile counts (tiles x 128 words,
+// two 16-bit counts each).  A wave reads two rows per load (a half-wave per
+// row, eight digits per lane), wave w rows 2w, 2w + 1, 2w + 8, ... with all
+// its loads in flight at once; the waves' sums meet in LDS.  Returns the
+// total of digit threadIdx.x and its count before `below` in *before.  Called
+// by all kRsBlock threads; tiles <= kRsDirectTiles.
+constexpr int kSumWaves = kRsBlock / 64;
+constexpr int kSumLoads = kRsDirectTiles / (2 * kSumWaves);
+TKA_DEV uint32_t rs_direct_sums(const uint32_t* counts, int tiles, int below, uint32_t* before,
+                                uint32_t (*sh)[256])
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, half = lane >> 5, q = lane & 31;
+    const uint4* c4 = reinterpret_cast<const uint4*>(counts);
+    uint32_t tot[8], bef[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) tot[c] = bef[c] = 0u;
+    constexpr int kBatch = 8;
+    for (int k0 = 0; k0 < kSumLoads; k0 += kBatch) {
+        if (2 * (k0 * kSumWaves + w) >= tiles) break;
+        uint4 v[kBatch];
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const int r = 2 * ((k0 + k) * kSumWaves + w) + half;
+            v[k] = r < tiles ? c4[(size_t)r * 32 + q] : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int k = 0; k < kBatch; k++) {
+            const int r = 2 * ((k0 + k) * kSumWaves + w) + half;
+            const uint32_t u[4] = {v[k].x, v[k].y, v[k].z, v[k].w};
+            const bool b = r < below;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const uint32_t lo = u[c] & 0xffffu, hi = u[c] >> 16;
+                tot[2 * c] += lo;
+                tot[2 * c + 1] += hi;
+                bef[2 * c] += b ? lo : 0u;
+                bef[2 * c + 1] += b ? hi : 0u;
+            }
+        }
+    }
+    // digit 8q + c of this half-wave
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        tot[c] += __shfl_xor(tot[c], 32, 64);
+        bef[c] += __shfl_xor(bef[c], 32, 64);
+    }
+    if (half == 0) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) {
+            sh[2 * w][8 * q + c] = tot[c];
+            sh[2 * w + 1][8 * q + c] = bef[c];
+        }
+    }
+    __syncthreads();
+    uint32_t t = 0u, b = 0u;
+#pragma unroll
+    for (int ww = 0; ww < kSumWaves; ww++) {
+        t += sh[2 * ww][threadIdx.x];
+        b += sh[2 * ww + 1][threadIdx.x];
+    }
+    *before = b;
+    return t;
+}
+
 // Stable scatter of one tile.  Item k of lane l in wave w sits at tile
 // position w*64*ITEMS + k*64 + l, so ranking items in (w, k, l) order keeps
 // the input order among equal digits.
@@ -69,7 +134,8 @@ __global__ __launch_bounds__(kRsBlock) void rs_count(const K* __restrict__ keys,
 template <class K, class V, int ITEMS, class Side>
 __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ kin, const V* __restrict__ vin,
                                                         K* __restrict__ kout, V* __restrict__ vout, size_t n,
-                                                        int pass, RsLayout L, int tiles, int ticketed, Side side)
+                                                        int pass, RsLayout L, int tiles, int ticketed, int direct_mode,
+                                                        Side side)
 {
     if ((int)blockIdx.x >= tiles) {
         side.run((int)blockIdx.x - tiles, (int)gridDim.x - tiles);
@@ -79,11 +145,15 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
     __shared__ uint32_t wcnt[kWaves][256];
     __shared__ uint32_t base[256];
     __shared__ uint32_t scan_sh[kWaves];
+    __shared__ uint32_t sum_sh[2 * kSumWaves][256];
     __shared__ uint32_t tile_sh;
     __shared__ int last_sh;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // direct mode (counted sort, few tiles): pass 0's totals and tile
+    // offsets from the per-tile counts, without look-back
+    const bool direct = direct_mode != 0 && pass == 0;
     // the global count of digit threadIdx.x, in flight with the keys
-    const uint32_t gc = L.gcnt[pass * 256 + threadIdx.x];
+    const uint32_t gc = direct ? 0u : L.gcnt[pass * 256 + threadIdx.x];
     for (int t = threadIdx.x; t < kWaves * 256; t += kRsBlock) (&wcnt[0][0])[t] = 0;
     if (ticketed) {
         if (threadIdx.x == 0) tile_sh = atomicAdd(&L.tile_ctr[pass], 1u);
@@ -104,12 +174,15 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
         key[k] = ok ? kin[i] : K(0);
         val[k] = ok ? vin[i] : V(0);
     }
+    uint32_t total = gc, before0 = 0u;
+    if (direct)
+        total = rs_direct_sums(rs_tile_counts(L.status, tiles), tiles, (int)tile, &before0, sum_sh);
     // digit bases; the barrier inside also completes the clearing of wcnt
-    const uint32_t gbase = block_exclusive_scan(gc, scan_sh, nullptr);
+    const uint32_t gbase = block_exclusive_scan(total, scan_sh, nullptr);
     // every thread has its count now: register this tile as a reader (the
     // result is needed only at the end)
     uint32_t readers = 0;
-    if (threadIdx.x == 0) readers = atomicAdd(&L.rd_ctr[pass], 1u);
+    if (!direct && threadIdx.x == 0) readers = atomicAdd(&L.rd_ctr[pass], 1u);
 #pragma unroll
     for (int k = 0; k < ITEMS; k++) dig[k] = (tile0 + (size_t)k * 64 < n) ? (int)((uint32_t)(key[k] >> shift) & 255u) : -1;
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -143,7 +216,9 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
         }
         uint32_t* st = L.status + (size_t)pass * tiles * 256 + d;
         uint32_t excl = 0;
-        if (tile == 0) {
+        if (direct) {
+            excl = before0;
+        } else if (tile == 0) {
             rs_st(st, kFlagInc | cnt);
         } else {
             rs_st(st + (size_t)tile * 256, kFlagAgg | cnt);
@@ -183,7 +258,7 @@ __global__ __launch_bounds__(kRsBlock) void rs_onesweep(const K* __restrict__ ki
             rs_st(st + (size_t)tile * 256, kFlagInc | (excl + cnt));
         }
         base[d] = gbase + excl;
-        if (threadIdx.x == 0) last_sh = readers == (uint32_t)(tiles - 1);
+        if (threadIdx.x == 0) last_sh = !direct && readers == (uint32_t)(tiles - 1);
     }
     __syncthreads();
 #pragma unroll
@@ -279,6 +354,9 @@ bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit
     const int passes = (end_bit + 7) / 8;
     const RsLayout L = rs_layout(scratch);
     require(counted || ITEMS == kRsItems, TKA_E_INTERNAL, "radix sort: the counting launch uses kRsItems tiles");
+    require(!counted || passes <= kRsCountedPasses, TKA_E_INTERNAL, "radix sort: the counted sort takes 32-bit keys");
+    // the key kernel's rs_count_tile chose the same mode from the same tile count
+    const int direct = counted && tiles <= kRsDirectTiles ? 1 : 0;
     if (!counted) {
         hipLaunchKernelGGL((rs_count<K>), dim3(std::min(tiles, kRsCountBlocks)), dim3(kRsBlock), 0, s, keys, n, passes, L,
                            tiles);
@@ -291,7 +369,7 @@ bool sort_impl(K* keys, V* vals, K* keys_alt, V* vals_alt, size_t n, int end_bit
     for (int p = 0; p < passes; p++) {
         const int extra = side ? side_wgs[p] : 0;
         hipLaunchKernelGGL((rs_onesweep<K, V, ITEMS, Side>), dim3(tiles + extra), dim3(kRsBlock), 0, s, ka, va, kb,
-                           vb, n, p, L, tiles, tiles > kRsMaxIndexedTiles ? 1 : 0, side ? side[p] : Side{});
+                           vb, n, p, L, tiles, tiles > kRsMaxIndexedTiles ? 1 : 0, direct, side ? side[p] : Side{});
         TKA_LAUNCH_CHECK();
         std::swap(ka, kb);
         std::swap(va, vb);

@@ -57,6 +57,20 @@ bool radix_sort_pairs_counted_side(K* keys, V* vals, K* keys_alt, V* vals_alt, s
 // the tile status words (passes x tiles x 256).
 constexpr size_t kRsHeader = 32 + (size_t)kRsMaxPasses * 256;
 
+// The counted sort (32-bit keys, at most 4 passes) keeps its tile status
+// words for 4 passes and, after them in the same scratch, every tile's pass-0
+// digit counts as 16-bit pairs (tiles x 128 words).  Up to kRsDirectTiles
+// tiles pass 0 takes its digit totals and tile offsets from those counts (no
+// look-back); the later passes, and every pass of larger sorts, take their
+// totals from the global per-pass counts and look back.
+constexpr int kRsCountedPasses = 4;
+constexpr int kRsDirectTiles = 256;
+static_assert(kRsBlock * kRsItemsCounted < 65536, "a tile's digit count fits 16 bits");
+TKA_DEV uint32_t* rs_tile_counts(uint32_t* status, int tiles)
+{
+    return status + (size_t)kRsCountedPasses * 256 * tiles;
+}
+
 // Tiles numbered by workgroup index (no arrival tickets) up to this many: all
 // of a sort's workgroups then fit on the device at once, so every tile a
 // workgroup waits on gets to run.  Larger sorts number tiles in arrival order.
@@ -135,10 +149,10 @@ TKA_DEV void rs_tile_histogram(const K (&key)[ITEMS], const bool (&ok)[ITEMS], i
 // kRsBlock threads after a barrier that completes h.  The counts are turned
 // into digit bases by each tile of the pass that uses them, and cleared by
 // the last tile to read them, ready for the next sort.
-TKA_DEV void rs_count_flush(int passes, RsLayout L, uint32_t (*h)[256])
+TKA_DEV void rs_count_flush(int passes, RsLayout L, uint32_t (*h)[256], int p0 = 0)
 {
     const int t = threadIdx.x;
-    for (int p = 0; p < passes; p++) {
+    for (int p = p0; p < passes; p++) {
         const uint32_t c = h[p][t];
         if (c) atomicAdd(&L.gcnt[p * 256 + t], c);
     }
@@ -148,25 +162,28 @@ TKA_DEV void rs_count_flush(int passes, RsLayout L, uint32_t (*h)[256])
 // threads of the workgroup that holds tile `tile` (keys
 // tile*kRsBlock*kRsItemsCounted + k*kRsBlock + threadIdx.x, k <
 // kRsItemsCounted; ok[k] false past n), one workgroup per tile.
-// Adds the tile's digit histograms of all passes to the global counts and
-// clears the tile's status words.  The sort's passes must follow: they clear
-// the counts again.
+// Stores the tile's pass-0 digit counts (direct mode) and adds the other
+// passes' (all passes' otherwise) to the global counts, and clears the
+// tile's status words.  The sort's passes must follow: they clear the global
+// counts again.
 template <class K>
 TKA_DEV void rs_count_tile(const K (&key)[kRsItemsCounted], const bool (&ok)[kRsItemsCounted], int passes, RsLayout L,
                            int tile,
                            int tiles)
 {
-    __shared__ uint32_t h[kRsMaxPasses][256];
+    __shared__ uint32_t h[kRsCountedPasses][256];
     const int t = threadIdx.x;
+    const bool direct = tiles <= kRsDirectTiles;
     for (int p = 0; p < passes; p++) {
         h[p][t] = 0;
-        L.status[((size_t)p * tiles + tile) * 256 + t] = 0;
+        if (p > 0 || !direct) L.status[((size_t)p * tiles + tile) * 256 + t] = 0;  // direct pass 0: no status
     }
     if (tile == 0 && t < kRsMaxPasses) L.tile_ctr[t] = 0;
     __syncthreads();
     rs_tile_histogram<K, kRsItemsCounted>(key, ok, passes, h);
     __syncthreads();
-    rs_count_flush(passes, L, h);
+    if (direct && t < 128) rs_tile_counts(L.status, tiles)[(size_t)tile * 128 + t] = h[0][2 * t] | (h[0][2 * t + 1] << 16);
+    rs_count_flush(passes, L, h, direct ? 1 : 0);
 }
 
 // Nonzero if a sort using this scratch gave up waiting on a predecessor tile
