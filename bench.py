@@ -59,10 +59,10 @@ FLOPS_PER_INTERACTION = 14   # dx, dy, D, q = 1/(1+D), z += c q, m = c q^2, f +=
 # dispatches) of `bench.py --steps 10 --warmup 2 --no-cpu --no-pipeline
 # --no-sweep --no-accuracy --no-parametric --copy-calibration 256`, each scaled
 # by what the counter reads for a plain 256 MiB device copy in the same run
-# (FETCH_SIZE reads 131093 KiB for 262144 KiB moved: x2.0; WRITE_SIZE reads
+# (FETCH_SIZE reads 131088 KiB for 262144 KiB moved: x2.0; WRITE_SIZE reads
 # 262144 KiB: x1.0).  Summary in profiles/pmc_traffic_bh_traverse_swissroll_n100k.md.
-TRAVERSE_FETCH_KIB = 2226.2
-TRAVERSE_WRITE_KIB = 2593.0
+TRAVERSE_FETCH_KIB = 2220.9
+TRAVERSE_WRITE_KIB = 2402.1
 FETCH_SCALE = 2.0
 WRITE_SCALE = 1.0
 

@@ -172,7 +172,8 @@ typedef struct tka_session tka_session;
 typedef struct tka_kernel_timing {
     int64_t launches_build;      /* tree builds (keys, sort, nodes) timed; while timing, the build runs
                                     without the attraction rows it otherwise carries */
-    int64_t launches_repulsion;  /* Barnes-Hut traversals timed */
+    int64_t launches_repulsion;  /* Barnes-Hut traversals timed (a timed step runs its traversal 4 times
+                                    back to back between one pair of events) */
     int64_t launches_step;       /* optimiser updates timed (with the attraction fused when theta = 0) */
     int64_t launches_exact;      /* exact all-pairs repulsion launches timed */
     double ms_build;             /* summed HIP-event durations on the session stream */

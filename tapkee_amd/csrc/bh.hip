@@ -792,9 +792,9 @@ __global__ __launch_bounds__(kTravBlock) void bh_traverse(const BhNode<T>* __res
     asm volatile("" ::"v"(warm));  // keeps the warming loads
     zd = block_sum(zd, sh);
     if (threadIdx.x == 0) zpart[lb] = zd;
-    if (interactions != nullptr) {
+    if (interactions != nullptr) {  // per workgroup: one atomic per workgroup would serialise 10^3 of them
         const double c = block_sum((double)cnt, sh);
-        if (threadIdx.x == 0) atomicAdd(interactions, (unsigned long long)c);
+        if (threadIdx.x == 0) interactions[1 + lb] = (unsigned long long)c;
     }
 }
 
@@ -992,6 +992,26 @@ int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* 
                        tr.sort_fault());
     TKA_LAUNCH_CHECK();
     return g;
+}
+
+static __global__ __launch_bounds__(256) void add_interactions(unsigned long long* counts, int n, int times)
+{
+    __shared__ unsigned long long sh[256];
+    unsigned long long c = 0;
+    for (int i = threadIdx.x; i < n; i += 256) c += counts[1 + i];
+    sh[threadIdx.x] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+        for (int i = 0; i < 256; i++) t += sh[i];
+        counts[0] += t * (unsigned long long)times;
+    }
+}
+
+void bh_add_interactions(unsigned long long* counts, int nblocks, int launches, hipStream_t s)
+{
+    hipLaunchKernelGGL(add_interactions, dim3(1), dim3(256), 0, s, counts, nblocks, launches);
+    TKA_LAUNCH_CHECK();
 }
 
 template <class T>

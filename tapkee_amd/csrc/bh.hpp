@@ -181,10 +181,14 @@ void bh_build_tree(const T* Y, int N, BhTree<T>& tr, hipStream_t s, const Attrac
 
 // Repulsive forces: neg (N x 2) = sum over accepted cells of
 // count*q^2*(y - com) and zpart[block] = partial sums of count*q
-// (quadtree.hpp:317-354).  Returns the number of zpart blocks.
+// (quadtree.hpp:317-354).  Returns the number of zpart blocks.  With
+// `interactions` (1 + blocks words), workgroup b leaves its count of cell and
+// leaf interactions in interactions[1 + b]; bh_add_interactions adds them,
+// times the number of launches that left them, to interactions[0].
 template <class T>
 int bh_repulsion(const BhTree<T>& tr, const T* Y, double theta, T* neg, double* zpart,
                  unsigned long long* interactions, hipStream_t s);
+void bh_add_interactions(unsigned long long* counts, int nblocks, int launches, hipStream_t s);
 // Points per traversal workgroup (2-D and 3-D): one wave, so that the waves
 // spread evenly over the CUs; one Z partial per workgroup.
 constexpr int kTravBlock = 64;

@@ -425,9 +425,9 @@ __global__ __launch_bounds__(kTravBlock) void bh3_traverse(const BhNode3<T>* __r
     }
     zd = block_sum(zd, sh);
     if (threadIdx.x == 0) zpart[lb] = zd;
-    if (interactions != nullptr) {
+    if (interactions != nullptr) {  // per workgroup, as the 2-D walk
         const double c = block_sum((double)cnt, sh);
-        if (threadIdx.x == 0) atomicAdd(interactions, (unsigned long long)c);
+        if (threadIdx.x == 0) interactions[1 + lb] = (unsigned long long)c;
     }
 }
 

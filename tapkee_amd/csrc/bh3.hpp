@@ -64,7 +64,8 @@ void bh3_build(T* Y, int N, BhTree3<T>& tr, hipStream_t s);
 
 // Repulsive forces: neg (N x 3) = sum over accepted cells of
 // count*q^2*(y - com) and zpart[block] = partial sums of count*q; returns
-// the number of zpart blocks (bh_zblocks(N)).
+// the number of zpart blocks (bh_zblocks(N)).  `interactions` as for
+// bh_repulsion.
 template <class T>
 int bh3_repulsion(const BhTree3<T>& tr, double theta, T* neg, double* zpart, unsigned long long* interactions,
                   hipStream_t s);

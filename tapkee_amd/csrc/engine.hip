@@ -349,7 +349,11 @@ struct LoopState {
 // The events carry no system-scope fence (they only time device work), so
 // recording them adds no cache writeback between the kernels they bracket.
 struct EventTimer {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    struct Span {
+        hipEvent_t a, b;
+        int counts;  // launches that end with the span
+    };
+    std::vector<Span> ev;
     hipEvent_t pending = nullptr;
     double ms = 0.0;
     int64_t launches = 0;
@@ -358,36 +362,43 @@ struct EventTimer {
         TKA_HIP(hipEventCreateWithFlags(&pending, hipEventDisableSystemFence));
         TKA_HIP(hipEventRecord(pending, s));
     }
-    void end(hipStream_t s)
+    // `launches` launches end here (0: a part of one, which continues at the
+    // next begin())
+    void end(hipStream_t s, int launches_done = 1)
     {
         hipEvent_t e;
         TKA_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         TKA_HIP(hipEventRecord(e, s));
-        ev.emplace_back(pending, e);
+        ev.push_back({pending, e, launches_done});
         pending = nullptr;
     }
+    void pause(hipStream_t s) { end(s, 0); }
     void collect()
     {
         for (auto& p : ev) {
             float t = 0.f;
-            TKA_HIP(hipEventElapsedTime(&t, p.first, p.second));
+            TKA_HIP(hipEventElapsedTime(&t, p.a, p.b));
             ms += t;
-            launches++;
-            (void)hipEventDestroy(p.first);
-            (void)hipEventDestroy(p.second);
+            launches += p.counts;
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
         }
         ev.clear();
     }
     ~EventTimer()
     {
         for (auto& p : ev) {
-            (void)hipEventDestroy(p.first);
-            (void)hipEventDestroy(p.second);
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
         }
     }
 };
 
 enum { kTimBuild = 0, kTimRepulsion = 1, kTimStep = 2, kTimExact = 3, kTimAttract = 4, kTimCount = 5 };
+// Profiled steps run the traversal this many times back to back (it rewrites
+// the same forces) between one pair of events, so that the events' own cost
+// is spread over the launches.
+constexpr int kProfileRepeats = 4;
 
 }  // namespace
 }  // namespace tka
@@ -539,7 +550,8 @@ struct Session : tka_session {
         Ynext = L.Yb.get();
         exag = o.exaggeration;
         momentum = o.momentum;
-        interactions.alloc(1, m);
+        // [0]: interactions counted over profiled traversals; [1 + b]: traversal workgroup b's, last launch
+        interactions.alloc(1 + (size_t)bh_zblocks(N), m);
         TKA_HIP(hipMemsetAsync(interactions.get(), 0, sizeof(unsigned long long), s));
         std::vector<int> id(N);
         for (int i = 0; i < N; i++) id[i] = i;
@@ -565,8 +577,14 @@ struct Session : tka_session {
                 if (profile) timers[kTimBuild].end(s);
                 attraction_alone();
                 if (profile) timers[kTimRepulsion].begin(s);
-                bh3_repulsion<T>(tree3, theta, L.neg.get(), L.zpart.get(), profile ? interactions.get() : nullptr, s);
-                if (profile) timers[kTimRepulsion].end(s);
+                int g3 = 0;
+                for (int r = 0; r < (profile ? kProfileRepeats : 1); r++)
+                    g3 = bh3_repulsion<T>(tree3, theta, L.neg.get(), L.zpart.get(),
+                                          profile ? interactions.get() : nullptr, s);
+                if (profile) {
+                    timers[kTimRepulsion].end(s, kProfileRepeats);
+                    bh_add_interactions(interactions.get(), g3, kProfileRepeats, s);
+                }
             } else if (!tree_ready) {
                 if (iter > 0 && iter % kRenumberEvery == 0 && tree.sorted_perm != nullptr)
                     renumber(tree.sorted_perm);  // order of the previous iteration's tree
@@ -574,17 +592,28 @@ struct Session : tka_session {
                 // runs inside the sort and node launches, which occupy few
                 // CUs; beside the traversal it would slow the traversal's
                 // node loads by about its own duration.  Profiling times the
-                // build without it, then the attraction alone.
+                // build without it and the attraction alone between the key
+                // kernel and the rest of the build, so that the traversal
+                // still follows the node kernels as in the loop.
                 const AttractSlice<T> job = attract_job();
                 if (profile) timers[kTimBuild].begin(s);
                 bh_build_keys<T>(Ycur, N, L.box.get(), nbox, tree, s);
+                if (profile) {
+                    timers[kTimBuild].pause(s);
+                    attraction_alone();
+                    timers[kTimBuild].begin(s);
+                }
                 bh_build_tree<T>(Ycur, N, tree, s, profile ? nullptr : &job);
                 if (profile) timers[kTimBuild].end(s);
-                if (profile) attraction_alone();
                 if (profile) timers[kTimRepulsion].begin(s);
-                bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(),
-                                profile ? interactions.get() : nullptr, s);
-                if (profile) timers[kTimRepulsion].end(s);
+                int g2 = 0;
+                for (int r = 0; r < (profile ? kProfileRepeats : 1); r++)
+                    g2 = bh_repulsion<T>(tree, Ycur, theta, L.neg.get(), L.zpart.get(),
+                                         profile ? interactions.get() : nullptr, s);
+                if (profile) {
+                    timers[kTimRepulsion].end(s, kProfileRepeats);
+                    bh_add_interactions(interactions.get(), g2, kProfileRepeats, s);
+                }
             } else {
                 attraction_alone();  // tree and repulsion came with the last KL evaluation
             }

@@ -463,6 +463,26 @@ def test_update_recurrence(engine, oracle):
         np.testing.assert_allclose(x, y, rtol=1e-14, atol=1e-14)
 
 
+# ---------------------------------------------------------------- profiling
+@pytest.mark.parametrize("dims", [2, 3])
+def test_profiled_interaction_count(engine, dims):
+    # With theta far below any cell's size ratio no cell is accepted: every
+    # point interacts with each of the other N-1 leaves, so each profiled
+    # traversal launch counts N(N-1) interactions (summed from the
+    # traversal's workgroups).
+    X = mixture(3000, 16, k=5)
+    n = X.shape[0]
+    s = engine.Session(X, no_dims=dims, perplexity=20.0, theta=1e-9, seed=2)
+    s.step(3)
+    s.profile(True)
+    s.step(2)
+    kt = s.kernel_timing()
+    s.profile(False)
+    s.close()
+    assert kt["launches_repulsion"] >= 2 and kt["launches_repulsion"] % 2 == 0
+    assert kt["interactions"] == kt["launches_repulsion"] * n * (n - 1)
+
+
 # ---------------------------------------------------------------- full run
 def test_renumbered_run_returns_caller_order(engine):
     # points are renumbered every 50 iterations; embedding() must undo it.
