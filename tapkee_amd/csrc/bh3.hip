@@ -144,8 +144,9 @@ TKA_DEV uint32_t quantise21(double t)
 
 template <class T>
 __global__ __launch_bounds__(256) void bh3_keys(T* __restrict__ Y, int N, const RootBox3* __restrict__ root,
-                                                uint64_t* __restrict__ keys, uint32_t* __restrict__ perm,
-                                                unsigned long long* __restrict__ mask, uint32_t* __restrict__ tile_cnt)
+                                                uint64_t* __restrict__ keys, uint32_t* __restrict__ ktop,
+                                                uint32_t* __restrict__ perm, unsigned long long* __restrict__ mask,
+                                                uint32_t* __restrict__ tile_cnt)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < (N + kTile3 - 1) / kTile3) tile_cnt[i] = 0u;
@@ -158,21 +159,24 @@ __global__ __launch_bounds__(256) void bh3_keys(T* __restrict__ Y, int N, const 
         key |= spread21(quantise21(((double)c + rb.h[d]) / (2.0 * rb.h[d]))) << d;
     }
     keys[i] = key << 1;
+    ktop[i] = (uint32_t)(key >> 39);  // the top 24 bits of the left-aligned key: levels 1-8
     perm[i] = (uint32_t)i;
     mask[i] = 0ull;
 }
 
-// Sorted coordinates.  Sorted point indices are clamped to N-1: a sort whose
-// tile look-back gave up (its fault word set) may leave positions unwritten,
-// and every gather downstream then still stays in bounds.
+// Full keys in sorted order, the final point order and the sorted
+// coordinates: the radix sort ordered the points by their top 24 key bits,
+// and each run of equal top bits is put in (full key, point) order
+// (run_sort_tile), which equals a stable sort of the full keys.
 template <class T>
-__global__ __launch_bounds__(256) void bh3_gather(const T* __restrict__ Y, const uint32_t* __restrict__ perm, int N,
-                                                  T* __restrict__ ys)
+__global__ __launch_bounds__(256) void bh3_runs(const uint32_t* __restrict__ ktop, const uint32_t* __restrict__ perm,
+                                                const uint64_t* __restrict__ keys, const T* __restrict__ Y, int N,
+                                                uint64_t* __restrict__ k64, uint32_t* __restrict__ p2,
+                                                T* __restrict__ ys)
 {
-    const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= N) return;
-    const uint32_t p = min(perm[j], (uint32_t)(N - 1));
-    for (int d = 0; d < 3; d++) ys[3 * (size_t)j + d] = Y[3 * (size_t)p + d];
+    run_sort_tile(ktop, perm, keys, N, k64, p2, [&](int d, uint32_t p) {
+        for (int c = 0; c < 3; c++) ys[3 * (size_t)d + c] = Y[3 * (size_t)p + c];
+    });
 }
 
 // ---------------------------------------------------------------- pairs
@@ -439,9 +443,12 @@ void BhTree3<T>::alloc(int n, MemBudget* mb)
     require(n >= 2 && n <= kMaxTreePoints, TKA_E_UNSUPPORTED, "Barnes-Hut octree: 2 to 2^26 - 1 points");
     N = n;
     keys.alloc(n, mb);
-    keys_alt.alloc(n, mb);
+    keys_sorted.alloc(n, mb);
+    ktop.alloc(n, mb);
+    ktop_alt.alloc(n, mb);
     perm.alloc(n, mb);
     perm_alt.alloc(n, mb);
+    p2.alloc(n, mb);
     sort_scratch.alloc(radix_sort_scratch_words(n), mb);
     // on the stream the buffer was allocated on (ordered after the allocation)
     TKA_HIP(hipMemsetAsync(sort_scratch.get(), 0, sort_scratch.size() * sizeof(uint32_t), alloc_stream()));
@@ -471,14 +478,18 @@ void bh3_build(T* Y, int N, BhTree3<T>& tr, hipStream_t s)
     hipLaunchKernelGGL(bh3_root, dim3(1), dim3(256), 0, s, (const double*)tr.box.get(), nb, N, tr.root.get());
     TKA_LAUNCH_CHECK();
     hipLaunchKernelGGL(bh3_keys<T>, dim3(g), dim3(256), 0, s, Y, N, (const RootBox3*)tr.root.get(), tr.keys.get(),
-                       tr.perm.get(), tr.mask.get(), tr.tile_cnt.get());
+                       tr.ktop.get(), tr.perm.get(), tr.mask.get(), tr.tile_cnt.get());
     TKA_LAUNCH_CHECK();
-    const bool alt = radix_sort_pairs<uint64_t, uint32_t>(tr.keys.get(), tr.perm.get(), tr.keys_alt.get(),
-                                                          tr.perm_alt.get(), N, 64, tr.sort_scratch.get(), s);
-    tr.k64 = alt ? tr.keys_alt.get() : tr.keys.get();
-    tr.sorted_perm = alt ? tr.perm_alt.get() : tr.perm.get();
-    hipLaunchKernelGGL(bh3_gather<T>, dim3(g), dim3(256), 0, s, (const T*)Y, tr.sorted_perm, N, tr.ys.get());
+    // 3 radix passes over the top 24 key bits (octree levels 1-8), then the
+    // runs of equal top bits by full key
+    const bool alt = radix_sort_pairs<uint32_t, uint32_t>(tr.ktop.get(), tr.perm.get(), tr.ktop_alt.get(),
+                                                          tr.perm_alt.get(), N, 24, tr.sort_scratch.get(), s);
+    hipLaunchKernelGGL(bh3_runs<T>, dim3(g), dim3(256), 0, s, alt ? tr.ktop_alt.get() : tr.ktop.get(),
+                       alt ? tr.perm_alt.get() : tr.perm.get(), (const uint64_t*)tr.keys.get(), (const T*)Y, N,
+                       tr.keys_sorted.get(), tr.p2.get(), tr.ys.get());
     TKA_LAUNCH_CHECK();
+    tr.k64 = tr.keys_sorted.get();
+    tr.sorted_perm = tr.p2.get();
     hipLaunchKernelGGL(bh3_pairs<T>, dim3(g), dim3(256), 0, s, tr.k64, (const T*)tr.ys.get(), N, tr.mask.get(),
                        tr.node_a.get(), tr.node_b.get(), tr.node_lvl.get(), tr.tile_cnt.get(), tr.part.get());
     TKA_LAUNCH_CHECK();

@@ -37,8 +37,9 @@ struct alignas(32) BhNode3 {
 template <class T>
 struct BhTree3 {
     int N = 0;
-    DevBuf<uint64_t> keys, keys_alt;        // Morton keys, radix-sorted in place (ping-pong)
-    DevBuf<uint32_t> perm, perm_alt, sort_scratch;
+    DevBuf<uint64_t> keys, keys_sorted;      // Morton keys per point; the same in sorted order
+    DevBuf<uint32_t> ktop, ktop_alt, perm, perm_alt, sort_scratch;  // top 24 key bits and points, radix-sorted
+    DevBuf<uint32_t> p2;                     // point at each sorted position
     const uint64_t* k64 = nullptr;          // sorted keys (valid after build)
     const uint32_t* sorted_perm = nullptr;  // point at each sorted position (valid after build)
     DevBuf<T> ys;                           // sorted coordinates, N x 3

@@ -105,6 +105,26 @@ def test_octree_tight_clusters(engine, oracle):
     assert abs(z - rz) / rz < 1e-10
 
 
+@pytest.mark.parametrize("m", [300, 3000])
+def test_octree_long_runs(engine, oracle, m):
+    # m points (and 40 more elsewhere) within ~1e-5 of each other in a box of
+    # width ~9: their keys share the sorted top 24 bits (level-8 cells), so
+    # their full-key order comes from the run sort: a wave's LDS sort for the
+    # 300-point run, an in-place sort in global memory for the 3000-point
+    # one, ranks for the 40
+    g = rng(19)
+    Y = g.normal(size=(12000, 3)) * np.array([3.0, 2.0, 1.5])
+    Y[:m] = g.normal(size=(m, 3)) * 1e-5 + [0.3, -0.2, 0.1]
+    Y[m:m + 40] = g.normal(size=(40, 3)) * 1e-5 + [-1.1, 0.7, 0.4]
+    Y = Y - Y.mean(0)
+    X = mixture(12000, 8, k=6, seed=5)
+    row, col, val = engine.build_p(X, 10.0, dist_dtype="fp32")
+    dY, z = engine.gradient(Y, row, col, val, 0.5, precision="fp64")
+    rdY, rz = oracle.gradient_bh(row, col, val, Y, 0.5)
+    assert rel_l2(dY, rdY) < 1e-8
+    assert abs(z - rz) / rz < 1e-10
+
+
 def test_octree_kl_entry_points(engine, oracle, problem3):
     row, col, val, Y = problem3
     kl = engine.kl(Y, row, col, val, mode="reference", theta=0.5)
