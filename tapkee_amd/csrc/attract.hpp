@@ -74,11 +74,11 @@ TKA_DEV void attract_row(const T* __restrict__ Y, const int* __restrict__ row_pt
 }
 
 
-// Rows [row0, row1) of the attraction of a 2-D embedding, run as extra work
-// by workgroups of 256 threads of another kernel: workgroup wg of nwg takes
-// row groups wg, wg + nwg, ...  Each row is computed exactly as by the
+// Rows [row0, row1) of the attraction of a DIMS-D embedding, run as extra
+// work by workgroups of 256 threads of another kernel: workgroup wg of nwg
+// takes row groups wg, wg + nwg, ...  Each row is computed exactly as by the
 // standalone launch, so the result does not depend on which kernel ran it.
-template <class T>
+template <class T, int DIMS = 2>
 struct AttractSlice {
     const T* Y = nullptr;
     const int* row_ptr = nullptr;
@@ -94,7 +94,7 @@ struct AttractSlice {
         for (int r = row0 + wg * kRowsPerBlock; r < row1; r += nwg * kRowsPerBlock) {
             const int i = r + g;
             if (i < row1)  // whole row groups skip together; group sums stay inside a group
-                attract_row<T, 2>(Y, row_ptr, col, val, exag, i, gl, pos);
+                attract_row<T, DIMS>(Y, row_ptr, col, val, exag, i, gl, pos);
         }
     }
 };

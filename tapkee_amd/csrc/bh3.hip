@@ -189,8 +189,13 @@ template <class T>
 __global__ __launch_bounds__(256) void bh3_pairs(const uint64_t* __restrict__ keys, const T* __restrict__ ys, int N,
                                                  unsigned long long* __restrict__ mask, int* __restrict__ node_a,
                                                  int* __restrict__ node_b, int* __restrict__ node_lvl,
-                                                 uint32_t* __restrict__ tile_cnt, double* __restrict__ part)
+                                                 uint32_t* __restrict__ tile_cnt, double* __restrict__ part,
+                                                 int nmain, AttractSlice<T, 3> side)
 {
+    if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
+        side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
+        return;
+    }
     __shared__ uint64_t win[kPairWin3];
     __shared__ uint32_t own_cnt[2];  // owners with a in tile blockIdx.x - 1, blockIdx.x
     __shared__ double sh[4];
@@ -265,8 +270,12 @@ template <class T>
 __global__ __launch_bounds__(256) void bh3_scan(const unsigned long long* __restrict__ mask, const T* __restrict__ ys,
                                                 int N, const uint32_t* __restrict__ tile_cnt,
                                                 const double* __restrict__ part, uint32_t* __restrict__ pre,
-                                                double* __restrict__ ps)
+                                                double* __restrict__ ps, int nmain, AttractSlice<T, 3> side)
 {
+    if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
+        side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
+        return;
+    }
     __shared__ double sh[4];
     __shared__ double wsum[4][4];
     uint32_t bc = 0;
@@ -314,8 +323,12 @@ __global__ __launch_bounds__(256) void bh3_emit(const unsigned long long* __rest
                                                 const int* __restrict__ node_lvl, const T* __restrict__ ys,
                                                 const uint32_t* __restrict__ perm, int N,
                                                 const uint32_t* __restrict__ pre, const double* __restrict__ ps,
-                                                BhNode3<T>* __restrict__ nodes)
+                                                BhNode3<T>* __restrict__ nodes, int nmain, AttractSlice<T, 3> side)
 {
+    if ((int)blockIdx.x >= nmain) {  // attraction rows riding in this launch
+        side.run((int)blockIdx.x - nmain, (int)gridDim.x - nmain);
+        return;
+    }
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= N) return;
     {  // leaf j
@@ -467,7 +480,7 @@ void BhTree3<T>::alloc(int n, MemBudget* mb)
 }
 
 template <class T>
-void bh3_build(T* Y, int N, BhTree3<T>& tr, hipStream_t s)
+void bh3_build(T* Y, int N, BhTree3<T>& tr, hipStream_t s, const AttractSlice<T, 3>* attract)
 {
     require(N >= 2 && tr.N >= N, TKA_E_INTERNAL, "octree storage too small");
     const int nb = (int)ceil_div(N, kBox3Block);
@@ -490,17 +503,35 @@ void bh3_build(T* Y, int N, BhTree3<T>& tr, hipStream_t s)
     TKA_LAUNCH_CHECK();
     tr.k64 = tr.keys_sorted.get();
     tr.sorted_perm = tr.p2.get();
-    hipLaunchKernelGGL(bh3_pairs<T>, dim3(g), dim3(256), 0, s, tr.k64, (const T*)tr.ys.get(), N, tr.mask.get(),
-                       tr.node_a.get(), tr.node_b.get(), tr.node_lvl.get(), tr.tile_cnt.get(), tr.part.get());
+    // The attraction rows ride in extra workgroups of the node kernels (Y is
+    // centred by the key kernel), split in proportion to the kernels'
+    // durations; the run sort takes none (its LDS would hold them back).
+    constexpr int kW[3] = {10, 5, 5};  // pairs, scan, emit
+    AttractSlice<T, 3> slice[3];
+    int wgs[3] = {0, 0, 0};
+    if (attract) {
+        const int r0 = attract->row0, rows = attract->row1 - attract->row0;
+        int wacc = 0;
+        for (int k = 0; k < 3; k++) {
+            slice[k] = *attract;
+            slice[k].row0 = r0 + (int)((int64_t)rows * wacc / 20);
+            wacc += kW[k];
+            slice[k].row1 = r0 + (int)((int64_t)rows * wacc / 20);
+            wgs[k] = (int)ceil_div((size_t)std::max(0, slice[k].row1 - slice[k].row0), kRowsPerBlock);
+        }
+    }
+    hipLaunchKernelGGL(bh3_pairs<T>, dim3(g + wgs[0]), dim3(256), 0, s, tr.k64, (const T*)tr.ys.get(), N,
+                       tr.mask.get(), tr.node_a.get(), tr.node_b.get(), tr.node_lvl.get(), tr.tile_cnt.get(),
+                       tr.part.get(), (int)g, slice[0]);
     TKA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bh3_scan<T>, dim3(g), dim3(256), 0, s, (const unsigned long long*)tr.mask.get(),
+    hipLaunchKernelGGL(bh3_scan<T>, dim3(g + wgs[1]), dim3(256), 0, s, (const unsigned long long*)tr.mask.get(),
                        (const T*)tr.ys.get(), N, (const uint32_t*)tr.tile_cnt.get(), (const double*)tr.part.get(),
-                       tr.pre.get(), tr.ps.get());
+                       tr.pre.get(), tr.ps.get(), (int)g, slice[1]);
     TKA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(bh3_emit<T>, dim3(g), dim3(256), 0, s, (const unsigned long long*)tr.mask.get(),
+    hipLaunchKernelGGL(bh3_emit<T>, dim3(g + wgs[2]), dim3(256), 0, s, (const unsigned long long*)tr.mask.get(),
                        (const int*)tr.node_a.get(), (const int*)tr.node_b.get(), (const int*)tr.node_lvl.get(),
                        (const T*)tr.ys.get(), tr.sorted_perm, N, (const uint32_t*)tr.pre.get(),
-                       (const double*)tr.ps.get(), tr.nodes.get());
+                       (const double*)tr.ps.get(), tr.nodes.get(), (int)g, slice[2]);
     TKA_LAUNCH_CHECK();
 }
 
@@ -532,8 +563,8 @@ void bh3_inject_sort_fault(BhTree3<T>& tr, hipStream_t s)
 
 template struct BhTree3<float>;
 template struct BhTree3<double>;
-template void bh3_build<float>(float*, int, BhTree3<float>&, hipStream_t);
-template void bh3_build<double>(double*, int, BhTree3<double>&, hipStream_t);
+template void bh3_build<float>(float*, int, BhTree3<float>&, hipStream_t, const AttractSlice<float, 3>*);
+template void bh3_build<double>(double*, int, BhTree3<double>&, hipStream_t, const AttractSlice<double, 3>*);
 template int bh3_repulsion<float>(const BhTree3<float>&, double, float*, double*, unsigned long long*, hipStream_t);
 template int bh3_repulsion<double>(const BhTree3<double>&, double, double*, double*, unsigned long long*,
                                    hipStream_t);

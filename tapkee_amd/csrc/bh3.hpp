@@ -60,8 +60,9 @@ struct BhTree3 {
 
 // Builds the octree of Y (N x 3 row-major, T).  Y is centred in place (its
 // mean removed), as the reference's tree build does on its data copy.
+// With `attract`, its rows run in extra workgroups of the node kernels.
 template <class T>
-void bh3_build(T* Y, int N, BhTree3<T>& tr, hipStream_t s);
+void bh3_build(T* Y, int N, BhTree3<T>& tr, hipStream_t s, const AttractSlice<T, 3>* attract = nullptr);
 
 // Repulsive forces: neg (N x 3) = sum over accepted cells of
 // count*q^2*(y - com) and zpart[block] = partial sums of count*q; returns

@@ -467,9 +467,10 @@ struct Session : tka_session {
     // CUs idle; no second stream, so the overlap does not depend on how the
     // runtime maps streams onto hardware queues (two streams that share a
     // queue run one after the other).
-    AttractSlice<T> attract_job() const
+    template <int DIMS = 2>
+    AttractSlice<T, DIMS> attract_job() const
     {
-        AttractSlice<T> a;
+        AttractSlice<T, DIMS> a;
         a.Y = Ycur;
         a.row_ptr = L.row.get();
         a.col = L.col.get();
@@ -570,12 +571,14 @@ struct Session : tka_session {
                                        L.dense ? L.pos.get() : nullptr, L.zpart.get(), L.work.get(), s);
                 if (profile) timers[kTimExact].end(s);
             } else if (!tree_ready && dims == 3) {
-                // octree: its own box partials, then the attraction alone (on
-                // the Y the build has centred)
+                // octree: its own box partials; the attraction rows (on the Y
+                // the key kernel centres) ride in the node kernels, or, when
+                // profiling, run alone after the build
+                const AttractSlice<T, 3> job3 = attract_job<3>();
                 if (profile) timers[kTimBuild].begin(s);
-                bh3_build<T>(Ycur, N, tree3, s);
+                bh3_build<T>(Ycur, N, tree3, s, profile ? nullptr : &job3);
                 if (profile) timers[kTimBuild].end(s);
-                attraction_alone();
+                if (profile) attraction_alone();
                 if (profile) timers[kTimRepulsion].begin(s);
                 int g3 = 0;
                 for (int r = 0; r < (profile ? kProfileRepeats : 1); r++)
