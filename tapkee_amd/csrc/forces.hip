@@ -17,10 +17,25 @@ TKA_DEV T sgn(T x)
     return x == (T)0 ? (T)0 : (x < (T)0 ? (T)-1 : (T)1);  // tsne.hpp:53-56
 }
 
+template <int kBlock>
 TKA_DEV double z_total(const double* __restrict__ zpart, int nz, double* sh)
 {
+    // the order of a loop over b = threadIdx.x, + kBlock, ..., with kU loads
+    // in flight at a time (a loop that waits on each load makes as many
+    // memory round trips as partials per thread; the block size is a
+    // constant, as reading blockDim would wait on the rows' loads)
+    constexpr int kU = 8;
     double z = 0.0;
-    for (int b = threadIdx.x; b < nz; b += blockDim.x) z += zpart[b];
+    for (int b0 = threadIdx.x; b0 < nz; b0 += kU * kBlock) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int b = b0 + u * kBlock;
+            v[u] = b < nz ? zpart[b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) z += v[u];
+    }
     return block_sum(z, sh);
 }
 
@@ -60,7 +75,7 @@ __global__ __launch_bounds__(256) void tsne_step(const T* __restrict__ Y, T* __r
 {
     __shared__ double sh[4];
     __shared__ double sbox[7][4];
-    const double Z = z_total(zpart, nz, sh);
+    const double Z = z_total<256>(zpart, nz, sh);
     const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
     const int i = blockIdx.x * kRowsPerBlock + g;
     T yi[DIMS], acc[DIMS];
@@ -142,6 +157,7 @@ __global__ __launch_bounds__(256) void attract(const T* __restrict__ Y, const in
 // launch is latency-bound); the key kernel reduces the N/256 box partials.
 constexpr int kUpdBlock = 256;
 
+static_assert(kUpdBlock == 256, "z_total's block size");
 template <class T, int DIMS>
 __global__ __launch_bounds__(kUpdBlock) void tsne_update(const T* __restrict__ Y, T* __restrict__ Ynext,
                                                          T* __restrict__ uY, T* __restrict__ gains,
@@ -155,18 +171,20 @@ __global__ __launch_bounds__(kUpdBlock) void tsne_update(const T* __restrict__ Y
     const int i = blockIdx.x * kUpdBlock + threadIdx.x;
     // The row's operands are requested before the Z reduction, so its loads
     // and theirs are in flight together.
+    // rows past N read row N-1 (unconditional loads, all in flight
+    // together) and are not written
     T rp[DIMS], rn[DIMS], rg[DIMS], ru[DIMS], ry[DIMS];
+    const size_t ic = (size_t)min(i, N - 1);
 #pragma unroll
     for (int d = 0; d < DIMS; d++) {
-        const size_t o = (size_t)i * DIMS + d;
-        const bool in = i < N;
-        rp[d] = in ? pos[o] : (T)0;
-        rn[d] = in ? neg[o] : (T)0;
-        rg[d] = in ? gains[o] : (T)0;
-        ru[d] = in ? uY[o] : (T)0;
-        ry[d] = in ? Y[o] : (T)0;
+        const size_t o = ic * DIMS + d;
+        rp[d] = pos[o];
+        rn[d] = neg[o];
+        rg[d] = gains[o];
+        ru[d] = uY[o];
+        ry[d] = Y[o];
     }
-    const double Z = z_total(zpart, nz, sh);
+    const double Z = z_total<256>(zpart, nz, sh);
     double bx[7] = {0.0, 0.0, 0.0, HUGE_VAL, HUGE_VAL, -HUGE_VAL, -HUGE_VAL};
     if (i < N) {
         T yn[DIMS];
@@ -254,7 +272,7 @@ __global__ __launch_bounds__(256) void grad_combine(const T* __restrict__ Y, con
                                                     T* __restrict__ dC)
 {
     __shared__ double sh[4];
-    const double Z = z_total(zpart, nz, sh);
+    const double Z = z_total<256>(zpart, nz, sh);
     const int g = threadIdx.x / kG, gl = threadIdx.x % kG;
     const int i = blockIdx.x * kRowsPerBlock + g;
     T yi[DIMS], acc[DIMS];
