@@ -492,7 +492,11 @@ struct Session : tka_session {
         if (profile) timers[kTimAttract].end(s);
     }
 
-    void renumber(const uint32_t* perm)
+    // The sort fault word of the session's tree (quadtree or octree).
+    const uint32_t* tree_fault() const { return dims == 3 ? tree3.sort_fault() : tree.sort_fault(); }
+
+    // `fault`: the sort fault word of the tree whose order `perm` is
+    void renumber(const uint32_t* perm, const uint32_t* fault)
     {
         hipStream_t s = this->s();
         MemBudget* m = &mb;
@@ -508,7 +512,7 @@ struct Session : tka_session {
             renumbered = true;
         }
         T* state[3] = {Ycur, L.uY.get(), L.gains.get()};
-        renumber_points_device<T>(N, dims, perm, tree.sort_fault(), state, 3, state_tmp.get(), L.row.get(), L.col.get(), L.val.get(),
+        renumber_points_device<T>(N, dims, perm, fault, state, 3, state_tmp.get(), L.row.get(), L.col.get(), L.val.get(),
                                   row2.get(), col2.get(), val2.get(), orig.get(), orig_tmp.get(), inv.get(),
                                   rn_scratch.get(), s);
         std::swap(L.row, row2);
@@ -574,6 +578,8 @@ struct Session : tka_session {
                 // octree: its own box partials; the attraction rows (on the Y
                 // the key kernel centres) ride in the node kernels, or, when
                 // profiling, run alone after the build
+                if (iter > 0 && iter % kRenumberEvery == 0 && tree3.sorted_perm != nullptr)
+                    renumber(tree3.sorted_perm, tree3.sort_fault());  // order of the previous iteration's octree
                 const AttractSlice<T, 3> job3 = attract_job<3>();
                 if (profile) timers[kTimBuild].begin(s);
                 bh3_build<T>(Ycur, N, tree3, s, profile ? nullptr : &job3);
@@ -590,7 +596,7 @@ struct Session : tka_session {
                 }
             } else if (!tree_ready) {
                 if (iter > 0 && iter % kRenumberEvery == 0 && tree.sorted_perm != nullptr)
-                    renumber(tree.sorted_perm);  // order of the previous iteration's tree
+                    renumber(tree.sorted_perm, tree.sort_fault());  // order of the previous iteration's tree
                 // Y is final after the key kernel: the (HBM-bound) attraction
                 // runs inside the sort and node launches, which occupy few
                 // CUs; beside the traversal it would slow the traversal's
@@ -892,7 +898,7 @@ tka_session* create_session(const SimilarityInput& in, int N, int no_dims, doubl
             DevBuf<double> dist((size_t)N * K, mb), val((size_t)N * K, mb);
             KnnStats ks;
             if (features) {
-                const bool want_order = !exact && no_dims == 2;  // the quadtree loop renumbers points
+                const bool want_order = !exact;  // the Barnes-Hut loops renumber points
                 if (want_order) order.alloc(N, mb);
                 knn_device(X.get(), N, D, K, o.dist_dtype, o.candidates, idx.get(), dist.get(), &ks, mb, s,
                            want_order ? order.get() : nullptr);
@@ -963,10 +969,16 @@ tka_session* create_session(const SimilarityInput& in, int N, int no_dims, doubl
     // takes over at the first periodic renumbering)
     if (o.loop_precision == TKA_FP64) {
         static_cast<Session<double>*>(sess.get())->init(Y0);
-        if (ordered) static_cast<Session<double>*>(sess.get())->renumber(order.get());
+        if (ordered) {
+            auto* S = static_cast<Session<double>*>(sess.get());
+            S->renumber(order.get(), S->tree_fault());
+        }
     } else {
         static_cast<Session<float>*>(sess.get())->init(Y0);
-        if (ordered) static_cast<Session<float>*>(sess.get())->renumber(order.get());
+        if (ordered) {
+            auto* S = static_cast<Session<float>*>(sess.get());
+            S->renumber(order.get(), S->tree_fault());
+        }
     }
     c.stream.sync();
     log_took("Input similarities computation", (now_ms() - t0) * 1e-3, o.verbose != 0);  // tsne.hpp:87

@@ -484,13 +484,14 @@ def test_profiled_interaction_count(engine, dims):
 
 
 # ---------------------------------------------------------------- full run
-def test_renumbered_run_returns_caller_order(engine):
-    # points are renumbered every 50 iterations; embedding() must undo it.
-    # A 1-iteration session has not been renumbered; after 120 iterations
-    # rows must still belong to their points: nearby inputs stay nearby.
+@pytest.mark.parametrize("dims", [2, 3])
+def test_renumbered_run_returns_caller_order(engine, dims):
+    # points are renumbered every 50 iterations (quadtree and octree loops);
+    # embedding() must undo it.  After 120 iterations rows must still belong
+    # to their points: nearby inputs stay nearby.
     X = mixture(2000, 16, k=4)
     labels = np.arange(2000) % 4
-    s = engine.Session(X, perplexity=20.0, theta=0.5, seed=3)
+    s = engine.Session(X, no_dims=dims, perplexity=20.0, theta=0.5, seed=3)
     s.step(120)
     Y = s.embedding()
     s.close()
